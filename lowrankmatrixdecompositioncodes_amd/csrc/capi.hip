@@ -1,0 +1,880 @@
+// capi.hip -- the RSVDPACK C API (rsvdpack.h) and the additive extensions
+// (rsvdpack_ext.h) on top of the device pipeline.
+//
+// Host `mat` / `vec` storage, element access, file I/O, gathers, resizes and
+// other pure data movement stay host code (the structs are host memory by
+// contract).  Every arithmetic routine -- GEMMs, QR, eigen / SVD, norms,
+// Gaussian draws and the L2 decompositions -- runs on the GPU; when no gfx950
+// device is present the call fails (outputs NULL, rsvd_last_error() set) and
+// nothing is computed on the CPU.
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <ctime>
+#include <mutex>
+#include <vector>
+
+#include "pipeline.h"
+#include "rsvdpack.h"
+#include "rsvdpack_ext.h"
+
+using namespace rsvd;
+
+namespace {
+
+std::mutex g_cfg_mu;
+bool g_seed_fixed = false;
+unsigned long long g_seed = 0;
+std::vector<double> g_omega;  // injected Omega (host copy)
+
+unsigned long long draw_seed() {
+  std::lock_guard<std::mutex> lk(g_cfg_mu);
+  return g_seed_fixed ? g_seed : (unsigned long long)time(NULL);
+}
+
+// Omega for a draw of `count` elements: injected host data uploaded to `buf`,
+// or the seeded stream.
+int host_omega(Context* c, int64_t n, int64_t cols, DBuf<double>& buf, OmegaSrc* om) {
+  std::vector<double> inj;
+  {
+    std::lock_guard<std::mutex> lk(g_cfg_mu);
+    if (!g_omega.empty()) inj = g_omega;
+  }
+  if (!inj.empty()) {
+    const size_t need = (size_t)n * (size_t)cols;
+    if (inj.size() < need)
+      return set_error("injected Omega holds %zu values, %zu needed", inj.size(), need);
+    RSVD_TRY(buf.alloc(need));
+    RSVD_HIP(hipMemcpyAsync(buf.p, inj.data(), need * sizeof(double), hipMemcpyHostToDevice,
+                            c->stream));
+    RSVD_HIP(hipStreamSynchronize(c->stream));
+    om->dev = buf.p;
+    om->ld = n;
+    return 0;
+  }
+  om->seed = draw_seed();
+  om->dev = nullptr;
+  return 0;
+}
+
+int upload(Context* c, const mat* M, DBuf<double>& d) {
+  const size_t N = (size_t)M->nrows * (size_t)M->ncols;
+  RSVD_TRY(d.alloc(N));
+  if (N)
+    RSVD_HIP(hipMemcpyAsync(d.p, M->d, N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  return 0;
+}
+
+int download(Context* c, const double* src, int64_t rows, int64_t cols, int64_t ld, double* dst) {
+  if (rows * cols == 0) return 0;
+  if (ld == rows) {
+    RSVD_HIP(hipMemcpyAsync(dst, src, sizeof(double) * rows * cols, hipMemcpyDeviceToHost,
+                            c->stream));
+  } else {
+    RSVD_HIP(hipMemcpy2DAsync(dst, sizeof(double) * rows, src, sizeof(double) * ld,
+                              sizeof(double) * rows, cols, hipMemcpyDeviceToHost, c->stream));
+  }
+  return 0;
+}
+
+int stream_sync(Context* c) {
+  RSVD_HIP(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+bool valid(const mat* M) { return M && M->nrows >= 0 && M->ncols >= 0 && (M->d || !M->nrows || !M->ncols); }
+
+mat* diag_mat(const std::vector<double>& s, int k) {
+  mat* S = matrix_new(k, k);
+  for (int i = 0; i < k; ++i) S->d[(size_t)i * k + i] = s[i];
+  return S;
+}
+
+struct Api {
+  Context* c = nullptr;
+  std::unique_lock<std::recursive_mutex> lk;
+  int rc = 0;
+  Api() {
+    clear_error();
+    rc = ctx_get(&c);
+    if (!rc) lk = std::unique_lock<std::recursive_mutex>(c->mu);
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+// ======================================================================
+// L1 storage (host)
+// ======================================================================
+mat* matrix_new(int nrows, int ncols) {
+  mat* M = (mat*)malloc(sizeof(mat));
+  if (!M) return nullptr;
+  size_t N = (size_t)(nrows > 0 ? nrows : 0) * (size_t)(ncols > 0 ? ncols : 0);
+  M->d = (double*)calloc(N ? N : 1, sizeof(double));
+  M->nrows = nrows;
+  M->ncols = ncols;
+  return M;
+}
+
+vec* vector_new(int nrows) {
+  vec* v = (vec*)malloc(sizeof(vec));
+  if (!v) return nullptr;
+  v->d = (double*)calloc(nrows > 0 ? (size_t)nrows : 1, sizeof(double));
+  v->nrows = nrows;
+  return v;
+}
+
+void matrix_delete(mat* M) {
+  if (!M) return;
+  free(M->d);
+  free(M);
+}
+
+void vector_delete(vec* v) {
+  if (!v) return;
+  free(v->d);
+  free(v);
+}
+
+void matrix_set_element(mat* M, int row_num, int col_num, double val) {
+  M->d[(size_t)col_num * M->nrows + row_num] = val;
+}
+double matrix_get_element(mat* M, int row_num, int col_num) {
+  return M->d[(size_t)col_num * M->nrows + row_num];
+}
+void vector_set_element(vec* v, int row_num, double val) { v->d[row_num] = val; }
+double vector_get_element(vec* v, int row_num) { return v->d[row_num]; }
+
+mat* matrix_load_from_binary_file(char* fname) {
+  clear_error();
+  FILE* fp = fopen(fname, "rb");
+  if (!fp) { set_error("cannot open %s", fname); return nullptr; }
+  int m = 0, n = 0;
+  if (fread(&m, sizeof(int), 1, fp) != 1 || fread(&n, sizeof(int), 1, fp) != 1 || m < 0 || n < 0) {
+    fclose(fp);
+    set_error("%s: bad header", fname);
+    return nullptr;
+  }
+  mat* M = matrix_new(m, n);
+  std::vector<double> row((size_t)(n > 0 ? n : 1));
+  for (int i = 0; i < m; ++i) {
+    if (fread(row.data(), sizeof(double), (size_t)n, fp) != (size_t)n) {
+      fclose(fp);
+      matrix_delete(M);
+      set_error("%s: truncated at row %d", fname, i);
+      return nullptr;
+    }
+    for (int j = 0; j < n; ++j) M->d[(size_t)j * m + i] = row[j];
+  }
+  fclose(fp);
+  return M;
+}
+
+void matrix_write_to_binary_file(mat* M, char* fname) {
+  clear_error();
+  FILE* fp = fopen(fname, "wb");
+  if (!fp) { set_error("cannot open %s", fname); return; }
+  int m = M->nrows, n = M->ncols;
+  fwrite(&m, sizeof(int), 1, fp);
+  fwrite(&n, sizeof(int), 1, fp);
+  std::vector<double> row((size_t)(n > 0 ? n : 1));
+  for (int i = 0; i < m; ++i) {
+    for (int j = 0; j < n; ++j) row[j] = M->d[(size_t)j * m + i];
+    fwrite(row.data(), sizeof(double), (size_t)n, fp);
+  }
+  fclose(fp);
+}
+
+void matrix_print(mat* M) {
+  for (int i = 0; i < M->nrows; ++i) {
+    for (int j = 0; j < M->ncols; ++j) printf("%f  ", matrix_get_element(M, i, j));
+    printf("\n");
+  }
+}
+
+void vector_print(vec* v) {
+  for (int i = 0; i < v->nrows; ++i) printf("%f\n", v->d[i]);
+}
+
+void matrix_copy(mat* D, mat* S) {
+  memcpy(D->d, S->d, sizeof(double) * (size_t)S->nrows * S->ncols);
+}
+
+void matrix_build_transpose(mat* Mt, mat* M) {
+  for (int j = 0; j < M->ncols; ++j)
+    for (int i = 0; i < M->nrows; ++i) Mt->d[(size_t)i * Mt->nrows + j] = M->d[(size_t)j * M->nrows + i];
+}
+
+void matrix_sub(mat* A, mat* B) {
+  const size_t N = (size_t)A->nrows * A->ncols;
+  for (size_t i = 0; i < N; ++i) A->d[i] -= B->d[i];
+}
+
+void matrix_get_selected_columns(mat* M, int* inds, mat* Mc) {
+  for (int j = 0; j < Mc->ncols; ++j)
+    memcpy(Mc->d + (size_t)j * M->nrows, M->d + (size_t)inds[j] * M->nrows,
+           sizeof(double) * M->nrows);
+}
+void matrix_set_selected_columns(mat* M, int* inds, mat* Mc) {
+  for (int j = 0; j < Mc->ncols; ++j)
+    memcpy(M->d + (size_t)inds[j] * M->nrows, Mc->d + (size_t)j * M->nrows,
+           sizeof(double) * M->nrows);
+}
+void matrix_get_selected_rows(mat* M, int* inds, mat* Mr) {
+  for (int j = 0; j < M->ncols; ++j)
+    for (int i = 0; i < Mr->nrows; ++i)
+      Mr->d[(size_t)j * Mr->nrows + i] = M->d[(size_t)j * M->nrows + inds[i]];
+}
+void matrix_set_selected_rows(mat* M, int* inds, mat* Mr) {
+  for (int j = 0; j < M->ncols; ++j)
+    for (int i = 0; i < Mr->nrows; ++i)
+      M->d[(size_t)j * M->nrows + inds[i]] = Mr->d[(size_t)j * Mr->nrows + i];
+}
+
+void matrix_copy_symmetric(mat* S, mat* M) {
+  for (int j = 0; j < M->ncols; ++j)
+    for (int i = 0; i <= j && i < M->nrows; ++i)
+      S->d[(size_t)j * S->nrows + i] = M->d[(size_t)j * M->nrows + i];
+}
+
+void initialize_diagonal_matrix(mat* D, vec* data) {
+  for (int i = 0; i < D->nrows; ++i) D->d[(size_t)i * D->nrows + i] = data->d[i];
+}
+
+void invert_diagonal_matrix(mat* Dinv, mat* D) {
+  for (int i = 0; i < D->nrows; ++i)
+    Dinv->d[(size_t)i * Dinv->nrows + i] = 1.0 / D->d[(size_t)i * D->nrows + i];
+}
+
+static void resize_cols(mat** M, int k, int from_end) {
+  mat* R = matrix_new((*M)->nrows, k);
+  const int off = from_end ? (*M)->ncols - k : 0;
+  memcpy(R->d, (*M)->d + (size_t)off * (*M)->nrows, sizeof(double) * (size_t)k * (*M)->nrows);
+  matrix_delete(*M);
+  *M = R;
+}
+static void resize_rows(mat** M, int k, int from_end) {
+  mat* R = matrix_new(k, (*M)->ncols);
+  const int off = from_end ? (*M)->nrows - k : 0;
+  for (int j = 0; j < R->ncols; ++j)
+    memcpy(R->d + (size_t)j * k, (*M)->d + (size_t)j * (*M)->nrows + off, sizeof(double) * k);
+  matrix_delete(*M);
+  *M = R;
+}
+void resize_matrix_by_columns(mat** M, int k) { resize_cols(M, k, 0); }
+void resize_matrix_by_columns_from_end(mat** M, int k) { resize_cols(M, k, 1); }
+void resize_matrix_by_rows(mat** M, int k) { resize_rows(M, k, 0); }
+void resize_matrix_by_rows_from_end(mat** M, int k) { resize_rows(M, k, 1); }
+
+double get_seconds_frac(struct timeval start_timeval, struct timeval end_timeval) {
+  long secs_used = (end_timeval.tv_sec - start_timeval.tv_sec);
+  long micros_used = ((secs_used * 1000000) + end_timeval.tv_usec) - (start_timeval.tv_usec);
+  return (micros_used / 1e6);
+}
+
+// ======================================================================
+// L1 compute (GPU)
+// ======================================================================
+void initialize_random_matrix(mat* M) {
+  Api api;
+  if (api.rc || !valid(M)) { if (!api.rc) set_error("initialize_random_matrix: bad matrix"); return; }
+  const int64_t m = M->nrows, n = M->ncols;
+  DBuf<double> d, ob;
+  OmegaSrc om;
+  if (host_omega(api.c, m, n, ob, &om)) return;
+  if (d.alloc((size_t)(m * n))) return;
+  if (om.dev) {
+    if (copy2d_f64(api.c, m, n, om.dev, om.ld, d.p, m)) return;
+  } else if (gaussian_f64(api.c, om.seed, m, n, 0, m, d.p, m)) {
+    return;
+  }
+  if (download(api.c, d.p, m, n, m, M->d)) return;
+  stream_sync(api.c);
+}
+
+static void gemm_host(char ta, char tb, mat* A, mat* B, mat* C, int64_t M, int64_t N, int64_t K) {
+  Api api;
+  if (api.rc) return;
+  if (!valid(A) || !valid(B) || !valid(C)) { set_error("gemm: bad matrix"); return; }
+  if (C->nrows != M || C->ncols != N) {
+    set_error("gemm: output is %dx%d, product is %lldx%lld", C->nrows, C->ncols, (long long)M,
+              (long long)N);
+    return;
+  }
+  DBuf<double> dA, dB, dC;
+  if (upload(api.c, A, dA) || upload(api.c, B, dB) || dC.alloc((size_t)(M * N))) return;
+  if (gemm_f64(api.c, ta, tb, M, N, K, 1.0, dA.p, A->nrows > 0 ? A->nrows : 1, dB.p,
+               B->nrows > 0 ? B->nrows : 1, 0.0, dC.p, M > 0 ? M : 1))
+    return;
+  if (download(api.c, dC.p, M, N, M, C->d)) return;
+  stream_sync(api.c);
+}
+
+void matrix_matrix_mult(mat* A, mat* B, mat* C) {
+  if (A && B && A->ncols != B->nrows) { set_error("matrix_matrix_mult: inner dims differ"); return; }
+  gemm_host('N', 'N', A, B, C, A->nrows, B->ncols, A->ncols);
+}
+void matrix_transpose_matrix_mult(mat* A, mat* B, mat* C) {
+  if (A && B && A->nrows != B->nrows) { set_error("matrix_transpose_matrix_mult: inner dims differ"); return; }
+  gemm_host('T', 'N', A, B, C, A->ncols, B->ncols, A->nrows);
+}
+void matrix_matrix_transpose_mult(mat* A, mat* B, mat* C) {
+  if (A && B && A->ncols != B->ncols) { set_error("matrix_matrix_transpose_mult: inner dims differ"); return; }
+  gemm_host('N', 'T', A, B, C, A->nrows, B->nrows, A->ncols);
+}
+
+static double sumsq_host(mat* A, mat* B) {
+  Api api;
+  if (api.rc) return NAN;
+  DBuf<double> dA, dB, out;
+  double h = NAN;
+  if (upload(api.c, A, dA) || out.alloc(1)) return NAN;
+  if (B) {
+    if (upload(api.c, B, dB)) return NAN;
+    if (sumsq_diff_f64(api.c, A->nrows, A->ncols, dA.p, A->nrows > 0 ? A->nrows : 1, dB.p,
+                       B->nrows > 0 ? B->nrows : 1, out.p))
+      return NAN;
+  } else if (sumsq_f64(api.c, A->nrows, A->ncols, dA.p, A->nrows > 0 ? A->nrows : 1, out.p)) {
+    return NAN;
+  }
+  if (hipMemcpyAsync(&h, out.p, sizeof h, hipMemcpyDeviceToHost, api.c->stream) != hipSuccess)
+    return NAN;
+  if (stream_sync(api.c)) return NAN;
+  return h;
+}
+
+double get_matrix_frobenius_norm(mat* M) {
+  if (!valid(M)) { set_error("get_matrix_frobenius_norm: bad matrix"); return NAN; }
+  return sqrt(sumsq_host(M, nullptr));
+}
+
+double get_percent_error_between_two_mats(mat* A, mat* B) {
+  if (!valid(A) || !valid(B) || A->nrows != B->nrows || A->ncols != B->ncols) {
+    set_error("get_percent_error_between_two_mats: shape mismatch");
+    return NAN;
+  }
+  const double d = sqrt(sumsq_host(A, B));
+  const double a = sqrt(sumsq_host(A, nullptr));
+  return 100.0 * d / a;
+}
+
+void compute_evals_and_evecs_of_symm_matrix(mat* S, vec* evals) {
+  Api api;
+  if (api.rc) return;
+  if (!valid(S) || S->nrows != S->ncols || !evals || evals->nrows < S->nrows) {
+    set_error("compute_evals_and_evecs_of_symm_matrix: bad arguments");
+    return;
+  }
+  Context* c = api.c;
+  const int64_t n = S->nrows;
+  if (n == 0) return;
+  DBuf<double> G, G0, V, lam, nrm;
+  if (upload(c, S, G) || G0.alloc((size_t)(n * n)) || V.alloc((size_t)(n * n)) ||
+      lam.alloc((size_t)n) || nrm.alloc(1))
+    return;
+  // symmetric from the upper triangle, shifted to be positive semidefinite so
+  // the one-sided Jacobi singular vectors are eigenvectors
+  if (symmetrize_upper_f64(c, n, G.p) || copy2d_f64(c, n, n, G.p, n, G0.p, n)) return;
+  double ss = 0;
+  if (sumsq_f64(c, n, n, G.p, n, nrm.p)) return;
+  if (hipMemcpyAsync(&ss, nrm.p, sizeof ss, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+      stream_sync(c))
+    return;
+  const double shift = sqrt(ss);
+  if (add_diag_f64(c, n, shift, G.p, n)) return;
+  int sw = 0;
+  if (jacobi_f64(c, n, n, G.p, n, V.p, n, 60, &sw)) return;
+  // eigenvalues as Rayleigh quotients v^T G v of the unshifted matrix
+  if (gemm_f64(c, 'N', 'N', n, n, n, 1.0, G0.p, n, V.p, n, 0.0, G.p, n)) return;
+  if (coldot_f64(c, n, n, V.p, n, G.p, n, lam.p)) return;
+  if (sort_pairs_f64(c, n, lam.p, 0, V.p, n, n, nullptr, 0, 0)) return;
+  if (download(c, V.p, n, n, n, S->d) || download(c, lam.p, n, 1, n, evals->d)) return;
+  stream_sync(c);
+}
+
+static void qr_host(mat* M, mat* Q, mat* R) {
+  Api api;
+  if (api.rc) return;
+  if (!valid(M) || !valid(Q) || Q->nrows != M->nrows || Q->ncols != M->ncols) {
+    set_error("QR: Q must have the shape of M");
+    return;
+  }
+  const int64_t m = M->nrows, n = M->ncols;
+  if (m < n) { set_error("QR: m=%lld < n=%lld is not supported", (long long)m, (long long)n); return; }
+  if (R && (R->nrows < n || R->ncols < n)) { set_error("QR: R must be at least %lldx%lld", (long long)n, (long long)n); return; }
+  DBuf<double> Y, dR;
+  if (upload(api.c, M, Y)) return;
+  if (R && dR.alloc((size_t)(n * n))) return;
+  int fb = 0;
+  if (orth_f64(api.c, m, n, Y.p, m, R ? dR.p : nullptr, &fb)) return;
+  if (download(api.c, Y.p, m, n, m, Q->d)) return;
+  if (R) {
+    if (R->nrows == n) {
+      if (download(api.c, dR.p, n, n, n, R->d)) return;
+    } else {
+      if (hipMemcpy2DAsync(R->d, sizeof(double) * R->nrows, dR.p, sizeof(double) * n,
+                           sizeof(double) * n, n, hipMemcpyDeviceToHost,
+                           api.c->stream) != hipSuccess) {
+        set_error("QR: R download failed");
+        return;
+      }
+    }
+  }
+  stream_sync(api.c);
+}
+
+void compact_QR_factorization(mat* M, mat* Q, mat* R) { qr_host(M, Q, R); }
+void QR_factorization_getQ(mat* M, mat* Q) { qr_host(M, Q, nullptr); }
+
+void singular_value_decomposition(mat* M, mat* U, mat* S, mat* Vt) {
+  Api api;
+  if (api.rc) return;
+  if (!valid(M) || !valid(U) || !valid(S) || !valid(Vt)) { set_error("SVD: bad matrix"); return; }
+  Context* c = api.c;
+  const int64_t m = M->nrows, n = M->ncols, k = m < n ? m : n;
+  if (k == 0) return;
+  const bool tall = m >= n;
+  // one-sided Jacobi on X (L x k) with L = max(m,n): X = M (tall) or M^T (wide)
+  const int64_t L = tall ? m : n;
+  DBuf<double> dM, X, V, sig;
+  if (upload(c, M, dM) || X.alloc((size_t)(L * k)) || V.alloc((size_t)(k * k)) || sig.alloc((size_t)k))
+    return;
+  if (tall) {
+    if (copy2d_f64(c, m, n, dM.p, m, X.p, m)) return;
+  } else if (transpose_f64(c, m, n, dM.p, m, X.p, n)) {
+    return;
+  }
+  int sw = 0;
+  if (jacobi_f64(c, L, k, X.p, L, V.p, k, 60, &sw)) return;
+  if (colnorms_f64(c, L, k, X.p, L, sig.p, 1)) return;
+  if (sort_pairs_f64(c, k, sig.p, 1, X.p, L, L, V.p, k, k)) return;
+  // tall: M = X_n Sigma V^T -> U = X_n (m x k), Vt = V^T (k x n)
+  // wide: M^T = X_n Sigma V^T -> U = V (m x k), Vt = X_n^T (k x n)
+  DBuf<double> T;
+  if (T.alloc((size_t)(k * (tall ? n : L)))) return;
+  if (tall) {
+    if (U->nrows != m || U->ncols < k || Vt->nrows < k || Vt->ncols != n) { set_error("SVD: output shapes"); return; }
+    if (download(c, X.p, m, k, m, U->d)) return;
+    if (transpose_f64(c, k, k, V.p, k, T.p, k)) return;
+    if (hipMemcpy2DAsync(Vt->d, sizeof(double) * Vt->nrows, T.p, sizeof(double) * k,
+                         sizeof(double) * k, n, hipMemcpyDeviceToHost, c->stream) != hipSuccess) {
+      set_error("SVD: Vt download failed");
+      return;
+    }
+  } else {
+    if (U->nrows != m || U->ncols < k || Vt->nrows < k || Vt->ncols != n) { set_error("SVD: output shapes"); return; }
+    if (download(c, V.p, m, k, k, U->d)) return;
+    if (transpose_f64(c, n, k, X.p, n, T.p, k)) return;
+    if (hipMemcpy2DAsync(Vt->d, sizeof(double) * Vt->nrows, T.p, sizeof(double) * k,
+                         sizeof(double) * k, n, hipMemcpyDeviceToHost, c->stream) != hipSuccess) {
+      set_error("SVD: Vt download failed");
+      return;
+    }
+  }
+  std::vector<double> s((size_t)k);
+  if (download(c, sig.p, k, 1, k, s.data()) || stream_sync(c)) return;
+  for (int64_t i = 0; i < k && i < S->nrows && i < S->ncols; ++i) S->d[(size_t)i * S->nrows + i] = s[i];
+}
+
+void form_svd_product_matrix(mat* U, mat* S, mat* V, mat* P) {
+  Api api;
+  if (api.rc) return;
+  if (!valid(U) || !valid(S) || !valid(V) || !valid(P)) { set_error("form_svd_product_matrix: bad matrix"); return; }
+  Context* c = api.c;
+  const int64_t m = P->nrows, n = P->ncols, k = S->nrows;
+  if (U->nrows != m || U->ncols != k || V->nrows != n || V->ncols != S->ncols || S->ncols != k) {
+    set_error("form_svd_product_matrix: shape mismatch");
+    return;
+  }
+  DBuf<double> dU, dS, dV, SVt, dP;
+  if (upload(c, U, dU) || upload(c, S, dS) || upload(c, V, dV) || SVt.alloc((size_t)(k * n)) ||
+      dP.alloc((size_t)(m * n)))
+    return;
+  if (gemm_f64(c, 'N', 'T', k, n, k, 1.0, dS.p, k > 0 ? k : 1, dV.p, n > 0 ? n : 1, 0.0, SVt.p, k > 0 ? k : 1)) return;
+  if (gemm_f64(c, 'N', 'N', m, n, k, 1.0, dU.p, m > 0 ? m : 1, SVt.p, k > 0 ? k : 1, 0.0, dP.p, m > 0 ? m : 1)) return;
+  if (download(c, dP.p, m, n, m, P->d)) return;
+  stream_sync(c);
+}
+
+// ======================================================================
+// L2 decompositions
+// ======================================================================
+void low_rank_svd_rand_decomp_fixed_rank(mat* M, int k, int p, int vnum, int q, int s, int* frank,
+                                         mat** U, mat** S, mat** V) {
+  (void)frank;  // not written by the reference either
+  if (U) *U = nullptr;
+  if (S) *S = nullptr;
+  if (V) *V = nullptr;
+  Api api;
+  if (api.rc) return;
+  if (!valid(M) || !U || !S || !V) { set_error("low_rank_svd_rand_decomp_fixed_rank: bad arguments"); return; }
+  Context* c = api.c;
+  const int64_t m = M->nrows, n = M->ncols;
+  if (k <= 0 || p < 0 || (int64_t)k + p > (m < n ? m : n)) {
+    set_error("low_rank_svd_rand_decomp_fixed_rank: need 0 < k, 0 <= p, k+p <= min(m,n) "
+              "(k=%d p=%d m=%lld n=%lld)", k, p, (long long)m, (long long)n);
+    return;
+  }
+  DBuf<double> dA, ob, dU, dS, dV;
+  OmegaSrc om;
+  if (upload(c, M, dA) || host_omega(c, n, (int64_t)k + p, ob, &om)) return;
+  if (dU.alloc((size_t)(m * k)) || dS.alloc((size_t)k) || dV.alloc((size_t)(n * k))) return;
+  SvdInfo info{};
+  if (fixed_rank_f64(c, dA.p, m, n, m, k, p, vnum, q, s, om, dU.p, m, dS.p, dV.p, n, &info)) return;
+  mat* Uh = matrix_new((int)m, k);
+  mat* Vh = matrix_new((int)n, k);
+  std::vector<double> sv((size_t)k);
+  if (download(c, dU.p, m, k, m, Uh->d) || download(c, dV.p, n, k, n, Vh->d) ||
+      download(c, dS.p, k, 1, k, sv.data()) || stream_sync(c)) {
+    matrix_delete(Uh);
+    matrix_delete(Vh);
+    return;
+  }
+  *U = Uh;
+  *S = diag_mat(sv, k);
+  *V = Vh;
+}
+
+void low_rank_svd_blockrand_decomp_fixed_rank_or_prec(mat* M, int k, int p, double TOL, int vnum,
+                                                      int kstep, int q, int s, int* frank,
+                                                      mat** U, mat** S, mat** V) {
+  if (U) *U = nullptr;
+  if (S) *S = nullptr;
+  if (V) *V = nullptr;
+  Api api;
+  if (api.rc) return;
+  if (!valid(M) || !U || !S || !V || !frank) {
+    set_error("low_rank_svd_blockrand_decomp_fixed_rank_or_prec: bad arguments");
+    return;
+  }
+  Context* c = api.c;
+  const int64_t m = M->nrows, n = M->ncols;
+  int p_eff = 0, nstep = 0, rank_mode = 0;
+  if (blockrand_plan(m, n, k, p, kstep, &p_eff, &nstep, &rank_mode)) return;
+  int ks = kstep, ns = nstep, tm = 0;
+  if (qb_adjust(m, n, &ks, &ns, 1, &tm)) return;
+  DBuf<double> dA, ob;
+  OmegaSrc om;
+  if (upload(c, M, dA) || host_omega(c, n, (int64_t)ks * ns, ob, &om)) return;
+  BlockrandOut out;
+  if (blockrand_f64(c, dA.p, m, n, m, k, p, TOL, vnum, kstep, q, s, om, &out)) return;
+  const int fr = out.frank;
+  mat* Uh = matrix_new((int)m, fr);
+  mat* Vh = matrix_new((int)n, fr);
+  std::vector<double> sv((size_t)fr);
+  if (download(c, out.U.p, m, fr, m, Uh->d) || download(c, out.V.p, n, fr, n, Vh->d) ||
+      download(c, out.S.p, fr, 1, fr, sv.data()) || stream_sync(c)) {
+    matrix_delete(Uh);
+    matrix_delete(Vh);
+    return;
+  }
+  *frank = fr;
+  *U = Uh;
+  *S = diag_mat(sv, fr);
+  *V = Vh;
+}
+
+static void qb_host(mat* M, int kstep, int nstep, double TOL, int q, int s, int* frank, mat** Q,
+                    mat** B, int is_new) {
+  if (Q) *Q = nullptr;
+  if (B) *B = nullptr;
+  Api api;
+  if (api.rc) return;
+  if (!valid(M) || !Q || !B) { set_error("randQB: bad arguments"); return; }
+  Context* c = api.c;
+  const int64_t m = M->nrows, n = M->ncols;
+  int ks = kstep, ns = nstep, tm = 0;
+  if (qb_adjust(m, n, &ks, &ns, is_new, &tm)) return;
+  const int64_t l = (int64_t)ks * ns;
+  DBuf<double> dA, ob, dQ, dB;
+  OmegaSrc om;
+  if (upload(c, M, dA) || host_omega(c, n, l, ob, &om)) return;
+  if (dQ.alloc((size_t)(m * l)) || dB.alloc((size_t)(l * n))) return;
+  int fr = 0, la = 0;
+  QbInfo qi{};
+  if (qb_blocked_f64(c, dA.p, m, n, m, kstep, nstep, TOL, q, s, is_new, om, dQ.p, m, dB.p, l, &fr,
+                     &la, &qi))
+    return;
+  const int64_t cols = tm ? fr : l;  // TOL mode resizes Q, B to frank (rra.c:1781-1788)
+  mat* Qh = matrix_new((int)m, (int)cols);
+  mat* Bh = matrix_new((int)cols, (int)n);
+  if (download(c, dQ.p, m, cols, m, Qh->d) || download(c, dB.p, cols, n, l, Bh->d) || stream_sync(c)) {
+    matrix_delete(Qh);
+    matrix_delete(Bh);
+    return;
+  }
+  if (frank) *frank = fr;
+  *Q = Qh;
+  *B = Bh;
+}
+
+void randQB_pb(mat* M, int kstep, int nstep, int q, int s, mat** Q, mat** B) {
+  qb_host(M, kstep, nstep, 0.0, q, s, nullptr, Q, B, 0);
+}
+
+void randQB_pb_new(mat* M, int kstep, int nstep, double TOL, int q, int s, int* frank, mat** Q,
+                   mat** B) {
+  qb_host(M, kstep, nstep, TOL, q, s, frank, Q, B, 1);
+}
+
+static void report(mat* M, mat* P) {
+  printf("norm(M,fro) = %f\n", get_matrix_frobenius_norm(M));
+  printf("norm(P,fro) = %f\n", get_matrix_frobenius_norm(P));
+  printf("percent error = %f\n", get_percent_error_between_two_mats(M, P));
+}
+
+void use_low_rank_svd_for_approximation(mat* M, mat* U, mat* S, mat* V) {
+  mat* P = matrix_new(M->nrows, M->ncols);
+  form_svd_product_matrix(U, S, V, P);
+  report(M, P);
+  matrix_delete(P);
+}
+
+void use_QB_decomp_for_approximation(mat* M, mat* Q, mat* B) {
+  mat* P = matrix_new(M->nrows, M->ncols);
+  matrix_matrix_mult(Q, B, P);
+  report(M, P);
+  matrix_delete(P);
+}
+
+// ======================================================================
+// extensions
+// ======================================================================
+const char* rsvd_last_error(void) { return last_error(); }
+const char* rsvd_version(void) { return "rsvdpack-mi355x 0.1 (gfx950)"; }
+
+int rsvd_init(void) {
+  Api api;
+  return api.rc;
+}
+
+void rsvd_set_seed(unsigned long long seed) {
+  std::lock_guard<std::mutex> lk(g_cfg_mu);
+  g_seed = seed;
+  g_seed_fixed = true;
+}
+void rsvd_use_time_seed(void) {
+  std::lock_guard<std::mutex> lk(g_cfg_mu);
+  g_seed_fixed = false;
+}
+int rsvd_set_omega(const double* host, long long count) {
+  std::lock_guard<std::mutex> lk(g_cfg_mu);
+  if (!host || count <= 0) {
+    g_omega.clear();
+    return 0;
+  }
+  g_omega.assign(host, host + count);
+  return 0;
+}
+
+int rsvd_gaussian(unsigned long long seed, long long offset, long long count, double* host_out) {
+  Api api;
+  if (api.rc) return api.rc;
+  if (count <= 0) return 0;
+  DBuf<double> d;
+  RSVD_TRY(d.alloc((size_t)count));
+  // a 1 x count slice of the stream starting at `offset`: one row, the
+  // element index of column j is offset + j
+  RSVD_TRY(gaussian_f64(api.c, seed, 1, count, offset, 1, d.p, 1));
+  RSVD_TRY(download(api.c, d.p, count, 1, count, host_out));
+  return stream_sync(api.c);
+}
+
+void* rsvd_dev_alloc(size_t bytes) {
+  Api api;
+  if (api.rc) return nullptr;
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, bytes ? bytes : 1);
+  if (e != hipSuccess) {
+    set_error("hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    return nullptr;
+  }
+  return p;
+}
+void rsvd_dev_free(void* p) {
+  if (!p) return;
+  Api api;
+  if (api.rc) return;
+  (void)hipStreamSynchronize(api.c->stream);
+  (void)hipFree(p);
+}
+int rsvd_memcpy_h2d(void* dst, const void* src, size_t bytes) {
+  Api api;
+  if (api.rc) return api.rc;
+  RSVD_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, api.c->stream));
+  return stream_sync(api.c);
+}
+int rsvd_memcpy_d2h(void* dst, const void* src, size_t bytes) {
+  Api api;
+  if (api.rc) return api.rc;
+  RSVD_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, api.c->stream));
+  return stream_sync(api.c);
+}
+int rsvd_memset_dev(void* dst, int value, size_t bytes) {
+  Api api;
+  if (api.rc) return api.rc;
+  RSVD_HIP(hipMemsetAsync(dst, value, bytes, api.c->stream));
+  return 0;
+}
+int rsvd_dev_gaussian(unsigned long long seed, long long m, long long n, double* dst, long long ld) {
+  Api api;
+  if (api.rc) return api.rc;
+  return gaussian_f64(api.c, seed, m, n, 0, m, dst, ld);
+}
+int rsvd_sync(void) {
+  Api api;
+  if (api.rc) return api.rc;
+  return stream_sync(api.c);
+}
+void rsvd_release_cache(void) {
+  Api api;
+  if (api.rc) return;
+  pool_trim();
+}
+
+int rsvd_dev_fixed_rank(const double* dA, long long m, long long n, long long lda, int k, int p,
+                        int vnum, int q, int s, unsigned long long seed, const double* dOmega,
+                        double* dU, long long ldu, double* dS, double* dV, long long ldv) {
+  Api api;
+  if (api.rc) return api.rc;
+  OmegaSrc om;
+  om.seed = seed;
+  om.dev = dOmega;
+  om.ld = n;
+  return fixed_rank_f64(api.c, dA, m, n, lda, k, p, vnum, q, s, om, dU, ldu, dS, dV, ldv, nullptr);
+}
+
+long long rsvd_qb_max_cols(long long m, long long n, int kstep, int nstep, int is_new) {
+  int ks = kstep, ns = nstep, tm = 0;
+  if (qb_adjust(m, n, &ks, &ns, is_new, &tm)) return -1;
+  return (long long)ks * ns;
+}
+
+int rsvd_dev_randqb(const double* dA, long long m, long long n, long long lda, int kstep, int nstep,
+                    double TOL, int q, int s, int is_new, unsigned long long seed, const double* dRN,
+                    double* dQ, long long ldq, double* dB, long long ldb, int* frank,
+                    double* final_norm) {
+  Api api;
+  if (api.rc) return api.rc;
+  OmegaSrc om;
+  om.seed = seed;
+  om.dev = dRN;
+  om.ld = n;
+  int fr = 0, la = 0;
+  QbInfo qi{};
+  RSVD_TRY(qb_blocked_f64(api.c, dA, m, n, lda, kstep, nstep, TOL, q, s, is_new, om, dQ, ldq, dB,
+                          ldb, &fr, &la, &qi));
+  if (frank) *frank = fr;
+  if (final_norm) *final_norm = qi.last_norm;
+  return stream_sync(api.c);
+}
+
+int rsvd_dev_blockrand(const double* dA, long long m, long long n, long long lda, int k, int p,
+                       double TOL, int vnum, int kstep, int q, int s, unsigned long long seed,
+                       const double* dRN, int* frank, double** dU, double** dS, double** dV) {
+  Api api;
+  if (api.rc) return api.rc;
+  OmegaSrc om;
+  om.seed = seed;
+  om.dev = dRN;
+  om.ld = n;
+  BlockrandOut out;
+  RSVD_TRY(blockrand_f64(api.c, dA, m, n, lda, k, p, TOL, vnum, kstep, q, s, om, &out));
+  const int fr = out.frank;
+  double *u = nullptr, *sv = nullptr, *v = nullptr;
+  RSVD_HIP(hipMalloc(&u, sizeof(double) * (size_t)m * fr));
+  RSVD_HIP(hipMalloc(&sv, sizeof(double) * (size_t)fr));
+  RSVD_HIP(hipMalloc(&v, sizeof(double) * (size_t)n * fr));
+  RSVD_HIP(hipMemcpyAsync(u, out.U.p, sizeof(double) * (size_t)m * fr, hipMemcpyDeviceToDevice, api.c->stream));
+  RSVD_HIP(hipMemcpyAsync(sv, out.S.p, sizeof(double) * (size_t)fr, hipMemcpyDeviceToDevice, api.c->stream));
+  RSVD_HIP(hipMemcpyAsync(v, out.V.p, sizeof(double) * (size_t)n * fr, hipMemcpyDeviceToDevice, api.c->stream));
+  RSVD_TRY(stream_sync(api.c));
+  *frank = fr;
+  *dU = u;
+  *dS = sv;
+  *dV = v;
+  return 0;
+}
+
+int rsvd_dev_gemm(int dtype, char ta, char tb, long long M, long long N, long long K, double alpha,
+                  const void* A, long long lda, const void* B, long long ldb, double beta, void* C,
+                  long long ldc) {
+  Api api;
+  if (api.rc) return api.rc;
+  if (dtype == 0)
+    return gemm_f64(api.c, ta, tb, M, N, K, alpha, (const double*)A, lda, (const double*)B, ldb,
+                    beta, (double*)C, ldc);
+  if (dtype == 1)
+    return gemm_f32(api.c, ta, tb, M, N, K, (float)alpha, (const float*)A, lda, (const float*)B,
+                    ldb, (float)beta, (float*)C, ldc);
+  return set_error("rsvd_dev_gemm: dtype %d not supported (0 = f64, 1 = f32)", dtype);
+}
+
+// ---------------------------------------------------------------- timing
+static hipEvent_t g_t0 = nullptr, g_t1 = nullptr;
+
+int rsvd_timer_start(void) {
+  Api api;
+  if (api.rc) return api.rc;
+  if (!g_t0) {
+    RSVD_HIP(hipEventCreate(&g_t0));
+    RSVD_HIP(hipEventCreate(&g_t1));
+  }
+  RSVD_HIP(hipEventRecord(g_t0, api.c->stream));
+  return 0;
+}
+
+double rsvd_timer_stop_ms(void) {
+  Api api;
+  if (api.rc || !g_t0) return -1.0;
+  float ms = 0;
+  if (hipEventRecord(g_t1, api.c->stream) != hipSuccess) return -1.0;
+  if (hipEventSynchronize(g_t1) != hipSuccess) return -1.0;
+  if (hipEventElapsedTime(&ms, g_t0, g_t1) != hipSuccess) return -1.0;
+  return ms;
+}
+
+static void kstats_resolve(Context* c) {
+  if (c->pending.empty()) return;
+  (void)hipStreamSynchronize(c->stream);
+  for (auto& pe : c->pending) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, pe.a, pe.b) == hipSuccess) {
+      c->stats[pe.idx].total_ms += ms;
+      c->stats[pe.idx].count += 1;
+    }
+    c->event_pool.push_back(pe.a);
+    c->event_pool.push_back(pe.b);
+  }
+  c->pending.clear();
+}
+
+void rsvd_kstats_enable(int on) {
+  Api api;
+  if (api.rc) return;
+  kstats_resolve(api.c);
+  api.c->kstats = on != 0;
+}
+void rsvd_kstats_reset(void) {
+  Api api;
+  if (api.rc) return;
+  kstats_resolve(api.c);
+  api.c->stats.clear();
+}
+int rsvd_kstats_get(const char* name, double* total_ms, long long* count) {
+  Api api;
+  if (api.rc) return api.rc;
+  kstats_resolve(api.c);
+  for (auto& s : api.c->stats)
+    if (s.name == name) {
+      if (total_ms) *total_ms = s.total_ms;
+      if (count) *count = s.count;
+      return 0;
+    }
+  return 1;
+}
+
+}  // extern "C"

@@ -1,0 +1,195 @@
+// rsvd_internal.h -- device context, error plumbing, workspace pool and the
+// kernel-level entry points shared by the librsvdpack translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <mutex>
+#include <string>
+#include <vector>
+
+namespace rsvd {
+
+// ---------------------------------------------------------------- errors
+// Every internal routine returns int: 0 = success, nonzero = failure, with the
+// message stored in a thread-local buffer that rsvd_last_error() exposes.
+int set_error(const char* fmt, ...);
+void clear_error();
+const char* last_error();
+
+#define RSVD_HIP(call)                                                          \
+  do {                                                                          \
+    hipError_t e_ = (call);                                                     \
+    if (e_ != hipSuccess)                                                       \
+      return ::rsvd::set_error("%s:%d %s -> %s", __FILE__, __LINE__, #call,     \
+                               hipGetErrorString(e_));                          \
+  } while (0)
+
+#define RSVD_TRY(call)        \
+  do {                        \
+    int r_ = (call);          \
+    if (r_) return r_;        \
+  } while (0)
+
+#define RSVD_LAUNCH_CHECK() RSVD_HIP(hipGetLastError())
+
+// ---------------------------------------------------------------- context
+struct KernelStat {
+  std::string name;
+  double total_ms = 0;
+  long long count = 0;
+};
+
+struct Context {
+  int device = -1;
+  int num_cus = 0;
+  hipStream_t stream = nullptr;
+  std::recursive_mutex mu;
+  // cached device buffers, reused across calls (size-bucketed free list)
+  struct Block { void* p; size_t bytes; };
+  std::vector<Block> free_blocks;
+  size_t pooled_bytes = 0;
+  // per-kernel timing (enabled by rsvd_kstats_enable)
+  bool kstats = false;
+  std::vector<KernelStat> stats;
+  // pending event pairs, resolved on rsvd_kstats_get / sync
+  struct Pending { int idx; hipEvent_t a, b; };
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> event_pool;
+};
+
+// Lazily initialises the current device and the library stream.
+int ctx_get(Context** out);
+
+// Device buffer from the pool (contents undefined).  free() returns it.
+int dmalloc(void** p, size_t bytes);
+void dfree(void* p);
+// Release every pooled buffer back to HIP.
+void pool_trim();
+
+// RAII buffer that returns to the pool.
+template <typename T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  DBuf(DBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+  DBuf& operator=(DBuf&& o) noexcept {
+    if (this != &o) { reset(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+    return *this;
+  }
+  ~DBuf() { reset(); }
+  int alloc(size_t count) {
+    reset();
+    void* q = nullptr;
+    int r = dmalloc(&q, (count ? count : 1) * sizeof(T));
+    if (r) return r;
+    p = static_cast<T*>(q);
+    n = count;
+    return 0;
+  }
+  void reset() {
+    if (p) dfree(p);
+    p = nullptr;
+    n = 0;
+  }
+};
+
+// Kernel timing scope: records HIP events around launches on the library
+// stream when kernel statistics are enabled.
+struct KTimer {
+  Context* c;
+  int idx = -1;
+  hipEvent_t a = nullptr, b = nullptr;
+  KTimer(Context* ctx, const char* name);
+  ~KTimer();
+};
+
+// ---------------------------------------------------------------- views
+// Column-major device matrix view: element (i,j) at p[i + j*ld].
+template <typename T>
+struct DMat {
+  T* p;
+  int64_t rows, cols, ld;
+  __host__ __device__ T* col(int64_t j) const { return p + j * ld; }
+};
+
+// ---------------------------------------------------------------- kernels
+// GEMM: C = alpha * op(A) * op(B) + beta * C, column-major, double or float.
+// ta/tb: 'N' or 'T'.  sumsq (optional, double[grid]) receives per-workgroup
+// partial sums of squares of the stored C (fused Frobenius norm); pass nullptr
+// to skip.
+int gemm_f64(Context* c, char ta, char tb, int64_t M, int64_t N, int64_t K, double alpha,
+             const double* A, int64_t lda, const double* B, int64_t ldb, double beta, double* C,
+             int64_t ldc);
+int gemm_f32(Context* c, char ta, char tb, int64_t M, int64_t N, int64_t K, float alpha,
+             const float* A, int64_t lda, const float* B, int64_t ldb, float beta, float* C,
+             int64_t ldc);
+// C = C - Qp*Bp with fused FP64 sum of squares of the updated C into *sumsq
+// (device scalar, overwritten).
+int gemm_f64_deflate(Context* c, int64_t M, int64_t N, int64_t K, const double* Q, int64_t ldq,
+                     const double* B, int64_t ldb, double* C, int64_t ldc, double* sumsq);
+
+// Gaussian fill: out[i + j*ld] = element (col0+j)*rows_total + i of the stream.
+int gaussian_f64(Context* c, uint64_t seed, int64_t rows, int64_t cols, int64_t col0,
+                 int64_t rows_total, double* out, int64_t ld);
+int gaussian_f32(Context* c, uint64_t seed, int64_t rows, int64_t cols, int64_t col0,
+                 int64_t rows_total, float* out, int64_t ld);
+
+// Elementwise / reductions (double)
+int copy2d_f64(Context* c, int64_t rows, int64_t cols, const double* src, int64_t lds, double* dst,
+               int64_t ldd);
+int fill2d_f64(Context* c, int64_t rows, int64_t cols, double v, double* dst, int64_t ldd);
+int sumsq_f64(Context* c, int64_t rows, int64_t cols, const double* src, int64_t lds,
+              double* out_scalar);
+int sumsq_diff_f64(Context* c, int64_t rows, int64_t cols, const double* a, int64_t lda,
+                   const double* b, int64_t ldb, double* out_scalar);
+int transpose_f64(Context* c, int64_t rows, int64_t cols, const double* src, int64_t lds,
+                  double* dst, int64_t ldd);
+int scale_cols_f64(Context* c, int64_t rows, int64_t cols, double* a, int64_t lda,
+                   const double* colscale, int invert);
+int f32_to_f64(Context* c, int64_t rows, int64_t cols, const float* src, int64_t lds, double* dst,
+               int64_t ldd);
+int f64_to_f32(Context* c, int64_t rows, int64_t cols, const double* src, int64_t lds, float* dst,
+               int64_t ldd);
+// G(i,j) = G(j,i) for i > j (upper triangle mirrored).
+int symmetrize_upper_f64(Context* c, int64_t n, double* G);
+int add_diag_f64(Context* c, int64_t n, double v, double* a, int64_t lda);
+int add_scalar_f64(Context* c, int64_t n, double v, double* a);
+// out[j] = X(:,j) . Y(:,j)
+int coldot_f64(Context* c, int64_t L, int64_t n, const double* X, int64_t ldx, const double* Y,
+               int64_t ldy, double* out);
+
+// Orthonormalisation.  In place: Y (m x n) -> Q with Q^T Q = I and Y = Q R.
+// R (n x n, upper, ld n) optional.  Uses CholeskyQR2; falls back to
+// Householder when the Gram matrix is numerically singular.  *used_fallback
+// (optional) reports which path ran.
+int orth_f64(Context* c, int64_t m, int64_t n, double* Y, int64_t ldy, double* R,
+             int* used_fallback);
+// Householder QR (geqr2 + org2r) in place, R optional.
+int householder_qr_f64(Context* c, int64_t m, int64_t n, double* Y, int64_t ldy, double* R);
+// Cholesky of SPD G (n x n, ld n) -> upper R (n x n, ld n, lower part zero);
+// G's trailing part is overwritten.  *ok = 0 when a pivot is not safely
+// positive.
+int cholesky_upper_f64(Context* c, int64_t n, double* G, double* R, int* ok);
+// Inverse of an upper-triangular R (n x n, ld n) into Rinv (n x n, ld n).
+int trtri_upper_f64(Context* c, int64_t n, const double* R, double* Rinv);
+
+// One-sided Jacobi.  W (L x n, ld L) is rotated in place until its columns
+// are mutually orthogonal; V (n x n, ld n, optional) accumulates the
+// rotations (start from identity).  Returns the number of sweeps.
+int jacobi_f64(Context* c, int64_t L, int64_t n, double* W, int64_t ldw, double* V, int64_t ldv,
+               int max_sweeps, int* sweeps_out);
+// Column norms of W into nrm[n]; optional normalisation of the columns.
+int colnorms_f64(Context* c, int64_t L, int64_t n, double* W, int64_t ldw, double* nrm,
+                 int normalise);
+// Sorts (values, columns of X[, columns of Y]) by value, ascending or
+// descending; perm scratch handled internally.
+int sort_pairs_f64(Context* c, int64_t n, double* vals, int descending, double* X, int64_t rowsX,
+                   int64_t ldx, double* Y, int64_t rowsY, int64_t ldy);
+
+}  // namespace rsvd

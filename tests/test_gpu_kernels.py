@@ -1,0 +1,172 @@
+"""GPU kernel checks through the C ABI: Gaussian stream, GEMM, QR, eigen/SVD, norms.
+
+Floating-point tolerances are stated per test; the Gaussian stream must be
+bit-identical to the oracle's.
+"""
+import numpy as np
+import pytest
+
+from tests.helpers import lowrank_matrix, rel
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("seed,offset,count", [(777, 0, 1), (777, 0, 4099), (1, 5, 1000),
+                                               (2**50 + 3, 10**10, 777)])
+def test_gaussian_stream_bit_identical(gpu_lib, oracle, seed, offset, count):
+    got = gpu_lib.gaussian(seed, offset, count)
+    want = oracle.gaussian(seed, offset, count)
+    assert np.array_equal(got, want)
+
+
+def test_initialize_random_matrix_seeded(gpu_lib, oracle):
+    gpu_lib.set_seed(99)
+    try:
+        M = gpu_lib.initialize_random_matrix(37, 11)
+    finally:
+        gpu_lib.use_time_seed()
+    want = oracle.gaussian(99, 0, 37 * 11).reshape(11, 37).T
+    assert np.array_equal(M, want)
+
+
+GEMM_SHAPES = [(1, 1, 1), (17, 33, 5), (130, 70, 300), (2000, 110, 1000), (64, 64, 64),
+               (129, 65, 17), (300, 7, 5000)]
+
+
+@pytest.mark.parametrize("m,n,k", GEMM_SHAPES)
+def test_gemm_nn(gpu_lib, m, n, k):
+    rng = np.random.default_rng(m * 7 + n)
+    A, B = rng.standard_normal((m, k)), rng.standard_normal((k, n))
+    C = gpu_lib.matrix_matrix_mult(A, B)
+    ref = A @ B
+    # FP64: error bound ~ k * eps * |A||B|
+    assert np.max(np.abs(C - ref)) <= 4 * k * 2.3e-16 * np.max(np.abs(A) @ np.abs(B))
+
+
+@pytest.mark.parametrize("m,n,k", GEMM_SHAPES)
+def test_gemm_tn(gpu_lib, m, n, k):
+    rng = np.random.default_rng(m * 5 + k)
+    A, B = rng.standard_normal((k, m)), rng.standard_normal((k, n))
+    C = gpu_lib.matrix_transpose_matrix_mult(A, B)
+    ref = A.T @ B
+    assert np.max(np.abs(C - ref)) <= 4 * k * 2.3e-16 * np.max(np.abs(A.T) @ np.abs(B))
+
+
+@pytest.mark.parametrize("m,n,k", GEMM_SHAPES)
+def test_gemm_nt(gpu_lib, m, n, k):
+    rng = np.random.default_rng(n * 3 + k)
+    A, B = rng.standard_normal((m, k)), rng.standard_normal((n, k))
+    C = gpu_lib.matrix_matrix_transpose_mult(A, B)
+    ref = A @ B.T
+    assert np.max(np.abs(C - ref)) <= 4 * k * 2.3e-16 * np.max(np.abs(A) @ np.abs(B.T))
+
+
+def test_gemm_shape_mismatch_reports_error(gpu_lib):
+    lib = gpu_lib.load()
+    A, B, Cm = gpu_lib.to_mat(np.ones((3, 4))), gpu_lib.to_mat(np.ones((4, 5))), gpu_lib.empty_mat(3, 4)
+    lib.matrix_matrix_mult(A, B, Cm)
+    assert "output is 3x4" in gpu_lib.last_error()
+    for x in (A, B, Cm):
+        lib.matrix_delete(x)
+
+
+@pytest.mark.parametrize("m,n", [(50, 3), (2000, 110), (1000, 110), (4096, 256), (300, 300)])
+def test_qr_getQ(gpu_lib, m, n):
+    rng = np.random.default_rng(m + n)
+    M = rng.standard_normal((m, n)) * np.logspace(0, -3, n)
+    Q = gpu_lib.QR_factorization_getQ(M)
+    assert np.abs(Q.T @ Q - np.eye(n)).max() < 1e-13
+    # same range as LAPACK's Q: Q Q^T M = M
+    assert np.linalg.norm(Q @ (Q.T @ M) - M) < 1e-13 * np.linalg.norm(M) * np.sqrt(n)
+
+
+def test_compact_qr_R_upper_and_product(gpu_lib):
+    rng = np.random.default_rng(4)
+    M = rng.standard_normal((500, 60))
+    Q, R = gpu_lib.compact_QR_factorization(M)
+    assert np.allclose(np.tril(R, -1), 0)
+    assert np.linalg.norm(Q @ R - M) < 1e-13 * np.linalg.norm(M)
+    # R agrees with LAPACK's up to row signs
+    Rl = np.linalg.qr(M, mode="r")
+    assert np.allclose(np.abs(R), np.abs(Rl), rtol=1e-11, atol=1e-12)
+
+
+def test_qr_householder_fallback_on_rank_deficient(gpu_lib):
+    """A repeated column breaks CholeskyQR; the Householder path must take over."""
+    rng = np.random.default_rng(5)
+    M = rng.standard_normal((400, 30))
+    M[:, 7] = M[:, 3]  # exactly dependent columns
+    Q = gpu_lib.QR_factorization_getQ(M)
+    assert np.abs(Q.T @ Q - np.eye(30)).max() < 1e-13
+    assert np.linalg.norm(Q @ (Q.T @ M) - M) < 1e-12 * np.linalg.norm(M)
+
+
+def test_qr_ill_conditioned(gpu_lib):
+    rng = np.random.default_rng(6)
+    U, _ = np.linalg.qr(rng.standard_normal((1000, 40)))
+    V, _ = np.linalg.qr(rng.standard_normal((40, 40)))
+    M = (U * np.logspace(0, -12, 40)) @ V.T
+    Q = gpu_lib.QR_factorization_getQ(M)
+    assert np.abs(Q.T @ Q - np.eye(40)).max() < 1e-12
+
+
+@pytest.mark.parametrize("n", [2, 5, 33, 110, 300])
+def test_symmetric_eig(gpu_lib, n):
+    rng = np.random.default_rng(n)
+    X = rng.standard_normal((n, n))
+    S = X + X.T
+    vals, vecs = gpu_lib.compute_evals_and_evecs_of_symm_matrix(S)
+    w, W = np.linalg.eigh(S)
+    scale = np.abs(w).max()
+    assert np.all(np.diff(vals) >= 0)
+    assert np.abs(vals - w).max() < 1e-12 * scale
+    assert np.linalg.norm(S @ vecs - vecs * vals) < 1e-11 * scale * np.sqrt(n)
+    assert np.abs(vecs.T @ vecs - np.eye(n)).max() < 1e-12
+
+
+def test_symmetric_eig_uses_upper_triangle(gpu_lib):
+    rng = np.random.default_rng(9)
+    X = rng.standard_normal((20, 20))
+    S = X + X.T
+    garbage = S.copy()
+    garbage[np.tril_indices(20, -1)] = 1e6  # dsyev('U') never reads these
+    vals, _ = gpu_lib.compute_evals_and_evecs_of_symm_matrix(garbage)
+    assert np.abs(vals - np.linalg.eigvalsh(S)).max() < 1e-12 * np.abs(vals).max()
+
+
+@pytest.mark.parametrize("m,n", [(110, 110), (60, 40), (40, 60), (544, 544)])
+def test_svd(gpu_lib, m, n):
+    rng = np.random.default_rng(m * n)
+    M = rng.standard_normal((m, n))
+    U, S, Vt = gpu_lib.singular_value_decomposition(M)
+    s = np.linalg.svd(M, compute_uv=False)
+    k = min(m, n)
+    assert rel(np.diag(S)[:k], s) < 1e-12
+    assert np.linalg.norm(U @ S @ Vt - M) < 1e-12 * np.linalg.norm(M) * np.sqrt(k)
+
+
+def test_norms(gpu_lib):
+    rng = np.random.default_rng(1)
+    A, B = rng.standard_normal((300, 77)), rng.standard_normal((300, 77))
+    assert abs(gpu_lib.get_matrix_frobenius_norm(A) - np.linalg.norm(A)) < 1e-13 * np.linalg.norm(A)
+    pe = gpu_lib.get_percent_error_between_two_mats(A, B)
+    assert abs(pe - 100 * np.linalg.norm(A - B) / np.linalg.norm(A)) < 1e-11
+
+
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("T", "N"), ("N", "T"), ("T", "T")])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("m,n,k", [(33, 47, 29), (256, 130, 600), (1000, 64, 4096)])
+def test_dev_gemm_all_ops(gpu_lib, ta, tb, dtype, m, n, k):
+    rng = np.random.default_rng(m + 3 * n + 7 * k)
+    A = rng.standard_normal((k, m) if ta == "T" else (m, k)).astype(dtype)
+    B = rng.standard_normal((n, k) if tb == "T" else (k, n)).astype(dtype)
+    C0 = rng.standard_normal((m, n)).astype(dtype)
+    dA, dB, dC = (gpu_lib.DeviceMatrix.from_host(x) for x in (A, B, C0))
+    gpu_lib.dev_gemm(ta, tb, dA, dB, dC, alpha=0.5, beta=-2.0)
+    got = dC.to_host()
+    opA = A.T if ta == "T" else A
+    opB = B.T if tb == "T" else B
+    want = 0.5 * (opA.astype(np.float64) @ opB.astype(np.float64)) - 2.0 * C0.astype(np.float64)
+    eps = np.finfo(dtype).eps
+    bound = 4 * k * eps * (np.abs(opA) @ np.abs(opB)).max() + 4 * eps * np.abs(C0).max()
+    assert np.abs(got - want).max() <= bound
