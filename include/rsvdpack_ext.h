@@ -62,6 +62,24 @@ int rsvd_dev_fixed_rank(const double *dA, long long m, long long n, long long ld
                         int vnum, int q, int s, unsigned long long seed, const double *dOmega,
                         double *dU, long long ldu, double *dS, double *dV, long long ldv);
 
+/* Mixed-precision fixed-rank rSVD of a float32 device matrix A (m x n, lda).
+ * precision 1: sketch and power-iteration passes on the bf16 matrix cores
+ * (fp32 accumulation); 2: the same passes in split-bf16 (fp32 accuracy).
+ * Always: CholeskyQR with FP64 Cholesky, split-bf16 projection B^T = A^T Q,
+ * FP64 BB^T and eigen/SVD solve.  Omega: dOmega (n x (k+p) float, ld n) or the
+ * seeded stream rounded to float.  Outputs: dU m x k float (ldu), dS k doubles
+ * (reference order), dV n x k float (ldv). */
+int rsvd_dev_fixed_rank_mp(const float *dA, long long m, long long n, long long lda, int k, int p,
+                           int vnum, int q, int s, unsigned long long seed, const float *dOmega,
+                           int precision, float *dU, long long ldu, double *dS, float *dV,
+                           long long ldv);
+
+/* ||A - U diag(S) V^T||_F for a double device matrix A and device factors U
+ * (m x k), S (k), V (n x k) stored as double (uv_f32 = 0) or float (1). */
+int rsvd_dev_lowrank_residual(const double *dA, long long m, long long n, long long lda,
+                              long long k, const void *dU, long long ldu, const double *dS,
+                              const void *dV, long long ldv, int uv_f32, double *out);
+
 /* Columns that randQB_pb (is_new = 0) / randQB_pb_new (is_new = 1) may
  * produce for these arguments (after the kstep clamp / TOL-mode nstep). */
 long long rsvd_qb_max_cols(long long m, long long n, int kstep, int nstep, int is_new);
@@ -80,11 +98,28 @@ int rsvd_dev_blockrand(const double *dA, long long m, long long n, long long lda
                        double TOL, int vnum, int kstep, int q, int s, unsigned long long seed,
                        const double *dRN, int *frank, double **dU, double **dS, double **dV);
 
-/* C = alpha*op(A)*op(B) + beta*C on device matrices; dtype 0 = double,
- * 1 = float; ta / tb = 'N' or 'T'. */
+/* C = alpha*op(A)*op(B) + beta*C on device matrices; ta / tb = 'N' or 'T'.
+ * dtype 0: double; 1: float (fp32 matrix cores); 2: float storage, products on
+ * the bf16 matrix cores with fp32 accumulation; 3: float storage, split-bf16
+ * (hi*hi + hi*lo + lo*hi, about fp32 accuracy). */
 int rsvd_dev_gemm(int dtype, char ta, char tb, long long M, long long N, long long K, double alpha,
                   const void *A, long long lda, const void *B, long long ldb, double beta, void *C,
                   long long ldc);
+
+/* Synthetic test matrix in HBM: A = U diag(sigma) V^T + noise_std * N(0,1)
+ * with Haar U (m x r), V (n x r) drawn from the seeded stream; sigma is a host
+ * array of r values.  dA64 (double) and / or dA32 (float) may be NULL; when
+ * both are given dA64 holds exactly the float-rounded values. */
+int rsvd_dev_lowrank_test_matrix(long long m, long long n, int r, const double *sigma,
+                                 double noise_std, unsigned long long seed, double *dA64,
+                                 long long lda64, float *dA32, long long lda32);
+
+/* Eigendecomposition of a symmetric positive semidefinite device matrix G
+ * (n x n, upper triangle used): eigenvalues ascending into dEvals, orthonormal
+ * eigenvectors into dEvecs (n x n, ldv); *sweeps = Jacobi sweeps used.  This is
+ * the solver behind the vnum = 2 path. */
+int rsvd_dev_sym_eig_psd(long long n, const double *dG, long long ldg, double *dEvals,
+                         double *dEvecs, long long ldv, int *sweeps);
 
 /* ---- timing of the library's stream ---- */
 int rsvd_timer_start(void);

@@ -113,6 +113,17 @@ __global__ void gaussian_kernel(uint64_t seed, int64_t rows, int64_t cols, int64
   }
 }
 
+// A(i,j) += s * element (j*rows + i) of the stream for `seed`.
+__global__ void add_gaussian_kernel(uint64_t seed, int64_t rows, int64_t cols, double s,
+                                    double* __restrict__ A, int64_t lda) {
+  int64_t total = rows * cols;
+  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    int64_t i = idx % rows, j = idx / rows;
+    A[i + j * lda] += s * gaussian_at(seed, (uint64_t)idx);
+  }
+}
+
 static inline int grid_for(int64_t total) {
   int64_t b = (total + 255) / 256;
   if (b > 8192) b = 8192;
@@ -137,6 +148,16 @@ int gaussian_f32(Context* c, uint64_t seed, int64_t rows, int64_t cols, int64_t 
   return 0;
 }
 
+int add_gaussian_f64(Context* c, uint64_t seed, int64_t rows, int64_t cols, double s, double* A,
+                     int64_t lda) {
+  if (rows * cols == 0) return 0;
+  int64_t b = (rows * cols + 255) / 256;
+  hipLaunchKernelGGL(add_gaussian_kernel, dim3((unsigned)(b > 16384 ? 16384 : b)), dim3(256), 0,
+                     c->stream, seed, rows, cols, s, A, lda);
+  RSVD_LAUNCH_CHECK();
+  return 0;
+}
+
 // ---------------------------------------------------------------- 2-D helpers
 template <typename S, typename D>
 __global__ void copy2d_kernel(int64_t rows, int64_t cols, const S* __restrict__ src, int64_t lds,
@@ -153,6 +174,14 @@ int copy2d_f64(Context* c, int64_t rows, int64_t cols, const double* src, int64_
                int64_t ldd) {
   if (rows * cols == 0) return 0;
   hipLaunchKernelGGL((copy2d_kernel<double, double>), dim3(grid_for(rows * cols)), dim3(256), 0,
+                     c->stream, rows, cols, src, lds, dst, ldd);
+  RSVD_LAUNCH_CHECK();
+  return 0;
+}
+int copy2d_f32(Context* c, int64_t rows, int64_t cols, const float* src, int64_t lds, float* dst,
+               int64_t ldd) {
+  if (rows * cols == 0) return 0;
+  hipLaunchKernelGGL((copy2d_kernel<float, float>), dim3(grid_for(rows * cols)), dim3(256), 0,
                      c->stream, rows, cols, src, lds, dst, ldd);
   RSVD_LAUNCH_CHECK();
   return 0;

@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "pipeline.h"
+#include "pipeline_mp.h"
 #include "rsvdpack.h"
 #include "rsvdpack_ext.h"
 
@@ -386,7 +387,8 @@ void compute_evals_and_evecs_of_symm_matrix(mat* S, vec* evals) {
   const double shift = sqrt(ss);
   if (add_diag_f64(c, n, shift, G.p, n)) return;
   int sw = 0;
-  if (jacobi_f64(c, n, n, G.p, n, V.p, n, 60, &sw)) return;
+  // the shift inflates the column norms: orthogonalise to the rounding floor
+  if (jacobi_f64(c, n, n, G.p, n, V.p, n, 60, &sw, 0.0)) return;
   // eigenvalues as Rayleigh quotients v^T G v of the unshifted matrix
   if (gemm_f64(c, 'N', 'N', n, n, n, 1.0, G0.p, n, V.p, n, 0.0, G.p, n)) return;
   if (coldot_f64(c, n, n, V.p, n, G.p, n, lam.p)) return;
@@ -810,7 +812,53 @@ int rsvd_dev_gemm(int dtype, char ta, char tb, long long M, long long N, long lo
   if (dtype == 1)
     return gemm_f32(api.c, ta, tb, M, N, K, (float)alpha, (const float*)A, lda, (const float*)B,
                     ldb, (float)beta, (float*)C, ldc);
-  return set_error("rsvd_dev_gemm: dtype %d not supported (0 = f64, 1 = f32)", dtype);
+  if (dtype == 2 || dtype == 3)
+    return gemm_bf16(api.c, "gemm_bf16", ta, tb, M, N, K, (float)alpha, (const float*)A, lda,
+                     (const float*)B, ldb, (float)beta, (float*)C, ldc, dtype == 2 ? 1 : 3);
+  return set_error("rsvd_dev_gemm: dtype %d not supported", dtype);
+}
+
+int rsvd_dev_lowrank_test_matrix(long long m, long long n, int r, const double* sigma,
+                                 double noise_std, unsigned long long seed, double* dA64,
+                                 long long lda64, float* dA32, long long lda32) {
+  Api api;
+  if (api.rc) return api.rc;
+  return lowrank_test_matrix(api.c, m, n, r, sigma, noise_std, seed, dA64, lda64, dA32, lda32);
+}
+
+int rsvd_dev_fixed_rank_mp(const float* dA, long long m, long long n, long long lda, int k, int p,
+                           int vnum, int q, int s, unsigned long long seed, const float* dOmega,
+                           int precision, float* dU, long long ldu, double* dS, float* dV,
+                           long long ldv) {
+  Api api;
+  if (api.rc) return api.rc;
+  return fixed_rank_mp(api.c, dA, m, n, lda, k, p, vnum, q, s, seed, dOmega, precision, dU, ldu,
+                       dS, dV, ldv, nullptr);
+}
+
+int rsvd_dev_lowrank_residual(const double* dA, long long m, long long n, long long lda,
+                              long long k, const void* dU, long long ldu, const double* dS,
+                              const void* dV, long long ldv, int uv_f32, double* out) {
+  Api api;
+  if (api.rc) return api.rc;
+  return lowrank_residual(api.c, dA, m, n, lda, k, dU, ldu, dS, dV, ldv, uv_f32, out);
+}
+
+int rsvd_dev_sym_eig_psd(long long n, const double* dG, long long ldg, double* dEvals,
+                         double* dEvecs, long long ldv, int* sweeps) {
+  Api api;
+  if (api.rc) return api.rc;
+  Context* c = api.c;
+  DBuf<double> W;
+  RSVD_TRY(W.alloc((size_t)(n * n)));
+  RSVD_TRY(copy2d_f64(c, n, n, dG, ldg, W.p, n));
+  RSVD_TRY(symmetrize_upper_f64(c, n, W.p));
+  int sw = 0;
+  RSVD_TRY(jacobi_f64(c, n, n, W.p, n, dEvecs, ldv, 60, &sw));
+  RSVD_TRY(colnorms_f64(c, n, n, W.p, n, dEvals, 0));
+  RSVD_TRY(sort_pairs_f64(c, n, dEvals, 0, dEvecs, n, ldv, nullptr, 0, 0));
+  if (sweeps) *sweeps = sw;
+  return stream_sync(c);
 }
 
 // ---------------------------------------------------------------- timing

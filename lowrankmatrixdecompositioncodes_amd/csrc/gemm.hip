@@ -320,6 +320,12 @@ int gemm_f64(Context* c, char ta, char tb, int64_t M, int64_t N, int64_t K, doub
                           "gemm_f64");
 }
 
+int gemm_f64_tag(Context* c, const char* tag, char ta, char tb, int64_t M, int64_t N, int64_t K,
+                 double alpha, const double* A, int64_t lda, const double* B, int64_t ldb,
+                 double beta, double* C, int64_t ldc) {
+  return gemm_any<double>(c, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tag);
+}
+
 int gemm_f32(Context* c, char ta, char tb, int64_t M, int64_t N, int64_t K, float alpha,
              const float* A, int64_t lda, const float* B, int64_t ldb, float beta, float* C,
              int64_t ldc) {

@@ -67,4 +67,11 @@ int blockrand_f64(Context* c, const double* M, int64_t m, int64_t n, int64_t ldm
                   double TOL, int vnum, int kstep, int q, int s, const OmegaSrc& om,
                   BlockrandOut* out);
 
+// A = U diag(sigma) V^T + noise_std * N(0,1), U, V Haar (m x r, n x r).  Writes
+// the double matrix to A64 and / or its float rounding to A32 (when both are
+// given, A64 holds exactly the float values).
+int lowrank_test_matrix(Context* c, int64_t m, int64_t n, int64_t r, const double* sigma_host,
+                        double noise_std, uint64_t seed, double* A64, int64_t lda64, float* A32,
+                        int64_t lda32);
+
 }  // namespace rsvd

@@ -60,6 +60,13 @@ static int grid1(int64_t n) {
   return (int)(b < 1 ? 1 : (b > 4096 ? 4096 : b));
 }
 
+int sqrt_f64(Context* c, int64_t n, const double* in, double* out) {
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(sqrt_kernel, dim3(grid1(n)), dim3(256), 0, c->stream, n, in, out);
+  RSVD_LAUNCH_CHECK();
+  return 0;
+}
+
 int svd_from_Q_f64(Context* c, const double* A, int64_t m, int64_t n, int64_t lda,
                    const double* Q, int64_t ldq, int64_t l, int64_t k, int vnum, double* U,
                    int64_t ldu, double* S, double* V, int64_t ldv, SvdInfo* info) {
@@ -75,7 +82,7 @@ int svd_from_Q_f64(Context* c, const double* A, int64_t m, int64_t n, int64_t ld
     RSVD_TRY(R.alloc((size_t)(l * l)));
     RSVD_TRY(Vh.alloc((size_t)(l * l)));
     RSVD_TRY(sig.alloc((size_t)l));
-    RSVD_TRY(gemm_f64(c, 'T', 'N', n, l, m, 1.0, A, lda, Q, ldq, 0.0, Bt.p, n));
+    RSVD_TRY(gemm_f64_tag(c, "apass_f64", 'T', 'N', n, l, m, 1.0, A, lda, Q, ldq, 0.0, Bt.p, n));
     int fb = 0;
     RSVD_TRY(orth_f64(c, n, l, Bt.p, n, R.p, &fb));
     int sw = 0;
@@ -96,7 +103,7 @@ int svd_from_Q_f64(Context* c, const double* A, int64_t m, int64_t n, int64_t ld
   RSVD_TRY(Uh.alloc((size_t)(l * l)));
   RSVD_TRY(lam.alloc((size_t)l));
   RSVD_TRY(sig.alloc((size_t)l));
-  RSVD_TRY(gemm_f64(c, 'T', 'N', l, n, m, 1.0, Q, ldq, A, lda, 0.0, B.p, l));
+  RSVD_TRY(gemm_f64_tag(c, "apass_f64", 'T', 'N', l, n, m, 1.0, Q, ldq, A, lda, 0.0, B.p, l));
   RSVD_TRY(gemm_f64(c, 'N', 'T', l, l, n, 1.0, B.p, l, B.p, l, 0.0, G.p, l));
   hipLaunchKernelGGL(symmetrize_upper_kernel, dim3(grid1(l * l)), dim3(256), 0, c->stream, l, G.p);
   RSVD_LAUNCH_CHECK();
@@ -132,13 +139,13 @@ int fixed_rank_f64(Context* c, const double* A, int64_t m, int64_t n, int64_t ld
   RSVD_TRY(omega_block(c, om, n, 0, l, Ob, &Om, &ldo));               // rra.c:90-91
   RSVD_TRY(Y.alloc((size_t)(m * l)));
   RSVD_TRY(Z.alloc((size_t)(n * l)));
-  RSVD_TRY(gemm_f64(c, 'N', 'N', m, l, n, 1.0, A, lda, Om, ldo, 0.0, Y.p, m));  // rra.c:95
+  RSVD_TRY(gemm_f64_tag(c, "apass_f64", 'N', 'N', m, l, n, 1.0, A, lda, Om, ldo, 0.0, Y.p, m));  // rra.c:95
   int fb = 0, f1 = 0;
   for (int j = 1; j < q; ++j) {                                        // rra.c:101-126
     if ((2 * j - 2) % s == 0) { RSVD_TRY(orth_f64(c, m, l, Y.p, m, nullptr, &f1)); fb += f1; }
-    RSVD_TRY(gemm_f64(c, 'T', 'N', n, l, m, 1.0, A, lda, Y.p, m, 0.0, Z.p, n));
+    RSVD_TRY(gemm_f64_tag(c, "apass_f64", 'T', 'N', n, l, m, 1.0, A, lda, Y.p, m, 0.0, Z.p, n));
     if ((2 * j - 1) % s == 0) { RSVD_TRY(orth_f64(c, n, l, Z.p, n, nullptr, &f1)); fb += f1; }
-    RSVD_TRY(gemm_f64(c, 'N', 'N', m, l, n, 1.0, A, lda, Z.p, n, 0.0, Y.p, m));
+    RSVD_TRY(gemm_f64_tag(c, "apass_f64", 'N', 'N', m, l, n, 1.0, A, lda, Z.p, n, 0.0, Y.p, m));
   }
   RSVD_TRY(orth_f64(c, m, l, Y.p, m, nullptr, &f1));                   // rra.c:129-130
   fb += f1;
@@ -184,13 +191,13 @@ int qb_blocked_f64(Context* c, const double* M, int64_t m, int64_t n, int64_t ld
     const double* RNp;
     int64_t ldr;
     RSVD_TRY(omega_block(c, om, n, c0, kstep, Ob, &RNp, &ldr));       // rra.c:1639-1642
-    RSVD_TRY(gemm_f64(c, 'N', 'N', m, kstep, n, 1.0, A.p, m, RNp, ldr, 0.0, Yp.p, m));
+    RSVD_TRY(gemm_f64_tag(c, "apass_f64", 'N', 'N', m, kstep, n, 1.0, A.p, m, RNp, ldr, 0.0, Yp.p, m));
     for (int j = 1; j <= q; ++j) {                                      // rra.c:1652-1686
       if ((2 * j - 2) % s == 0) { RSVD_TRY(orth_f64(c, m, kstep, Yp.p, m, nullptr, &f1)); fb += f1; }
       // rra.c:1657-1658 forms Qp^T A and transposes it; A^T Qp directly here
-      RSVD_TRY(gemm_f64(c, 'T', 'N', n, kstep, m, 1.0, A.p, m, Yp.p, m, 0.0, AtQp.p, n));
+      RSVD_TRY(gemm_f64_tag(c, "apass_f64", 'T', 'N', n, kstep, m, 1.0, A.p, m, Yp.p, m, 0.0, AtQp.p, n));
       if ((2 * j - 1) % s == 0) { RSVD_TRY(orth_f64(c, n, kstep, AtQp.p, n, nullptr, &f1)); fb += f1; }
-      RSVD_TRY(gemm_f64(c, 'N', 'N', m, kstep, n, 1.0, A.p, m, AtQp.p, n, 0.0, Yp.p, m));
+      RSVD_TRY(gemm_f64_tag(c, "apass_f64", 'N', 'N', m, kstep, n, 1.0, A.p, m, AtQp.p, n, 0.0, Yp.p, m));
     }
     RSVD_TRY(orth_f64(c, m, kstep, Yp.p, m, nullptr, &f1));             // rra.c:1690
     fb += f1;
@@ -206,7 +213,7 @@ int qb_blocked_f64(Context* c, const double* M, int64_t m, int64_t n, int64_t ld
     }
     // Bp = Qp^T A straight into rows c0.. of B; A -= Qp Bp with fused ||A||^2
     double* Bp = B + c0;
-    RSVD_TRY(gemm_f64(c, 'T', 'N', kstep, n, m, 1.0, Yp.p, m, A.p, m, 0.0, Bp, ldb));  // :1741
+    RSVD_TRY(gemm_f64_tag(c, "apass_f64", 'T', 'N', kstep, n, m, 1.0, Yp.p, m, A.p, m, 0.0, Bp, ldb));  // :1741
     RSVD_TRY(gemm_f64_deflate(c, m, n, kstep, Yp.p, m, Bp, ldb, A.p, m, sums.p));        // :1750
     RSVD_TRY(copy2d_f64(c, m, kstep, Yp.p, m, Q + c0 * ldq, ldq));                       // :1760
     fr = (step + 1) * kstep;                                            // rra.c:1770

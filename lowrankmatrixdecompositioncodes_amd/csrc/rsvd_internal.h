@@ -126,9 +126,20 @@ struct DMat {
 int gemm_f64(Context* c, char ta, char tb, int64_t M, int64_t N, int64_t K, double alpha,
              const double* A, int64_t lda, const double* B, int64_t ldb, double beta, double* C,
              int64_t ldc);
+// Same as gemm_f64, timed under the kernel-statistics family `tag`.
+int gemm_f64_tag(Context* c, const char* tag, char ta, char tb, int64_t M, int64_t N, int64_t K,
+                 double alpha, const double* A, int64_t lda, const double* B, int64_t ldb,
+                 double beta, double* C, int64_t ldc);
 int gemm_f32(Context* c, char ta, char tb, int64_t M, int64_t N, int64_t K, float alpha,
              const float* A, int64_t lda, const float* B, int64_t ldb, float beta, float* C,
              int64_t ldc);
+// fp32 C = alpha*op(A)*op(B) + beta*C on the bf16 matrix cores, fp32 operands
+// rounded to bf16 while staged (split = 1) or split into bf16 hi + lo with
+// three products (split = 3, ~fp32 accuracy).  `tag` names the launch for
+// kernel statistics.
+int gemm_bf16(Context* c, const char* tag, char ta, char tb, int64_t M, int64_t N, int64_t K,
+              float alpha, const float* A, int64_t lda, const float* B, int64_t ldb, float beta,
+              float* C, int64_t ldc, int split);
 // C = C - Qp*Bp with fused FP64 sum of squares of the updated C into *sumsq
 // (device scalar, overwritten).
 int gemm_f64_deflate(Context* c, int64_t M, int64_t N, int64_t K, const double* Q, int64_t ldq,
@@ -139,6 +150,10 @@ int gaussian_f64(Context* c, uint64_t seed, int64_t rows, int64_t cols, int64_t 
                  int64_t rows_total, double* out, int64_t ld);
 int gaussian_f32(Context* c, uint64_t seed, int64_t rows, int64_t cols, int64_t col0,
                  int64_t rows_total, float* out, int64_t ld);
+
+// A(i,j) += s * element (j*rows + i) of the Gaussian stream for `seed`.
+int add_gaussian_f64(Context* c, uint64_t seed, int64_t rows, int64_t cols, double s, double* A,
+                     int64_t lda);
 
 // Elementwise / reductions (double)
 int copy2d_f64(Context* c, int64_t rows, int64_t cols, const double* src, int64_t lds, double* dst,
@@ -152,6 +167,10 @@ int transpose_f64(Context* c, int64_t rows, int64_t cols, const double* src, int
                   double* dst, int64_t ldd);
 int scale_cols_f64(Context* c, int64_t rows, int64_t cols, double* a, int64_t lda,
                    const double* colscale, int invert);
+int copy2d_f32(Context* c, int64_t rows, int64_t cols, const float* src, int64_t lds, float* dst,
+               int64_t ldd);
+// out[j] = sqrt(in[j])
+int sqrt_f64(Context* c, int64_t n, const double* in, double* out);
 int f32_to_f64(Context* c, int64_t rows, int64_t cols, const float* src, int64_t lds, double* dst,
                int64_t ldd);
 int f64_to_f32(Context* c, int64_t rows, int64_t cols, const double* src, int64_t lds, float* dst,
@@ -180,10 +199,12 @@ int cholesky_upper_f64(Context* c, int64_t n, double* G, double* R, int* ok);
 int trtri_upper_f64(Context* c, int64_t n, const double* R, double* Rinv);
 
 // One-sided Jacobi.  W (L x n, ld L) is rotated in place until its columns
-// are mutually orthogonal; V (n x n, ld n, optional) accumulates the
-// rotations (start from identity).  Returns the number of sweeps.
+// are mutually orthogonal to rel_tol (0: the rounding floor eps*sqrt(L));
+// V (n x n, ld n, optional) accumulates the rotations (start from identity).
+// *sweeps_out = sweeps used.
+constexpr double kJacobiTol = 1e-12;
 int jacobi_f64(Context* c, int64_t L, int64_t n, double* W, int64_t ldw, double* V, int64_t ldv,
-               int max_sweeps, int* sweeps_out);
+               int max_sweeps, int* sweeps_out, double rel_tol = kJacobiTol);
 // Column norms of W into nrm[n]; optional normalisation of the columns.
 int colnorms_f64(Context* c, int64_t L, int64_t n, double* W, int64_t ldw, double* nrm,
                  int normalise);

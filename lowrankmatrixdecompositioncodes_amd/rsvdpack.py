@@ -126,6 +126,19 @@ _SIGS = {
     "rsvd_dev_gemm": (C.c_int, [C.c_int, C.c_char, C.c_char, C.c_longlong, C.c_longlong,
                                 C.c_longlong, C.c_double, C.c_void_p, C.c_longlong, C.c_void_p,
                                 C.c_longlong, C.c_double, C.c_void_p, C.c_longlong]),
+    "rsvd_dev_sym_eig_psd": (C.c_int, [C.c_longlong, C.c_void_p, C.c_longlong, C.c_void_p,
+                                       C.c_void_p, C.c_longlong, C.POINTER(C.c_int)]),
+    "rsvd_dev_lowrank_test_matrix": (C.c_int, [C.c_longlong, C.c_longlong, C.c_int, C.c_void_p,
+                                               C.c_double, C.c_ulonglong, C.c_void_p,
+                                               C.c_longlong, C.c_void_p, C.c_longlong]),
+    "rsvd_dev_fixed_rank_mp": (C.c_int, [C.c_void_p, C.c_longlong, C.c_longlong, C.c_longlong,
+                                         C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                         C.c_ulonglong, C.c_void_p, C.c_int, C.c_void_p,
+                                         C.c_longlong, C.c_void_p, C.c_void_p, C.c_longlong]),
+    "rsvd_dev_lowrank_residual": (C.c_int, [C.c_void_p, C.c_longlong, C.c_longlong,
+                                            C.c_longlong, C.c_longlong, C.c_void_p,
+                                            C.c_longlong, C.c_void_p, C.c_void_p, C.c_longlong,
+                                            C.c_int, C.POINTER(C.c_double)]),
     "rsvd_timer_start": (C.c_int, []),
     "rsvd_timer_stop_ms": (C.c_double, []),
     "rsvd_kstats_enable": (None, [C.c_int]),
@@ -451,12 +464,18 @@ def dev_fixed_rank(A: DeviceMatrix, k, p, vnum, q, s, seed, omega: DeviceMatrix 
 
 
 def dev_gemm(ta: str, tb: str, A: DeviceMatrix, B: DeviceMatrix, C_: DeviceMatrix,
-             alpha: float = 1.0, beta: float = 0.0) -> None:
-    """C = alpha*op(A)*op(B) + beta*C on device matrices of one dtype (f64 / f32)."""
+             alpha: float = 1.0, beta: float = 0.0, mode: str | None = None) -> None:
+    """C = alpha*op(A)*op(B) + beta*C on device matrices of one dtype (f64 / f32).
+
+    mode (float32 only): None -> fp32 matrix cores, "bf16" -> bf16 products with
+    fp32 accumulation, "bf16x3" -> split-bf16 products.
+    """
     M = C_.rows
     N = C_.cols
     K = A.rows if ta == "T" else A.cols
     dt = {np.dtype(np.float64): 0, np.dtype(np.float32): 1}[A.dtype]
+    if mode is not None:
+        dt = {"bf16": 2, "bf16x3": 3}[mode]
     rc = load().rsvd_dev_gemm(dt, ta.encode(), tb.encode(), M, N, K, alpha, A.ptr, A.rows,
                               B.ptr, B.rows, beta, C_.ptr, C_.rows)
     _check(rc, "rsvd_dev_gemm")

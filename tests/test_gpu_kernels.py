@@ -170,3 +170,53 @@ def test_dev_gemm_all_ops(gpu_lib, ta, tb, dtype, m, n, k):
     eps = np.finfo(dtype).eps
     bound = 4 * k * eps * (np.abs(opA) @ np.abs(opB)).max() + 4 * eps * np.abs(C0).max()
     assert np.abs(got - want).max() <= bound
+
+
+def bf16_round(x: np.ndarray) -> np.ndarray:
+    """float32 -> nearest bf16 (ties to even), returned as float64."""
+    u = np.asarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("T", "N"), ("N", "T"), ("T", "T")])
+@pytest.mark.parametrize("m,n,k", [(33, 47, 29), (300, 130, 700), (1000, 544, 2048)])
+def test_gemm_bf16_matches_emulation(gpu_lib, ta, tb, m, n, k):
+    """bf16 products with fp32 accumulation: equal to the exact product of the
+    bf16-rounded operands up to fp32 accumulation error."""
+    rng = np.random.default_rng(m + n + k)
+    A = rng.standard_normal((k, m) if ta == "T" else (m, k)).astype(np.float32)
+    B = rng.standard_normal((n, k) if tb == "T" else (k, n)).astype(np.float32)
+    C0 = rng.standard_normal((m, n)).astype(np.float32)
+    dA, dB, dC = (gpu_lib.DeviceMatrix.from_host(x) for x in (A, B, C0))
+    gpu_lib.dev_gemm(ta, tb, dA, dB, dC, alpha=1.0, beta=0.5, mode="bf16")
+    got = dC.to_host().astype(np.float64)
+    opA = bf16_round(A.T if ta == "T" else A)
+    opB = bf16_round(B.T if tb == "T" else B)
+    want = opA @ opB + 0.5 * C0.astype(np.float64)
+    bound = 2 * k * 2.0**-24 * (np.abs(opA) @ np.abs(opB)).max() + 1e-6 * np.abs(C0).max()
+    assert np.abs(got - want).max() <= bound
+
+
+@pytest.mark.parametrize("ta,tb", [("T", "N"), ("N", "N")])
+def test_gemm_bf16x3_is_fp32_accurate(gpu_lib, ta, tb):
+    rng = np.random.default_rng(11)
+    m, n, k = 520, 96, 4096
+    A = rng.standard_normal((k, m) if ta == "T" else (m, k)).astype(np.float32)
+    B = rng.standard_normal((n, k) if tb == "T" else (k, n)).astype(np.float32)
+    dA, dB = gpu_lib.DeviceMatrix.from_host(A), gpu_lib.DeviceMatrix.from_host(B)
+    dC = gpu_lib.DeviceMatrix(m, n, np.float32)
+    gpu_lib.dev_gemm(ta, tb, dA, dB, dC, mode="bf16x3")
+    got = dC.to_host().astype(np.float64)
+    opA = (A.T if ta == "T" else A).astype(np.float64)
+    opB = (B.T if tb == "T" else B).astype(np.float64)
+    want = opA @ opB
+    scale = (np.abs(opA) @ np.abs(opB)).max()
+    err = np.abs(got - want).max() / scale
+    # split-bf16 carries ~16 bits per operand; fp32 accumulation over k
+    assert err < 2e-5, err
+    # and clearly better than plain bf16
+    dC2 = gpu_lib.DeviceMatrix(m, n, np.float32)
+    gpu_lib.dev_gemm(ta, tb, dA, dB, dC2, mode="bf16")
+    err1 = np.abs(dC2.to_host() - want).max() / scale
+    assert err < err1 / 20
