@@ -85,6 +85,92 @@ def _timed(lib, rp, step, steps, warmup, tags):
     return ms / steps, ks
 
 
+def bench_batched(args, rp, lib) -> int:
+    """4096 float32 matrices, m log-uniform in [64, 2048], n log-uniform in
+    [32, min(m, 1024)], sigma_j = exp(-j / tau), tau ~ U[4, 64], Haar factors;
+    k = min(32, n / 2), p = 8, q = 2; one call of rsvd_dev_fixed_rank_batched
+    per step (a replayed HIP graph)."""
+    log = lambda s: print("[bench] " + s, file=sys.stderr, flush=True)  # noqa: E731
+    rng = np.random.default_rng(4096)
+    cnt, p, q = args.batch, 8, 2
+    m = np.exp(rng.uniform(np.log(64), np.log(2048), cnt)).astype(np.int32)
+    n = np.array([int(np.exp(rng.uniform(np.log(32), np.log(min(mi, 1024))))) for mi in m],
+                 dtype=np.int32)
+    k = np.minimum(32, n // 2).astype(np.int32)
+    tau = rng.uniform(4, 64, cnt)
+    b = rp.Batch(m, n, k, p)
+    log(f"generating {cnt} matrices ({int((m.astype(np.int64) * n).sum())} elements)")
+    b.fill_test_matrices(tau, 4096)
+    seed = 5000
+    t0 = time.perf_counter()
+    b.run(q, seed)  # plan: tune + capture
+    log(f"first call (tuning + graph capture): {time.perf_counter() - t0:.2f} s; "
+        f"{rp.batched_last_config()}")
+    for _ in range(args.warmup):
+        b.run(q, seed)
+    lib.rsvd_sync()
+    lib.rsvd_kstats_reset()
+    lib.rsvd_kstats_enable(1)
+    lib.rsvd_timer_start()
+    for _ in range(args.steps):
+        b.run(q, seed)
+    ms = lib.rsvd_timer_stop_ms() / args.steps
+    lib.rsvd_kstats_enable(0)
+    mn = (m.astype(np.float64) * n).sum()
+    l = (k + p).astype(np.float64)
+    flops = float((2.0 * m * n * l * 2 * q).sum())
+    bytes_alg = 4.0 * mn * 2 * q  # A read once per pass, 2q passes
+    # accuracy against the FP64 plain device path on a sample (same Omega)
+    res = b.results()
+    sample = list(range(0, cnt, max(1, cnt // 32)))
+    worst = 0.0
+    for i in sample:
+        Ai = b.matrix(i).astype(np.float64)
+        dA = rp.DeviceMatrix.from_host(Ai)
+        _, s64, _ = rp.dev_fixed_rank(dA, int(k[i]), p, 2, q, 1, seed + i)
+        worst = max(worst, float(np.abs(res[i][1] - s64).max()))
+    cpu = None
+    if not args.no_cpu:
+        from oracle import ref
+        threads = args.cpu_threads or min(physical_cores(), 16)
+        ref.load(threads)
+        mats = [(i, b.matrix(i).astype(np.float64)) for i in range(0, cnt, 16)]
+        t0 = time.perf_counter()
+        done = 0
+        for i, Ai in mats:
+            ni, li = int(n[i]), int(k[i]) + p
+            om = rp.gaussian(seed + i, 0, ni * li).reshape(li, ni).T
+            ref.fixed_rank(Ai, int(k[i]), p, 2, q, 1, omega=om)
+            done += 1
+            if time.perf_counter() - t0 > 20:
+                break
+        cs = time.perf_counter() - t0
+        cpu = {"value": round(done / cs, 2), "unit": "solves/s", "cores": threads,
+               "kind": "port", "sample": f"oracle (FP64, {ref.backend().split(':')[0]}) one call "
+                                         f"per matrix on {done} matrices of the batch "
+                                         f"(every 16th), same Omega"}
+    line = {
+        "metric": "batched rSVD solves/s (4096 uneven fp32 matrices, one HIP graph)",
+        "value": round(cnt / (ms * 1e-3), 1), "unit": "solves/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16x3",
+        "data": "synthetic",
+        "config": {"workload": f"batched fixed-rank rSVD, {cnt} matrices m in [64,2048], n in "
+                               f"[32,min(m,1024)] log-uniform, sigma_j=exp(-j/tau), "
+                               f"k=min(32,n/2) p=8 q=2", "count": cnt},
+        "tflops": round(flops / (ms * 1e-3) / 1e12, 2),
+        "roofline": {"bound": "hbm", "achieved": round(bytes_alg / (ms * 1e-3) / 1e9, 1),
+                     "peak": 8000.0, "unit": "GB/s",
+                     "frac": round(bytes_alg / (ms * 1e-3) / 1e9 / 8000.0, 4),
+                     "traffic": None, "kernel": "whole graph (A read once per pass, 2q passes)"},
+        "error_vs_fp64": {"max_abs_sigma_diff_sigma0_1": worst, "sample": len(sample)},
+        "tuner": rp.batched_last_config(),
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -100,6 +186,14 @@ def main() -> int:
     ap.add_argument("--fp64-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-rows", type=int, default=16384,
+                    help="rows of A in the CPU baseline's sample (bounds it to ~10-30 s)")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="run one plain device copy of this size first (HBM counter calibration)")
+    ap.add_argument("--workload", default="fixed_rank", choices=["fixed_rank", "batched"],
+                    help="fixed_rank: the headline 65536x16384 problem; batched: 4096 uneven "
+                         "matrices through rsvd_dev_fixed_rank_batched")
+    ap.add_argument("--batch", type=int, default=4096)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -111,6 +205,8 @@ def main() -> int:
     lib = _setup_lib(rp)
     if lib.rsvd_init() != 0:
         raise SystemExit("GPU init failed: " + rp.last_error())
+    if args.workload == "batched":
+        return bench_batched(args, rp, lib)
     m, n, r, k, p, q = args.m, args.n, args.rank, args.k, args.p, args.q
     l = k + p
     seed_a, seed_om = 20260101, 777
@@ -132,10 +228,22 @@ def main() -> int:
                                    S64.ptr, V64.ptr, n):
             raise SystemExit("rsvd_dev_fixed_rank: " + rp.last_error())
 
+    if args.calib_copy_mib:
+        # a plain streaming copy of known size (stream_copy_kernel), so that
+        # rocprofv3's FETCH_SIZE / WRITE_SIZE of the A-streaming kernels can be
+        # scaled to bytes
+        nbytes = min(args.calib_copy_mib << 20, 4 * m * n)
+        scratch = rp.DeviceMatrix(nbytes // 4, 1, np.float32)
+        if lib.rsvd_dev_copy(A32.ptr, scratch.ptr, nbytes):
+            raise SystemExit("rsvd_dev_copy: " + rp.last_error())
+        del scratch
+        log(f"calibration copy: {nbytes >> 20} MiB read + written")
     n64 = args.steps if args.precision == "fp64" else args.fp64_steps
-    ms64, ks64 = _timed(lib, rp, step64, n64, 1, ["apass_f64", "jacobi_round"])
-    log(f"fp64 plain: {ms64:.2f} ms/step")
-    s64 = S64.to_host()[:, 0]
+    ms64, ks64, s64 = None, None, None
+    if n64 > 0:
+        ms64, ks64 = _timed(lib, rp, step64, n64, 1, ["apass_f64", "jacobi_round"])
+        log(f"fp64 plain: {ms64:.2f} ms/step")
+        s64 = S64.to_host()[:, 0]
 
     # ---- mixed precision ----
     prec_id = {"bf16": 1, "bf16x3": 2}.get(args.precision)
@@ -160,10 +268,12 @@ def main() -> int:
     else:
         ms, ks, tag, smp = ms64, ks64, "apass_f64", s64
         res_mp = None
-    res = C.c_double(0)
-    lib.rsvd_dev_lowrank_residual(A64.ptr, m, n, m, k, U64.ptr, m, S64.ptr, V64.ptr, n, 0,
-                                  C.byref(res))
-    res_64 = res.value
+    res_64 = None
+    if ms64 is not None:
+        res = C.c_double(0)
+        lib.rsvd_dev_lowrank_residual(A64.ptr, m, n, m, k, U64.ptr, m, S64.ptr, V64.ptr, n, 0,
+                                      C.byref(res))
+        res_64 = res.value
     value = F / (ms * 1e-3) / 1e12
 
     # ---- roofline of the dominant A-streaming kernel ----
@@ -189,7 +299,7 @@ def main() -> int:
 
     # ---- error vs the FP64 plain method (same Omega) ----
     err = None
-    if prec_id:
+    if prec_id and s64 is not None:
         rel = np.abs(smp - s64) / s64
         err = {"sigma_rel_max": float(rel.max()), "sigma_rel_top64": float(rel[-64:].max()),
                "residual_fp64": res_64, "residual_mp": res_mp,
@@ -200,18 +310,18 @@ def main() -> int:
         from oracle import ref
         threads = args.cpu_threads or min(physical_cores(), 16)
         ref.load(threads)
-        log(f"cpu baseline: oracle on {threads} threads ({ref.backend()})")
-        A_host = A64.to_host()
+        ms_ = min(args.cpu_rows, m)
+        log(f"cpu baseline: oracle on {threads} threads ({ref.backend()}), rows 0..{ms_ - 1}")
+        A_host = np.asfortranarray(A64.to_host()[:ms_])
         Om = rp.gaussian(seed_om, 0, n * l).reshape(l, n).T
         t0 = time.perf_counter()
-        Ur, Sr, Vr = ref.fixed_rank(A_host, k, p, 2, q, 1, omega=Om)
+        ref.fixed_rank(A_host, k, p, 2, q, 1, omega=Om)
         cpu_s = time.perf_counter() - t0
-        srel = float(np.max(np.abs(np.diag(Sr) - s64) / np.diag(Sr)))
-        cpu = {"value": round(F / cpu_s / 1e12, 4), "unit": "TFLOP/s", "cores": threads,
-               "kind": "port", "seconds": round(cpu_s, 3),
-               "sample": f"full workload, one call ({m}x{n}, k={k}, p={p}, q={q}) in FP64 on "
-                         f"{ref.backend().split(':')[0]}",
-               "fp64_gpu_vs_cpu_sigma_rel_max": srel}
+        cpu = {"value": round(falg(ms_, n, l, q) / cpu_s / 1e12, 4), "unit": "TFLOP/s",
+               "cores": threads, "kind": "port", "seconds": round(cpu_s, 3),
+               "sample": f"one call of the oracle (FP64, {ref.backend().split(':')[0]}) on rows "
+                         f"0..{ms_ - 1} of the same A ({ms_}x{n}), k={k}, p={p}, q={q}, same "
+                         f"Omega; rate = F_alg({ms_},{n},{l},{q}) / seconds"}
         del A_host
 
     line = {
@@ -231,9 +341,10 @@ def main() -> int:
                                f"k={k} p={p} q={q} s=1 vnum=2",
                    "m": m, "n": n, "k": k, "p": p, "q": q, "precision": args.precision},
         "solves_per_s": round(1000.0 / ms, 3),
-        "fp64_plain": {"ms_per_step": round(ms64, 3), "tflops": round(F / (ms64 * 1e-3) / 1e12, 3),
-                       "solves_per_s": round(1000.0 / ms64, 3)},
-        "speedup_vs_fp64_plain": round(ms64 / ms, 3),
+        "fp64_plain": None if ms64 is None else {
+            "ms_per_step": round(ms64, 3), "tflops": round(F / (ms64 * 1e-3) / 1e12, 3),
+            "solves_per_s": round(1000.0 / ms64, 3)},
+        "speedup_vs_fp64_plain": None if ms64 is None else round(ms64 / ms, 3),
         "error_vs_fp64": err,
         "falg_flops": F,
         "kernel_ms_per_step": breakdown,

@@ -106,6 +106,82 @@ int rsvd_dev_gemm(int dtype, char ta, char tb, long long M, long long N, long lo
                   const void *A, long long lda, const void *B, long long ldb, double beta, void *C,
                   long long ldc);
 
+/* Batched fixed-rank rSVD (vnum = 2 semantics) of `count` independent float32
+ * device matrices.  Matrix i is m[i] x n[i], column-major with lda = m[i], at
+ * dA + a_off[i] (elements); rank k[i], oversampling p, q as in
+ * low_rank_svd_rand_decomp_fixed_rank, with 1 <= k[i], k[i] + p <= min(m[i],
+ * n[i]) and k[i] + p <= 48.  Omega_i is the seeded Gaussian stream with seed
+ * seed + i (the same Omega rsvd_dev_fixed_rank draws with that seed).
+ * Outputs per matrix: dU + u_off[i] (m[i] x k[i] float, ld m[i]), dS + s_off[i]
+ * (k[i] doubles, ascending as vnum = 2 returns them), dV + v_off[i] (n[i] x
+ * k[i] float, ld n[i]).  The passes over A run on the split-bf16 matrix cores
+ * (fp32 accuracy); Gram matrices, Cholesky factors and the small eigenproblems
+ * in FP64.  The first call for a batch layout tunes the kernels on that batch
+ * and captures the call as a HIP graph; later calls with the same layout and
+ * pointers replay it.  *status (optional) is nonzero when a Cholesky pivot of
+ * some matrix was not positive (its sketch was numerically rank deficient).
+ * Host arrays; device buffers as elsewhere. */
+int rsvd_dev_fixed_rank_batched(int count, const int *m, const int *n, const int *k, int p, int q,
+                                unsigned long long seed, const float *dA, const long long *a_off,
+                                float *dU, const long long *u_off, double *dS,
+                                const long long *s_off, float *dV, const long long *v_off,
+                                int *status);
+
+/* Kernel variants the batched autotuner picked for the most recent new batch
+ * layout, with the timings it measured (a static string). */
+const char *rsvd_batched_last_config(void);
+
+/* Test batch for the batched path: matrix i (m[i] x n[i], float, at dA +
+ * a_off[i], lda m[i]) = U diag(sigma) V^T with Haar U, V from the seeded
+ * stream (seed + i) and sigma_j = exp(-j / tau[i]) for j < min(m, n, 19 tau),
+ * zero beyond (below float resolution relative to sigma_0 = 1). */
+int rsvd_dev_batch_test_matrices(int count, const int *m, const int *n, const double *tau,
+                                 unsigned long long seed, float *dA, const long long *a_off);
+
+/* ---- learned (q, rank) front end (no counterpart in the reference) ----
+ * A 64-128-128-128-6 perceptron (GELU; fp32 parameters packed W1 b1 W2 b2 W3 b3
+ * W4 b4, weights [out][in] row-major) maps the 64 sketch features of a matrix
+ * to logits of q in {0..4} (outputs 0..4) and rank / min(m, n) (output 5). */
+int rsvd_mlp_num_params(void);
+/* He initialisation from the seeded stream (biases zero). */
+int rsvd_dev_mlp_init(unsigned long long seed, float *dTheta);
+/* dOut (batch x 6, row-major) = network(dX) for dX batch x 64 (row-major). */
+int rsvd_dev_mlp_forward(const float *dTheta, const float *dX, int batch, float *dOut);
+/* Gradient of the mean loss logsumexp(o[0:5]) - o[q] + lam (o[5] - r)^2 over
+ * the batch into dGrad (num_params); per-sample losses into dLoss (batch). */
+int rsvd_dev_mlp_grad(const float *dTheta, const float *dX, const int *dq, const float *dr,
+                      float lam, int batch, float *dGrad, float *dLoss);
+/* Adam training over shuffled mini-batches of the n samples; loss_hist (host,
+ * optional) receives the mean loss of each epoch. */
+int rsvd_dev_mlp_train(float *dTheta, const float *dX, const int *dq, const float *dr, int n,
+                       float lam, int epochs, int batch, float lr, unsigned long long seed,
+                       float *loss_hist);
+/* The 64 features of a float matrix (m, n >= 64): singular values s_j of a
+ * 64-column bf16 sketch Y = A Omega (seeded stream), x_0 = log(||Y||_F / (8 s_0)),
+ * x_j = log(s_j / s_0). */
+int rsvd_dev_sketch_features(const float *dA, long long m, long long n, long long lda,
+                             unsigned long long seed, float *dFeat);
+/* TOL-driven low-rank approximation with the learned front end: predicts
+ * (q, rank) from the sketch features of dA32 (the float copy of dA), runs one
+ * FP64 fixed-rank rSVD of dA (vnum = 2, oversampling p, q_pred + 1, i.e. q_pred
+ * power iterations, k = ceil(rank (1 + margin)) + 2), and accepts it when its
+ * FP64 residual ||A - U S V^T||_F = sqrt(||A||_F^2 - sum sigma^2) is at most
+ * tol_rel ||A||_F; otherwise it falls back to
+ * low_rank_svd_blockrand_decomp_fixed_rank_or_prec in TOL mode (TOL = tol_rel
+ * ||A||_F, kstep, q_fallback).  Outputs as rsvd_dev_blockrand; *q_pred, *k_pred
+ * the prediction, *fell_back 1 when the fallback ran, *resid_rel the relative
+ * residual of the accepted result. */
+int rsvd_dev_learned_lowrank(const float *dTheta, const double *dA, const float *dA32,
+                             long long m, long long n, long long lda, double tol_rel, int p,
+                             double margin, int kstep, int q_fallback, unsigned long long seed,
+                             int *frank, double **dU, double **dS, double **dV, int *q_pred,
+                             int *k_pred, int *fell_back, double *resid_rel);
+
+/* Plain device-to-device copy of `bytes` bytes (a multiple of 16; both
+ * pointers 16-byte aligned) by a streaming kernel: exactly `bytes` read and
+ * written.  Used to calibrate HBM byte counters against the library's kernels. */
+int rsvd_dev_copy(const void *src, void *dst, long long bytes);
+
 /* Synthetic test matrix in HBM: A = U diag(sigma) V^T + noise_std * N(0,1)
  * with Haar U (m x r), V (n x r) drawn from the seeded stream; sigma is a host
  * array of r values.  dA64 (double) and / or dA32 (float) may be NULL; when

@@ -1,0 +1,294 @@
+// apass.hip -- the A-streaming passes of the mixed-precision rSVD on the bf16
+// matrix cores: C = alpha * op(A) * S + beta * C with A the large fp32 matrix
+// (m x n, column-major, never copied) and S a skinny fp32 panel (K x N, N up to
+// a few hundred): Y = A Omega and Y = A Z (op = N), Z = A^T Q and B^T = A^T Q
+// (op = T) -- the dominant cost of rra.c:96-131 in the reference.
+//
+// Design for a skinny N: one workgroup owns 64 rows of op(A) and ALL the
+// columns of C (up to 576 = 4 waves x 9 tiles of 16), so A is read from HBM
+// exactly once per pass; only the skinny operand is re-read, and it is shared
+// by every workgroup at the same k, so it comes from L2.
+//   * S is first rounded to bf16 (and, for split-bf16, its residual lo =
+//     bf16(s - hi)) and re-laid out in MFMA fragment order: for every k block
+//     of 32 and n tile of 16 a contiguous 1 KiB block whose 64 lanes each hold
+//     8 consecutive k of one column.  A wave then loads a B fragment with one
+//     coalesced 16-byte load per lane, prefetched one k block ahead.
+//   * A's 64 x 32 tile of a k block is loaded into registers three blocks
+//     ahead (16-byte loads when op(A) is k-contiguous, 4-byte loads along the
+//     contiguous m otherwise -- both coalesced), rounded to bf16 (hi, lo) and
+//     written to LDS in fragment order, three stages, one barrier per block;
+//     each wave reads its A fragments one block ahead.
+//   * v_mfma_f32_16x16x32_bf16 with srcA = the S fragment and srcB = the A
+//     fragment, so the accumulator lanes run along m and C's column-major
+//     stores are coalesced.  Split-bf16 (SPLIT = 3) adds hi*lo and lo*hi.
+// One workgroup of 4 waves per CU (launch bounds 256, 1) leaves each wave
+// the full register file: 144 accumulators, double-buffered S and A
+// fragments (the k loop is unrolled by two so the buffers swap by name).
+#include <hip/hip_bf16.h>
+
+#include "rsvd_internal.h"
+
+namespace rsvd {
+
+namespace {
+
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float f4v_t __attribute__((ext_vector_type(4)));
+typedef float f4s_t __attribute__((ext_vector_type(4)));
+
+constexpr int kWaves = 4;
+constexpr int kThreads = 64 * kWaves;
+constexpr int kBM = 64;               // rows of op(A) per workgroup
+constexpr int kNTW = 9;               // n tiles (16 columns) per wave
+constexpr int kBNT = kWaves * kNTW;   // n tiles per workgroup (576 columns)
+
+// St[((p * KB + kb) * NTP + nt) * 64 + lane][e] = plane p of
+// S(32 kb + 8 (lane / 16) + e, 16 nt + lane % 16), zero outside K x N.
+__global__ void tile_skinny_kernel(int64_t K, int64_t N, const float* __restrict__ S, int64_t lds,
+                                   int64_t KB, int64_t NTP, int planes,
+                                   bf16x8_t* __restrict__ St) {
+  const int64_t total = KB * NTP * 64;
+  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int lane = (int)(idx & 63);
+    const int64_t t = idx >> 6;
+    const int64_t nt = t % NTP, kb = t / NTP;
+    const int64_t n = nt * 16 + (lane & 15);
+    const int64_t k0 = kb * 32 + 8 * (lane >> 4);
+    bf16x8_t h, l;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int64_t k = k0 + e;
+      const float v = (n < N && k < K) ? S[k + n * lds] : 0.0f;
+      const __bf16 hb = (__bf16)v;
+      h[e] = hb;
+      l[e] = (__bf16)(v - (float)hb);
+    }
+    St[t * 64 + lane] = h;
+    if (planes == 2) St[(KB * NTP + t) * 64 + lane] = l;
+  }
+}
+
+// K must be a multiple of 32 here (the caller adds the remainder with a
+// second product).  Rows past M are clamped to row M - 1 for the loads (their
+// results are never stored) and the S tiles past N are zero, so the k loop
+// has no per-lane predicates and no branches: the memory-counter waits the
+// compiler inserts can then leave the prefetches of the next k block in
+// flight across the MFMAs of the current one.
+template <bool TA, int SPLIT, bool VEC>
+__global__ void __launch_bounds__(kThreads, 1)
+apass_kernel(int64_t M, int64_t N, int64_t K, float alpha, const float* __restrict__ A,
+             int64_t lda, const bf16x8_t* __restrict__ St, int64_t KB, int64_t NTP, float beta,
+             float* __restrict__ C, int64_t ldc) {
+  constexpr int NP = (SPLIT == 3) ? 2 : 1;
+  constexpr int MT = kBM / 16;
+  // three A stages: block kb + 2 is written while block kb + 1 is read into
+  // registers and block kb is consumed
+  __shared__ bf16x8_t As[3][NP][MT][64];
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int64_t m0 = (int64_t)blockIdx.x * kBM;
+  const int64_t ntb = (int64_t)blockIdx.y * kBNT + wave * kNTW;  // this wave's first n tile
+
+  // A staging: thread -> row r of the tile, k group g (8 consecutive k)
+  const int r = TA ? (tid >> 2) : (tid & 63);
+  const int g = TA ? (tid & 3) : (tid >> 6);
+  const int64_t gm = (m0 + r < M) ? m0 + r : M - 1;
+  const float* Arow = TA ? A + gm * lda + 8 * g : A + gm + (int64_t)(8 * g) * lda;
+  const int64_t last = KB - 1;
+  float av[8];
+  auto loadA = [&](int64_t kb) {
+    kb = kb < last ? kb : last;
+    if (TA) {
+      const float* p = Arow + kb * 32;
+      if (VEC) {
+        const f4s_t x = *reinterpret_cast<const f4s_t*>(p);
+        const f4s_t y = *reinterpret_cast<const f4s_t*>(p + 4);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          av[e] = x[e];
+          av[4 + e] = y[e];
+        }
+      } else {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) av[e] = p[e];
+      }
+    } else {
+      const float* p = Arow + kb * 32 * lda;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) av[e] = p[e * lda];
+    }
+  };
+  auto storeA = [&](int64_t kb) {
+    const int st = (int)(kb % 3);
+    bf16x8_t h, l;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const __bf16 hb = (__bf16)av[e];
+      h[e] = hb;
+      if (SPLIT == 3) l[e] = (__bf16)(av[e] - (float)hb);
+    }
+    const int mt = r >> 4, ln = (r & 15) + 16 * g;
+    As[st][0][mt][ln] = h;
+    if (SPLIT == 3) As[st][NP - 1][mt][ln] = l;
+  };
+  auto readA = [&](int64_t kb, bf16x8_t* h, bf16x8_t* l) {
+    const int st = (int)(kb % 3);
+#pragma unroll
+    for (int i = 0; i < MT; ++i) {
+      h[i] = As[st][0][i][lane];
+      if (SPLIT == 3) l[i] = As[st][NP - 1][i][lane];
+    }
+  };
+  const bf16x8_t* Sh = St + ntb * 64 + lane;
+  const bf16x8_t* Sl = St + (KB * NTP + ntb) * 64 + lane;
+  auto loadS = [&](int64_t kb, bf16x8_t* h, bf16x8_t* l) {
+    kb = kb < last ? kb : last;
+#pragma unroll
+    for (int j = 0; j < kNTW; ++j) {
+      h[j] = Sh[(kb * NTP + j) * 64];
+      if (SPLIT == 3) l[j] = Sl[(kb * NTP + j) * 64];
+    }
+  };
+
+  f4v_t acc[kNTW][MT];
+#pragma unroll
+  for (int j = 0; j < kNTW; ++j)
+#pragma unroll
+    for (int i = 0; i < MT; ++i) acc[j][i] = f4v_t{0.f, 0.f, 0.f, 0.f};
+
+  // one k block: prefetch S and the A fragments of block kb + 1 (issued
+  // first and kept ahead of the MFMAs by a scheduling barrier), MFMAs on
+  // block kb, stage A block kb + 2, fetch A block kb + 3
+  auto step = [&](int64_t kb, bf16x8_t* sch, bf16x8_t* scl, bf16x8_t* ach, bf16x8_t* acl,
+                  bf16x8_t* snh, bf16x8_t* snl, bf16x8_t* anh, bf16x8_t* anl) {
+    loadS(kb + 1, snh, snl);
+    readA(kb + 1, anh, anl);
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int j = 0; j < kNTW; ++j)
+#pragma unroll
+      for (int i = 0; i < MT; ++i)
+        acc[j][i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(sch[j], ach[i], acc[j][i], 0, 0, 0);
+    if (SPLIT == 3) {
+#pragma unroll
+      for (int j = 0; j < kNTW; ++j)
+#pragma unroll
+        for (int i = 0; i < MT; ++i)
+          acc[j][i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(sch[j], acl[i], acc[j][i], 0, 0, 0);
+#pragma unroll
+      for (int j = 0; j < kNTW; ++j)
+#pragma unroll
+        for (int i = 0; i < MT; ++i)
+          acc[j][i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(scl[j], ach[i], acc[j][i], 0, 0, 0);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    storeA(kb + 2);
+    loadA(kb + 3);
+    __syncthreads();
+  };
+
+  bf16x8_t s0h[kNTW], s0l[kNTW], s1h[kNTW], s1l[kNTW];
+  bf16x8_t a0h[MT], a0l[MT], a1h[MT], a1l[MT];
+  loadA(0);
+  storeA(0);
+  loadA(1);
+  storeA(1);
+  loadA(2);
+  __syncthreads();
+  readA(0, a0h, a0l);
+  loadS(0, s0h, s0l);
+  int64_t kb = 0;
+  for (; kb + 1 < KB; kb += 2) {
+    step(kb, s0h, s0l, a0h, a0l, s1h, s1l, a1h, a1l);
+    step(kb + 1, s1h, s1l, a1h, a1l, s0h, s0l, a0h, a0l);
+  }
+  if (kb < KB) step(kb, s0h, s0l, a0h, a0l, s1h, s1l, a1h, a1l);
+
+  // D[n][m] of tile (j, i): lane holds n = 4 (lane / 16) + rr, m = lane % 16
+#pragma unroll
+  for (int j = 0; j < kNTW; ++j) {
+#pragma unroll
+    for (int i = 0; i < kBM / 16; ++i) {
+      const int64_t m = m0 + i * 16 + (lane & 15);
+#pragma unroll
+      for (int rr = 0; rr < 4; ++rr) {
+        const int64_t n = (ntb + j) * 16 + 4 * (lane >> 4) + rr;
+        if (m < M && n < N) {
+          float* cp = C + m + n * ldc;
+          float v = alpha * acc[j][i][rr];
+          if (beta != 0.0f) v += beta * (*cp);
+          *cp = v;
+        }
+      }
+    }
+  }
+}
+
+}  // namespace
+
+bool apass_suitable(Context* c, int64_t M, int64_t N) {
+  const int64_t gx = (M + kBM - 1) / kBM;
+  const int64_t gy = ((N + 15) / 16 + kBNT - 1) / kBNT;
+  const int64_t cus = c->num_cus > 0 ? c->num_cus : 256;
+  return gx * gy >= cus && gy <= 65535;
+}
+
+int apass_bf16(Context* c, const char* tag, bool ta, int64_t M, int64_t N, int64_t K, float alpha,
+               const float* A, int64_t lda, const float* S, int64_t lds, float beta, float* C,
+               int64_t ldc, int split) {
+  if (M < 0 || N < 0 || K < 0) return set_error("apass: negative dimension");
+  if (split != 1 && split != 3) return set_error("apass: split must be 1 or 3");
+  if (M == 0 || N == 0) return 0;
+  const int64_t K32 = K - K % 32;  // streamed here; the remainder below
+  if (K32 == 0)
+    return gemm_bf16_general(c, tag, ta ? 'T' : 'N', 'N', M, N, K, alpha, A, lda, S, lds, beta, C,
+                             ldc, split);
+  const int64_t KB = K32 / 32;
+  const int64_t gx = (M + kBM - 1) / kBM;
+  const int64_t gy = ((N + 15) / 16 + kBNT - 1) / kBNT;
+  if (gy > 65535 || gx > 0x7fffffff) return set_error("apass: grid too large");
+  const int64_t NTP = gy * kBNT;
+  const int planes = (split == 3) ? 2 : 1;
+  DBuf<bf16x8_t> St;
+  RSVD_TRY(St.alloc((size_t)(planes * KB * NTP * 64)));
+  {
+    const int64_t total = KB * NTP * 64;
+    int64_t blocks = (total + 255) / 256;
+    if (blocks > 16384) blocks = 16384;
+    hipLaunchKernelGGL(tile_skinny_kernel, dim3((unsigned)blocks), dim3(256), 0, c->stream, K32,
+                       N, S, lds, KB, NTP, planes, St.p);
+    RSVD_LAUNCH_CHECK();
+  }
+  const bool vec = ta && (reinterpret_cast<uintptr_t>(A) % 16 == 0) && (lda % 4 == 0);
+  dim3 grid((unsigned)gx, (unsigned)gy);
+  {
+    KTimer kt(c, tag);
+#define RSVD_APASS(TA_, S_, V_)                                                               \
+  hipLaunchKernelGGL((apass_kernel<TA_, S_, V_>), grid, dim3(kThreads), 0, c->stream, M, N, K32, \
+                     alpha, A, lda, (const bf16x8_t*)St.p, KB, NTP, beta, C, ldc)
+    if (ta) {
+      if (split == 3) {
+        if (vec) RSVD_APASS(true, 3, true);
+        else RSVD_APASS(true, 3, false);
+      } else {
+        if (vec) RSVD_APASS(true, 1, true);
+        else RSVD_APASS(true, 1, false);
+      }
+    } else {
+      if (split == 3) RSVD_APASS(false, 3, false);
+      else RSVD_APASS(false, 1, false);
+    }
+#undef RSVD_APASS
+    RSVD_LAUNCH_CHECK();
+  }
+  if (K32 < K) {
+    // the last K % 32 columns of op(A): C += alpha op(A)(:, K32:) S(K32:, :)
+    const float* Ar = ta ? A + K32 : A + K32 * lda;
+    RSVD_TRY(gemm_bf16_general(c, tag, ta ? 'T' : 'N', 'N', M, N, K - K32, alpha, Ar, lda,
+                               S + K32, lds, 1.0f, C, ldc, split));
+  }
+  return 0;
+}
+
+}  // namespace rsvd

@@ -1,0 +1,788 @@
+// batched.hip -- fixed-rank randomized SVD of thousands of independent,
+// unevenly sized fp32 matrices in one pass over the GPU (the batched use of
+// low_rank_svd_rand_decomp_fixed_rank, rra.c:73-234, vnum = 2: the reference
+// would loop over the matrices one call at a time).
+//
+// Every stage of the algorithm is one launch over the whole batch; a call is
+// a fixed sequence of about a dozen launches, captured once into a hipGraph
+// per batch layout and replayed:
+//   bomega   Omega_i (n_i x l_i) from the seeded stream, seed + i, into Z_i
+//   btile    the skinny operand (Omega / Z / Q, K x 48) rounded to split bf16
+//            (hi, lo) in MFMA fragment order (as in apass.hip)
+//   bpass    Y_i = A_i X_i or Z_i = A_i^T Q_i: one wave per 16-row (times RT)
+//            tile of one matrix, split-bf16 matrix cores with fp32
+//            accumulation, A_i streamed once; work items of all matrices in
+//            one grid, largest matrices first
+//   borth    CholeskyQR2 of Y_i (m_i x 48) per matrix: FP64 Gram, FP64
+//            Cholesky in LDS (first pass shifted by 1e-14 tr G so it never
+//            breaks down), Y <- Y R^-1, twice
+//   bfinal   G = B B^T (FP64, from B^T = A^T Q), two-sided Jacobi in LDS,
+//            ascending eigenvalues, sigma = sqrt(lambda), last k kept
+//            (rra.c:175-225), U = Q Uhat, V = B^T Uhat Sigma^-1
+// Sketch widths are padded to 48 columns (three 16-wide tiles; l_i = k_i + p
+// <= 48), padding columns are zero throughout.  The pass kernels have
+// variants (rows per wave RT = 1, 2, 4) chosen per batch layout by timing
+// them on the batch itself on the first call (the autotuner); the choice is
+// cached with the graph.
+#include <hip/hip_bf16.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <numeric>
+#include <vector>
+
+#include "rng.h"
+#include "rsvd_internal.h"
+
+namespace rsvd {
+
+namespace {
+
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float f4v_t __attribute__((ext_vector_type(4)));
+typedef float f4s_t __attribute__((ext_vector_type(4)));
+
+constexpr int kL = 48;       // padded sketch width
+constexpr int kNT = kL / 16; // 16-wide column tiles
+constexpr int kLP = kL + 1;  // padded LDS row (doubles)
+
+struct BMat {
+  int64_t m, n;
+  int32_t k, l;
+  int64_t a;      // A_i offset in the input (floats), lda = m
+  int64_t y, z;   // Y_i (m x 48, ld m) and Z_i (n x 48, ld n) in the fp32 workspace
+  int64_t t;      // tiled skinny operand (bf16x8 units), 2 planes x KB x 3 x 64
+  int64_t kbmax;  // KB capacity of the tile slot
+  int64_t u, s, v;  // outputs: U (m x k, ld m), S (k), V (n x k, ld n)
+  uint64_t seed;
+  int32_t aligned;  // A_i rows 16-byte aligned (m % 4 == 0 and offset % 4 == 0)
+  int32_t pad;
+};
+
+struct BItem {
+  int32_t mat;
+  int32_t tile;
+};
+
+// ---------------------------------------------------------------- Omega
+__global__ void bomega_kernel(const BMat* __restrict__ mats, float* __restrict__ work) {
+  const BMat d = mats[blockIdx.x];
+  const int j = blockIdx.y;
+  float* Z = work + d.z + (int64_t)j * d.n;
+  for (int64_t r = threadIdx.x; r < d.n; r += blockDim.x)
+    Z[r] = (j < d.l) ? (float)gaussian_at(d.seed, (uint64_t)((int64_t)j * d.n + r)) : 0.0f;
+}
+
+// ---------------------------------------------------------------- tiling
+// item = (matrix, k block); 192 threads: tile nt = tid / 64, lane = tid % 64.
+template <bool FROM_Y>
+__global__ void __launch_bounds__(192)
+btile_kernel(const BMat* __restrict__ mats, const BItem* __restrict__ items,
+             const float* __restrict__ work, bf16x8_t* __restrict__ tiles) {
+  const BItem it = items[blockIdx.x];
+  const BMat d = mats[it.mat];
+  const int64_t K = FROM_Y ? d.m : d.n;
+  const float* X = work + (FROM_Y ? d.y : d.z);
+  const int64_t KB = (K + 31) / 32;
+  const int64_t kb = it.tile;
+  const int nt = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t col = nt * 16 + (lane & 15);
+  const int64_t k0 = kb * 32 + 8 * (lane >> 4);
+  bf16x8_t h, lo;
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    const int64_t k = k0 + e;
+    const float v = (k < K) ? X[k + col * K] : 0.0f;
+    const __bf16 hb = (__bf16)v;
+    h[e] = hb;
+    lo[e] = (__bf16)(v - (float)hb);
+  }
+  bf16x8_t* T = tiles + d.t;
+  T[(kb * kNT + nt) * 64 + lane] = h;
+  T[((KB + kb) * kNT + nt) * 64 + lane] = lo;
+}
+
+// ---------------------------------------------------------------- A passes
+// One wave per item: rows [16 RT tile, 16 RT (tile + 1)) of op(A_i) (M x K)
+// times the tiled skinny operand (K x 48) -> C (M x 48, ld M) in fp32.
+// TA: op(A) = A^T (M = n, K = m, k-contiguous); else M = m, K = n.
+template <bool TA, int RT>
+__global__ void __launch_bounds__(256)
+bpass_kernel(const BMat* __restrict__ mats, const BItem* __restrict__ items, int nitems,
+             const float* __restrict__ A, float* __restrict__ work,
+             const bf16x8_t* __restrict__ tiles) {
+  const int wid = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (wid >= nitems) return;
+  const int lane = threadIdx.x & 63;
+  const BItem it = items[wid];
+  const BMat d = mats[it.mat];
+  const int64_t M = TA ? d.n : d.m, K = TA ? d.m : d.n;
+  const int64_t lda = d.m;
+  const float* Ai = A + d.a;
+  const bf16x8_t* T = tiles + d.t;
+  const int64_t KB = (K + 31) / 32;
+  float* Cw = work + (TA ? d.z : d.y);
+  const int64_t r0 = (int64_t)it.tile * 16 * RT;
+  const bool vec = TA && d.aligned;
+
+  f4v_t acc[RT][kNT];
+#pragma unroll
+  for (int r = 0; r < RT; ++r)
+#pragma unroll
+    for (int t = 0; t < kNT; ++t) acc[r][t] = f4v_t{0.f, 0.f, 0.f, 0.f};
+
+  for (int64_t kb = 0; kb < KB; ++kb) {
+    bf16x8_t sh[kNT], sl[kNT];
+#pragma unroll
+    for (int t = 0; t < kNT; ++t) {
+      sh[t] = T[(kb * kNT + t) * 64 + lane];
+      sl[t] = T[((KB + kb) * kNT + t) * 64 + lane];
+    }
+    const int64_t k0 = kb * 32 + 8 * (lane >> 4);
+#pragma unroll
+    for (int r = 0; r < RT; ++r) {
+      const int64_t row = r0 + r * 16 + (lane & 15);
+      const bool rok = row < M;
+      float av[8];
+      if (TA) {
+        const float* p = Ai + k0 + row * lda;
+        if (vec && rok && k0 + 8 <= K) {
+          const f4s_t x = *reinterpret_cast<const f4s_t*>(p);
+          const f4s_t y = *reinterpret_cast<const f4s_t*>(p + 4);
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            av[e] = x[e];
+            av[4 + e] = y[e];
+          }
+        } else {
+#pragma unroll
+          for (int e = 0; e < 8; ++e) av[e] = (rok && k0 + e < K) ? p[e] : 0.0f;
+        }
+      } else {
+        const float* p = Ai + row + k0 * lda;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) av[e] = (rok && k0 + e < K) ? p[e * lda] : 0.0f;
+      }
+      bf16x8_t ah, al;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const __bf16 hb = (__bf16)av[e];
+        ah[e] = hb;
+        al[e] = (__bf16)(av[e] - (float)hb);
+      }
+#pragma unroll
+      for (int t = 0; t < kNT; ++t) {
+        acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(sh[t], ah, acc[r][t], 0, 0, 0);
+        acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(sh[t], al, acc[r][t], 0, 0, 0);
+        acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(sl[t], ah, acc[r][t], 0, 0, 0);
+      }
+    }
+  }
+  // D[n][m]: lane holds n = 16 t + 4 (lane / 16) + rr, m = lane % 16
+#pragma unroll
+  for (int r = 0; r < RT; ++r) {
+    const int64_t row = r0 + r * 16 + (lane & 15);
+    if (row >= M) continue;
+#pragma unroll
+    for (int t = 0; t < kNT; ++t)
+#pragma unroll
+      for (int rr = 0; rr < 4; ++rr) {
+        const int64_t col = t * 16 + 4 * (lane >> 4) + rr;
+        Cw[row + col * M] = acc[r][t][rr];
+      }
+  }
+}
+
+// ---------------------------------------------------------------- LDS helpers
+// Upper-triangular Gram G = X^T X (l x l of the first l columns, FP64
+// accumulation of fp32 products) of X (R x 48, ld R), staged in 64-row chunks.
+__device__ void lds_gram(const float* __restrict__ X, int64_t R, int l, float (*Xs)[kL + 1],
+                         double (*G)[kLP]) {
+  const int tid = threadIdx.x;
+  const int ne = l * (l + 1) / 2;
+  double acc[5] = {0, 0, 0, 0, 0};
+  int ei[5], ej[5];
+#pragma unroll
+  for (int u = 0; u < 5; ++u) {
+    int e = tid + 256 * u;
+    ei[u] = -1;
+    ej[u] = -1;
+    if (e < ne) {
+      // e -> (i, j), i <= j, row-major over the upper triangle
+      int i = 0;
+      while (e >= l - i) {
+        e -= l - i;
+        ++i;
+      }
+      ei[u] = i;
+      ej[u] = i + e;
+    }
+  }
+  for (int64_t r0 = 0; r0 < R; r0 += 64) {
+    const int rows = (int)((R - r0) < 64 ? (R - r0) : 64);
+    __syncthreads();
+    for (int e = tid; e < 64 * l; e += 256) {
+      const int rr = e & 63, c = e >> 6;
+      Xs[rr][c] = (rr < rows) ? X[r0 + rr + (int64_t)c * R] : 0.0f;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < 5; ++u) {
+      if (ei[u] < 0) continue;
+      const int i = ei[u], j = ej[u];
+      double s = 0.0;
+      for (int rr = 0; rr < rows; ++rr) s += (double)Xs[rr][i] * (double)Xs[rr][j];
+      acc[u] += s;
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < 5; ++u)
+    if (ei[u] >= 0) G[ei[u]][ej[u]] = G[ej[u]][ei[u]] = acc[u];
+  __syncthreads();
+}
+
+// In-place upper Cholesky of G (l x l, both triangles) -> R in the upper
+// triangle (G = R^T R), then Ri = R^-1 (upper).  shift is added to the
+// diagonal first.  Returns through *ok (shared) whether every pivot was
+// positive.
+__device__ void lds_chol_inv(double (*G)[kLP], double (*Ri)[kLP], int l, double shift, int* ok) {
+  const int tid = threadIdx.x;
+  if (tid == 0) *ok = 1;
+  if (tid < l) G[tid][tid] += shift;
+  __syncthreads();
+  for (int j = 0; j < l; ++j) {
+    if (tid == 0) {
+      const double dj = G[j][j];
+      if (!(dj > 0.0)) *ok = 0;
+      G[j][j] = dj > 0.0 ? sqrt(dj) : 1.0;
+    }
+    __syncthreads();
+    const double rjj = G[j][j];
+    for (int c = j + 1 + tid; c < l; c += 256) G[j][c] /= rjj;
+    __syncthreads();
+    // trailing update G[a][b] -= R[j][a] R[j][b], j < a <= b
+    const int w = l - j - 1;
+    for (int e = tid; e < w * w; e += 256) {
+      const int a = j + 1 + e / w, b = j + 1 + e % w;
+      if (a <= b) G[a][b] -= G[j][a] * G[j][b];
+    }
+    __syncthreads();
+  }
+  // inverse of the upper R, one column per thread (back substitution)
+  if (tid < l) {
+    const int c = tid;
+    for (int r = 0; r < l; ++r) Ri[r][c] = 0.0;
+    Ri[c][c] = 1.0 / G[c][c];
+    for (int r = c - 1; r >= 0; --r) {
+      double s = 0.0;
+      for (int u = r + 1; u <= c; ++u) s += G[r][u] * Ri[u][c];
+      Ri[r][c] = -s / G[r][r];
+    }
+  }
+  __syncthreads();
+}
+
+// out (R x kout, ld ldo) = scale_j * X W for X (R x 48, ld R, first l
+// columns used) and W (l x kout, LDS, FP64), in 64-row chunks staged in LDS:
+// thread (row rr = tid % 64, column group tid / 64) owns 12 outputs of a row.
+// out may alias X when kout == l and W is upper triangular (each chunk is
+// fully read before it is written).
+__device__ void lds_rows_times(const float* X, int64_t R, int l, float (*Xs)[kL + 1],
+                               const double (*W)[kLP], int kout, const double* scale, float* out,
+                               int64_t ldo) {
+  const int tid = threadIdx.x, rr = tid & 63, cg = tid >> 6;
+  for (int64_t r0 = 0; r0 < R; r0 += 64) {
+    const int rows = (int)((R - r0) < 64 ? (R - r0) : 64);
+    __syncthreads();
+    for (int e = tid; e < 64 * l; e += 256) {
+      const int r = e & 63, c = e >> 6;
+      Xs[r][c] = (r < rows) ? X[r0 + r + (int64_t)c * R] : 0.0f;
+    }
+    __syncthreads();
+    double o[12];
+#pragma unroll
+    for (int j = 0; j < 12; ++j) o[j] = 0.0;
+    for (int u = 0; u < l; ++u) {
+      const double xu = (double)Xs[rr][u];
+#pragma unroll
+      for (int j = 0; j < 12; ++j) {
+        const int c = cg * 12 + j;
+        if (c < kout) o[j] += xu * W[u][c];
+      }
+    }
+    if (rr < rows) {
+#pragma unroll
+      for (int j = 0; j < 12; ++j) {
+        const int c = cg * 12 + j;
+        if (c < kout) out[r0 + rr + (int64_t)c * ldo] = (float)(scale ? o[j] * scale[c] : o[j]);
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// ---------------------------------------------------------------- orth
+// CholeskyQR2 of Y_i (FROM_Z: Z_i) per matrix, one workgroup each.
+template <bool FROM_Z>
+__global__ void __launch_bounds__(256)
+borth_kernel(const BMat* __restrict__ mats, float* __restrict__ work, int* __restrict__ flags) {
+  __shared__ float Xs[64][kL + 1];
+  __shared__ double G[kL][kLP];
+  __shared__ double Ri[kL][kLP];
+  __shared__ int ok;
+  const BMat d = mats[blockIdx.x];
+  const int64_t R = FROM_Z ? d.n : d.m;
+  float* X = work + (FROM_Z ? d.z : d.y);
+  const int l = d.l;
+  for (int pass = 0; pass < 2; ++pass) {
+    lds_gram(X, R, l, Xs, G);
+    double tr = 0.0;
+    for (int j = 0; j < l; ++j) tr += G[j][j];
+    __syncthreads();  // every thread has read the diagonal before it is shifted
+    lds_chol_inv(G, Ri, l, pass == 0 ? 1e-14 * tr : 0.0, &ok);
+    if (!ok && threadIdx.x == 0) atomicOr(flags, 1);
+    lds_rows_times(X, R, l, Xs, Ri, l, nullptr, X, R);
+  }
+}
+
+// ---------------------------------------------------------------- final
+__device__ __forceinline__ int circle(int pos, int r, int N) {
+  return pos == 0 ? 0 : 1 + ((pos - 1 + r) % (N - 1));
+}
+
+// two-sided cyclic Jacobi on H (np x np, np even) with J accumulated;
+// 2 x 2 blocks (bi, bj) of the current pairing owned round-robin by threads.
+__device__ void lds_jacobi(double (*H)[kLP], double (*H2)[kLP], double (*J)[kLP], int np,
+                           int* sig_flag) {
+  const int tid = threadIdx.x;
+  const int half = np / 2;
+  double (*Hc)[kLP] = H;
+  double (*Hn)[kLP] = H2;
+  for (int e = tid; e < np * np; e += 256) J[e / np][e % np] = (e / np == e % np) ? 1.0 : 0.0;
+  __syncthreads();
+  // H is positive semidefinite (a Gram matrix): ||H||_F <= trace(H)
+  double tr = 0.0;
+  for (int i = 0; i < np; ++i) tr += fabs(Hc[i][i]);
+  const double flo = 4.0 * 2.220446049250313e-16 * tr;
+  for (int sweep = 0; sweep < 40; ++sweep) {
+    if (tid == 0) *sig_flag = 0;
+    __syncthreads();
+    int sig = 0;
+    for (int t = 0; t < np - 1; ++t) {
+      for (int e = tid; e < half * half; e += 256) {
+        const int bi = e / half, bj = e % half;
+        const int pi = circle(bi, t, np), qi = circle(np - 1 - bi, t, np);
+        const int pj = circle(bj, t, np), qj = circle(np - 1 - bj, t, np);
+        double c[2] = {1.0, 1.0}, s[2] = {0.0, 0.0};
+        const int pp[2] = {pi, pj}, qq[2] = {qi, qj};
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+          const double a = Hc[pp[u]][pp[u]], b = Hc[qq[u]][qq[u]], g = Hc[pp[u]][qq[u]];
+          const double ag = fabs(g);
+          if (ag > flo) {
+            const double ab = fabs(a * b);
+            if (u == 0 && g * g > 1e-20 * ab) sig = 1;
+            if (g * g > 1e-30 * ab) {
+              const double dd = b - a;
+              const double den = fabs(dd) + sqrt(dd * dd + 4.0 * g * g);
+              double tt = 2.0 * g / den;
+              if (dd < 0.0) tt = -tt;
+              c[u] = 1.0 / sqrt(1.0 + tt * tt);
+              s[u] = c[u] * tt;
+            }
+          }
+        }
+        const double h00 = Hc[pi][pj], h01 = Hc[pi][qj], h10 = Hc[qi][pj], h11 = Hc[qi][qj];
+        const double x00 = c[1] * h00 - s[1] * h01, x01 = s[1] * h00 + c[1] * h01;
+        const double x10 = c[1] * h10 - s[1] * h11, x11 = s[1] * h10 + c[1] * h11;
+        double y00 = c[0] * x00 - s[0] * x10, y01 = c[0] * x01 - s[0] * x11;
+        double y10 = s[0] * x00 + c[0] * x10, y11 = s[0] * x01 + c[0] * x11;
+        if (bi == bj && s[0] != 0.0) y01 = y10 = 0.0;
+        Hn[pi][pj] = y00;
+        Hn[pi][qj] = y01;
+        Hn[qi][pj] = y10;
+        Hn[qi][qj] = y11;
+        if (bj == 0) {
+          // J <- J R_i on columns pi, qi (this thread owns pair bi for bj == 0)
+          for (int x = 0; x < np; ++x) {
+            const double jp = J[x][pi], jq = J[x][qi];
+            J[x][pi] = c[0] * jp - s[0] * jq;
+            J[x][qi] = s[0] * jp + c[0] * jq;
+          }
+        }
+      }
+      __syncthreads();
+      double (*tmp)[kLP] = Hc;
+      Hc = Hn;
+      Hn = tmp;
+    }
+    if (sig) atomicOr(sig_flag, 1);
+    __syncthreads();
+    const int any = *sig_flag;
+    __syncthreads();
+    if (!any) break;
+  }
+  // leave the converged matrix in H
+  if (Hc != H) {
+    for (int e = tid; e < np * np; e += 256) H[e / np][e % np] = Hc[e / np][e % np];
+  }
+  __syncthreads();
+}
+
+__global__ void __launch_bounds__(256)
+bfinal_kernel(const BMat* __restrict__ mats, float* __restrict__ work, float* __restrict__ U,
+              double* __restrict__ S, float* __restrict__ V) {
+  __shared__ float Xs[64][kL + 1];
+  __shared__ double H[kL][kLP];
+  __shared__ double H2[kL][kLP];
+  __shared__ double J[kL][kLP];
+  __shared__ double sig[kL];
+  __shared__ int order[kL];
+  __shared__ int flag;
+  const BMat d = mats[blockIdx.x];
+  const int l = d.l, k = d.k;
+  const int np = l + (l & 1);
+  const float* Q = work + d.y;   // m x 48, orthonormal
+  const float* Bt = work + d.z;  // n x 48 = A^T Q
+  lds_gram(Bt, d.n, l, Xs, H);   // H = B B^T (l x l)
+  const int tid = threadIdx.x;
+  if (np > l) {
+    // odd l: one padding row / column, decoupled
+    for (int e = tid; e < np; e += 256) H[l][e] = H[e][l] = 0.0;
+  }
+  __syncthreads();
+  lds_jacobi(H, H2, J, np, &flag);
+  // ascending order of the l real eigenvalues: rank by counting
+  if (tid < l) {
+    const double lam = H[tid][tid];
+    int rk = 0;
+    for (int j = 0; j < l; ++j) {
+      const double o = H[j][j];
+      rk += (o < lam) || (o == lam && j < tid);
+    }
+    order[rk] = tid;
+  }
+  __syncthreads();
+  // the last k in ascending order: output j <- eigenpair order[l - k + j]
+  if (tid < k) {
+    const int e = order[l - k + tid];
+    const double s = sqrt(fmax(H[e][e], 0.0));
+    sig[tid] = s;
+    S[d.s + tid] = s;
+  }
+  __syncthreads();
+  // selected eigenvectors (columns order[l - k + j]) into H2, 1 / sigma
+  for (int e = tid; e < l * k; e += 256) {
+    const int c = e % l, j = e / l;
+    H2[c][j] = J[c][order[l - k + j]];
+  }
+  if (tid < k) sig[tid] = 1.0 / sig[tid];
+  __syncthreads();
+  // U = Q Uhat_sel (m x k), V = B^T Uhat_sel Sigma^-1 (n x k)
+  lds_rows_times(Q, d.m, l, Xs, H2, k, nullptr, U + d.u, d.m);
+  lds_rows_times(Bt, d.n, l, Xs, H2, k, sig, V + d.v, d.n);
+}
+
+__global__ void bclear_kernel(int* __restrict__ p) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0;
+}
+
+// ---------------------------------------------------------------- plan
+struct Plan {
+  std::vector<int64_t> key;
+  int count = 0;
+  int q = 0;
+  DBuf<BMat> mats;
+  DBuf<BItem> it_nn, it_tn;         // pass items per RT variant chosen
+  DBuf<BItem> it_tile_n, it_tile_m;  // tiling items (K = n_i / K = m_i)
+  int n_nn = 0, n_tn = 0, n_tile_n = 0, n_tile_m = 0;
+  int rt_nn = 1, rt_tn = 1;
+  DBuf<float> work;
+  DBuf<bf16x8_t> tiles;
+  DBuf<int> flags;
+  const float* A = nullptr;
+  float* U = nullptr;
+  double* S = nullptr;
+  float* V = nullptr;
+  std::vector<BMat> host;
+  std::vector<int> by_cost;  // matrices, largest first
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  double tuned_ms[2][3] = {{0, 0, 0}, {0, 0, 0}};
+  ~Plan() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+  }
+};
+
+std::mutex g_plan_mu;
+std::map<std::vector<int64_t>, std::unique_ptr<Plan>> g_plans;
+std::string g_last_config;
+
+std::vector<BItem> pass_items(const Plan& P, bool ta, int rt) {
+  std::vector<BItem> v;
+  for (int i : P.by_cost) {
+    const BMat& d = P.host[i];
+    const int64_t M = ta ? d.n : d.m;
+    const int64_t tiles = (M + 16 * rt - 1) / (16 * rt);
+    for (int64_t t = 0; t < tiles; ++t) v.push_back({i, (int32_t)t});
+  }
+  return v;
+}
+
+std::vector<BItem> tile_items(const Plan& P, bool from_y) {
+  std::vector<BItem> v;
+  for (int i : P.by_cost) {
+    const BMat& d = P.host[i];
+    const int64_t K = from_y ? d.m : d.n;
+    for (int64_t kb = 0; kb < (K + 31) / 32; ++kb) v.push_back({i, (int32_t)kb});
+  }
+  return v;
+}
+
+int upload_items(Context* c, const std::vector<BItem>& v, DBuf<BItem>& buf, int* n) {
+  RSVD_TRY(buf.alloc(v.size() + 1));
+  if (!v.empty())
+    RSVD_HIP(hipMemcpyAsync(buf.p, v.data(), v.size() * sizeof(BItem), hipMemcpyHostToDevice,
+                            c->stream));
+  RSVD_HIP(hipStreamSynchronize(c->stream));
+  *n = (int)v.size();
+  return 0;
+}
+
+int launch_pass(Context* c, Plan& P, bool ta, int rt, const BItem* items, int nitems) {
+  if (nitems == 0) return 0;
+  const dim3 grid((unsigned)((nitems + 3) / 4));
+#define RSVD_BPASS(TA_, RT_)                                                                 \
+  hipLaunchKernelGGL((bpass_kernel<TA_, RT_>), grid, dim3(256), 0, c->stream, P.mats.p, items, \
+                     nitems, P.A, P.work.p, (const bf16x8_t*)P.tiles.p)
+  if (ta) {
+    if (rt == 1) RSVD_BPASS(true, 1);
+    else if (rt == 2) RSVD_BPASS(true, 2);
+    else RSVD_BPASS(true, 4);
+  } else {
+    if (rt == 1) RSVD_BPASS(false, 1);
+    else if (rt == 2) RSVD_BPASS(false, 2);
+    else RSVD_BPASS(false, 4);
+  }
+#undef RSVD_BPASS
+  RSVD_LAUNCH_CHECK();
+  return 0;
+}
+
+int launch_tile(Context* c, Plan& P, bool from_y) {
+  const int n = from_y ? P.n_tile_m : P.n_tile_n;
+  if (n == 0) return 0;
+  if (from_y)
+    hipLaunchKernelGGL((btile_kernel<true>), dim3((unsigned)n), dim3(192), 0, c->stream, P.mats.p,
+                       P.it_tile_m.p, (const float*)P.work.p, P.tiles.p);
+  else
+    hipLaunchKernelGGL((btile_kernel<false>), dim3((unsigned)n), dim3(192), 0, c->stream,
+                       P.mats.p, P.it_tile_n.p, (const float*)P.work.p, P.tiles.p);
+  RSVD_LAUNCH_CHECK();
+  return 0;
+}
+
+int launch_orth(Context* c, Plan& P, bool from_z) {
+  if (from_z)
+    hipLaunchKernelGGL((borth_kernel<true>), dim3((unsigned)P.count), dim3(256), 0, c->stream,
+                       P.mats.p, P.work.p, P.flags.p);
+  else
+    hipLaunchKernelGGL((borth_kernel<false>), dim3((unsigned)P.count), dim3(256), 0, c->stream,
+                       P.mats.p, P.work.p, P.flags.p);
+  RSVD_LAUNCH_CHECK();
+  return 0;
+}
+
+// The whole call: Omega, sketch, q - 1 power iterations, B^T, final solve.
+int enqueue_all(Context* c, Plan& P) {
+  hipLaunchKernelGGL(bclear_kernel, dim3(1), dim3(64), 0, c->stream, P.flags.p);
+  RSVD_LAUNCH_CHECK();
+  hipLaunchKernelGGL(bomega_kernel, dim3((unsigned)P.count, kL), dim3(256), 0, c->stream,
+                     P.mats.p, P.work.p);
+  RSVD_LAUNCH_CHECK();
+  RSVD_TRY(launch_tile(c, P, false));
+  RSVD_TRY(launch_pass(c, P, false, P.rt_nn, P.it_nn.p, P.n_nn));  // Y = A Omega
+  RSVD_TRY(launch_orth(c, P, false));
+  for (int it = 1; it < P.q; ++it) {
+    RSVD_TRY(launch_tile(c, P, true));
+    RSVD_TRY(launch_pass(c, P, true, P.rt_tn, P.it_tn.p, P.n_tn));  // Z = A^T Q
+    RSVD_TRY(launch_orth(c, P, true));
+    RSVD_TRY(launch_tile(c, P, false));
+    RSVD_TRY(launch_pass(c, P, false, P.rt_nn, P.it_nn.p, P.n_nn));  // Y = A Z
+    RSVD_TRY(launch_orth(c, P, false));
+  }
+  RSVD_TRY(launch_tile(c, P, true));
+  RSVD_TRY(launch_pass(c, P, true, P.rt_tn, P.it_tn.p, P.n_tn));  // B^T = A^T Q
+  hipLaunchKernelGGL(bfinal_kernel, dim3((unsigned)P.count), dim3(256), 0, c->stream, P.mats.p,
+                     P.work.p, P.U, P.S, P.V);
+  RSVD_LAUNCH_CHECK();
+  return 0;
+}
+
+// Autotuner: time each RT variant of both pass kernels on this batch (after
+// one untimed run so the operands are real) and keep the fastest.
+int autotune(Context* c, Plan& P) {
+  static const int kRT[3] = {1, 2, 4};
+  hipEvent_t e0, e1;
+  RSVD_HIP(hipEventCreate(&e0));
+  RSVD_HIP(hipEventCreate(&e1));
+  int rc = 0;
+  for (int ta = 0; ta < 2 && rc == 0; ++ta) {
+    float best = 1e30f;
+    int best_rt = 1;
+    for (int v = 0; v < 3 && rc == 0; ++v) {
+      const int rt = kRT[v];
+      DBuf<BItem> items;
+      int n = 0;
+      if ((rc = upload_items(c, pass_items(P, ta == 1, rt), items, &n))) break;
+      // warm up once, then time two launches
+      if ((rc = launch_pass(c, P, ta == 1, rt, items.p, n))) break;
+      (void)hipEventRecord(e0, c->stream);
+      for (int rep = 0; rep < 2 && rc == 0; ++rep) rc = launch_pass(c, P, ta == 1, rt, items.p, n);
+      (void)hipEventRecord(e1, c->stream);
+      if (rc) break;
+      (void)hipEventSynchronize(e1);
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      P.tuned_ms[ta][v] = ms / 2;
+      if (ms < best) {
+        best = ms;
+        best_rt = rt;
+      }
+    }
+    if (ta) P.rt_tn = best_rt;
+    else P.rt_nn = best_rt;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return rc;
+}
+
+}  // namespace
+
+int fixed_rank_batched(Context* c, int count, const int* m, const int* n, const int* k, int p,
+                       int q, uint64_t seed, const float* dA, const long long* a_off, float* dU,
+                       const long long* u_off, double* dS, const long long* s_off, float* dV,
+                       const long long* v_off, int* status) {
+  if (status) *status = 0;
+  if (count <= 0) return set_error("batched: count must be positive");
+  if (!m || !n || !k || !a_off || !u_off || !s_off || !v_off || !dA || !dU || !dS || !dV)
+    return set_error("batched: NULL argument");
+  if (p < 0 || q < 1) return set_error("batched: need p >= 0 and q >= 1");
+  std::vector<int64_t> key = {count, p, q, (int64_t)seed, (int64_t)(uintptr_t)dA,
+                              (int64_t)(uintptr_t)dU, (int64_t)(uintptr_t)dS,
+                              (int64_t)(uintptr_t)dV};
+  for (int i = 0; i < count; ++i) {
+    const int l = k[i] + p;
+    if (m[i] < 1 || n[i] < 1 || k[i] < 1 || l > m[i] || l > n[i] || l > kL)
+      return set_error("batched: matrix %d: need 1 <= k, k + p <= min(m, n) and k + p <= %d", i,
+                       kL);
+    key.insert(key.end(), {m[i], n[i], k[i], a_off[i], u_off[i], s_off[i], v_off[i]});
+  }
+  std::lock_guard<std::mutex> lk(g_plan_mu);
+  auto itp = g_plans.find(key);
+  Plan* P;
+  if (itp == g_plans.end()) {
+    if (g_plans.size() >= 8) g_plans.clear();  // bounded cache of layouts
+    auto np = std::make_unique<Plan>();
+    P = np.get();
+    P->key = key;
+    P->count = count;
+    P->q = q;
+    P->A = dA;
+    P->U = dU;
+    P->S = dS;
+    P->V = dV;
+    P->host.resize(count);
+    int64_t wofs = 0, tofs = 0;
+    for (int i = 0; i < count; ++i) {
+      BMat& d = P->host[i];
+      d.m = m[i];
+      d.n = n[i];
+      d.k = k[i];
+      d.l = k[i] + p;
+      d.a = a_off[i];
+      d.y = wofs;
+      wofs += d.m * kL;
+      d.z = wofs;
+      wofs += d.n * kL;
+      d.kbmax = ((d.m > d.n ? d.m : d.n) + 31) / 32;
+      d.t = tofs;
+      tofs += 2 * d.kbmax * kNT * 64;
+      d.u = u_off[i];
+      d.s = s_off[i];
+      d.v = v_off[i];
+      d.seed = seed + (uint64_t)i;
+      d.aligned = (d.m % 4 == 0) && (d.a % 4 == 0) && (reinterpret_cast<uintptr_t>(dA) % 16 == 0);
+      d.pad = 0;
+    }
+    P->by_cost.resize(count);
+    std::iota(P->by_cost.begin(), P->by_cost.end(), 0);
+    std::stable_sort(P->by_cost.begin(), P->by_cost.end(), [&](int a, int b) {
+      return P->host[a].m * P->host[a].n > P->host[b].m * P->host[b].n;
+    });
+    RSVD_TRY(P->mats.alloc(count));
+    RSVD_HIP(hipMemcpyAsync(P->mats.p, P->host.data(), count * sizeof(BMat),
+                            hipMemcpyHostToDevice, c->stream));
+    RSVD_TRY(P->work.alloc((size_t)wofs));
+    RSVD_TRY(P->tiles.alloc((size_t)tofs));
+    RSVD_TRY(P->flags.alloc(1));
+    RSVD_TRY(upload_items(c, tile_items(*P, false), P->it_tile_n, &P->n_tile_n));
+    RSVD_TRY(upload_items(c, tile_items(*P, true), P->it_tile_m, &P->n_tile_m));
+    // one eager run so the tuner times the passes on real operands
+    P->rt_nn = P->rt_tn = 1;
+    RSVD_TRY(upload_items(c, pass_items(*P, false, 1), P->it_nn, &P->n_nn));
+    RSVD_TRY(upload_items(c, pass_items(*P, true, 1), P->it_tn, &P->n_tn));
+    RSVD_TRY(enqueue_all(c, *P));
+    RSVD_TRY(autotune(c, *P));
+    RSVD_TRY(upload_items(c, pass_items(*P, false, P->rt_nn), P->it_nn, &P->n_nn));
+    RSVD_TRY(upload_items(c, pass_items(*P, true, P->rt_tn), P->it_tn, &P->n_tn));
+    // capture the call into a graph
+    RSVD_HIP(hipStreamSynchronize(c->stream));
+    RSVD_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    const int erc = enqueue_all(c, *P);
+    hipGraph_t g = nullptr;
+    const hipError_t ce = hipStreamEndCapture(c->stream, &g);
+    if (erc) {
+      if (g) (void)hipGraphDestroy(g);
+      return erc;
+    }
+    if (ce != hipSuccess) return set_error("batched: graph capture failed: %s", hipGetErrorString(ce));
+    P->graph = g;
+    RSVD_HIP(hipGraphInstantiate(&P->exec, g, nullptr, nullptr, 0));
+    char buf[160];
+    snprintf(buf, sizeof buf,
+             "rt_nn=%d rt_tn=%d (NN ms %.3f/%.3f/%.3f, TN ms %.3f/%.3f/%.3f for RT 1/2/4)",
+             P->rt_nn, P->rt_tn, P->tuned_ms[0][0], P->tuned_ms[0][1], P->tuned_ms[0][2],
+             P->tuned_ms[1][0], P->tuned_ms[1][1], P->tuned_ms[1][2]);
+    g_last_config = buf;
+    g_plans[key] = std::move(np);
+  } else {
+    P = itp->second.get();
+  }
+  {
+    KTimer kt(c, "batched_graph");
+    RSVD_HIP(hipGraphLaunch(P->exec, c->stream));
+  }
+  if (status) {
+    int h = 0;
+    RSVD_HIP(hipMemcpyAsync(&h, P->flags.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    RSVD_HIP(hipStreamSynchronize(c->stream));
+    *status = h;
+  }
+  return 0;
+}
+
+const char* batched_last_config() { return g_last_config.c_str(); }
+
+void batched_release_plans() {
+  std::lock_guard<std::mutex> lk(g_plan_mu);
+  g_plans.clear();
+}
+
+}  // namespace rsvd

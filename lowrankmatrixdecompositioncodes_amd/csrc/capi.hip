@@ -372,24 +372,15 @@ void compute_evals_and_evecs_of_symm_matrix(mat* S, vec* evals) {
   Context* c = api.c;
   const int64_t n = S->nrows;
   if (n == 0) return;
-  DBuf<double> G, G0, V, lam, nrm;
+  DBuf<double> G, G0, V, lam;
   if (upload(c, S, G) || G0.alloc((size_t)(n * n)) || V.alloc((size_t)(n * n)) ||
-      lam.alloc((size_t)n) || nrm.alloc(1))
+      lam.alloc((size_t)n))
     return;
-  // symmetric from the upper triangle, shifted to be positive semidefinite so
-  // the one-sided Jacobi singular vectors are eigenvectors
+  // symmetric from the upper triangle (dsyev 'U' reads only that triangle)
   if (symmetrize_upper_f64(c, n, G.p) || copy2d_f64(c, n, n, G.p, n, G0.p, n)) return;
-  double ss = 0;
-  if (sumsq_f64(c, n, n, G.p, n, nrm.p)) return;
-  if (hipMemcpyAsync(&ss, nrm.p, sizeof ss, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
-      stream_sync(c))
-    return;
-  const double shift = sqrt(ss);
-  if (add_diag_f64(c, n, shift, G.p, n)) return;
   int sw = 0;
-  // the shift inflates the column norms: orthogonalise to the rounding floor
-  if (jacobi_f64(c, n, n, G.p, n, V.p, n, 60, &sw, 0.0)) return;
-  // eigenvalues as Rayleigh quotients v^T G v of the unshifted matrix
+  if (symeig_jacobi_f64(c, n, G.p, n, V.p, n, nullptr, 60, &sw)) return;
+  // eigenvalues as Rayleigh quotients v^T G v of the input
   if (gemm_f64(c, 'N', 'N', n, n, n, 1.0, G0.p, n, V.p, n, 0.0, G.p, n)) return;
   if (coldot_f64(c, n, n, V.p, n, G.p, n, lam.p)) return;
   if (sort_pairs_f64(c, n, lam.p, 0, V.p, n, n, nullptr, 0, 0)) return;
@@ -734,6 +725,7 @@ int rsvd_sync(void) {
 void rsvd_release_cache(void) {
   Api api;
   if (api.rc) return;
+  batched_release_plans();
   pool_trim();
 }
 
@@ -818,6 +810,187 @@ int rsvd_dev_gemm(int dtype, char ta, char tb, long long M, long long N, long lo
   return set_error("rsvd_dev_gemm: dtype %d not supported", dtype);
 }
 
+int rsvd_dev_fixed_rank_batched(int count, const int* m, const int* n, const int* k, int p, int q,
+                                unsigned long long seed, const float* dA, const long long* a_off,
+                                float* dU, const long long* u_off, double* dS,
+                                const long long* s_off, float* dV, const long long* v_off,
+                                int* status) {
+  Api api;
+  if (api.rc) return api.rc;
+  RSVD_TRY(fixed_rank_batched(api.c, count, m, n, k, p, q, seed, dA, a_off, dU, u_off, dS, s_off,
+                              dV, v_off, status));
+  return stream_sync(api.c);
+}
+
+const char* rsvd_batched_last_config(void) { return batched_last_config(); }
+
+int rsvd_dev_batch_test_matrices(int count, const int* m, const int* n, const double* tau,
+                                 unsigned long long seed, float* dA, const long long* a_off) {
+  Api api;
+  if (api.rc) return api.rc;
+  if (count < 0 || (count > 0 && (!m || !n || !tau || !a_off || !dA)))
+    return set_error("rsvd_dev_batch_test_matrices: bad arguments");
+  std::vector<double> sig;
+  for (int i = 0; i < count; ++i) {
+    if (m[i] < 1 || n[i] < 1 || !(tau[i] > 0))
+      return set_error("rsvd_dev_batch_test_matrices: matrix %d: bad shape or tau", i);
+    long long r = m[i] < n[i] ? m[i] : n[i];
+    const long long cap = (long long)ceil(19.0 * tau[i]);
+    if (r > cap) r = cap;
+    sig.resize((size_t)r);
+    for (long long j = 0; j < r; ++j) sig[(size_t)j] = exp(-(double)j / tau[i]);
+    RSVD_TRY(lowrank_test_matrix(api.c, m[i], n[i], (int)r, sig.data(), 0.0, seed + (uint64_t)i,
+                                 nullptr, 0, dA + a_off[i], m[i]));
+  }
+  return stream_sync(api.c);
+}
+
+int rsvd_mlp_num_params(void) { return mlp_num_params(); }
+
+int rsvd_dev_mlp_init(unsigned long long seed, float* dTheta) {
+  Api api;
+  if (api.rc) return api.rc;
+  RSVD_TRY(mlp_init(api.c, seed, dTheta));
+  return stream_sync(api.c);
+}
+
+int rsvd_dev_mlp_forward(const float* dTheta, const float* dX, int batch, float* dOut) {
+  Api api;
+  if (api.rc) return api.rc;
+  RSVD_TRY(mlp_forward(api.c, dTheta, dX, batch, dOut));
+  return stream_sync(api.c);
+}
+
+int rsvd_dev_mlp_grad(const float* dTheta, const float* dX, const int* dq, const float* dr,
+                      float lam, int batch, float* dGrad, float* dLoss) {
+  Api api;
+  if (api.rc) return api.rc;
+  RSVD_TRY(mlp_grad(api.c, dTheta, dX, nullptr, dq, dr, lam, batch, dGrad, dLoss));
+  return stream_sync(api.c);
+}
+
+int rsvd_dev_mlp_train(float* dTheta, const float* dX, const int* dq, const float* dr, int n,
+                       float lam, int epochs, int batch, float lr, unsigned long long seed,
+                       float* loss_hist) {
+  Api api;
+  if (api.rc) return api.rc;
+  RSVD_TRY(mlp_train(api.c, dTheta, dX, dq, dr, n, lam, epochs, batch, lr, seed, loss_hist));
+  return stream_sync(api.c);
+}
+
+int rsvd_dev_sketch_features(const float* dA, long long m, long long n, long long lda,
+                             unsigned long long seed, float* dFeat) {
+  Api api;
+  if (api.rc) return api.rc;
+  RSVD_TRY(sketch_features(api.c, dA, m, n, lda, seed, dFeat));
+  return stream_sync(api.c);
+}
+
+int rsvd_dev_learned_lowrank(const float* dTheta, const double* dA, const float* dA32,
+                             long long m, long long n, long long lda, double tol_rel, int p,
+                             double margin, int kstep, int q_fallback, unsigned long long seed,
+                             int* frank, double** dU, double** dS, double** dV, int* q_pred,
+                             int* k_pred, int* fell_back, double* resid_rel) {
+  Api api;
+  if (api.rc) return api.rc;
+  Context* c = api.c;
+  if (!dTheta || !dA || !dA32 || !frank || !dU || !dS || !dV)
+    return set_error("rsvd_dev_learned_lowrank: NULL argument");
+  if (!(tol_rel > 0) || p < 0 || margin < 0)
+    return set_error("rsvd_dev_learned_lowrank: need tol_rel > 0, p >= 0, margin >= 0");
+  const long long mn = m < n ? m : n;
+  if (mn < 64 || mn - p < 1) return set_error("rsvd_dev_learned_lowrank: matrix too small");
+  *dU = *dS = *dV = nullptr;
+  // prediction
+  DBuf<float> feat, out;
+  RSVD_TRY(feat.alloc(64));
+  RSVD_TRY(out.alloc(6));
+  RSVD_TRY(sketch_features(c, dA32, m, n, lda, seed ^ 0x5eed5eedULL, feat.p));
+  RSVD_TRY(mlp_forward(c, dTheta, feat.p, 1, out.p));
+  float o[6];
+  RSVD_HIP(hipMemcpyAsync(o, out.p, sizeof o, hipMemcpyDeviceToHost, c->stream));
+  DBuf<double> nrm;
+  RSVD_TRY(nrm.alloc(1));
+  RSVD_TRY(sumsq_f64(c, m, n, dA, lda, nrm.p));
+  double a2 = 0;
+  RSVD_HIP(hipMemcpyAsync(&a2, nrm.p, sizeof a2, hipMemcpyDeviceToHost, c->stream));
+  RSVD_TRY(stream_sync(c));
+  int q = 0;
+  for (int i = 1; i < 5; ++i)
+    if (o[i] > o[q]) q = i;
+  double rk = std::isfinite(o[5]) ? o[5] : 1.0;
+  long long k = (long long)ceil(rk * (double)mn * (1.0 + margin)) + 2;
+  if (k < 1) k = 1;
+  if (k > mn - p) k = mn - p;
+  if (q_pred) *q_pred = q;
+  if (k_pred) *k_pred = (int)k;
+  // one fixed-rank rSVD, judged by its FP64 residual
+  double *u = nullptr, *sv = nullptr, *v = nullptr;
+  RSVD_HIP(hipMalloc(&u, sizeof(double) * (size_t)(m * k)));
+  RSVD_HIP(hipMalloc(&sv, sizeof(double) * (size_t)k));
+  RSVD_HIP(hipMalloc(&v, sizeof(double) * (size_t)(n * k)));
+  OmegaSrc om;
+  om.seed = seed;
+  int rc = fixed_rank_f64(c, dA, m, n, lda, (int)k, p, 2, q + 1, 1, om, u, m, sv, v, n, nullptr);
+  double s2 = 0;
+  if (rc == 0) rc = sumsq_f64(c, k, 1, sv, k, nrm.p);
+  if (rc == 0 && hipMemcpyAsync(&s2, nrm.p, sizeof s2, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+    rc = set_error("learned: copy failed");
+  if (rc == 0) rc = stream_sync(c);
+  if (rc) {
+    (void)hipFree(u);
+    (void)hipFree(sv);
+    (void)hipFree(v);
+    return rc;
+  }
+  const double res = sqrt(a2 > s2 ? a2 - s2 : 0.0);
+  if (res <= tol_rel * sqrt(a2)) {
+    *frank = (int)k;
+    *dU = u;
+    *dS = sv;
+    *dV = v;
+    if (fell_back) *fell_back = 0;
+    if (resid_rel) *resid_rel = res / sqrt(a2);
+    return 0;
+  }
+  (void)hipFree(u);
+  (void)hipFree(sv);
+  (void)hipFree(v);
+  // fallback: the adaptive blocked method in TOL mode
+  BlockrandOut bo;
+  RSVD_TRY(blockrand_f64(c, dA, m, n, lda, -1, p, tol_rel * sqrt(a2), 2, kstep, q_fallback, 1, om,
+                         &bo));
+  const int fr = bo.frank;
+  RSVD_HIP(hipMalloc(&u, sizeof(double) * (size_t)m * (fr > 0 ? fr : 1)));
+  RSVD_HIP(hipMalloc(&sv, sizeof(double) * (size_t)(fr > 0 ? fr : 1)));
+  RSVD_HIP(hipMalloc(&v, sizeof(double) * (size_t)n * (fr > 0 ? fr : 1)));
+  if (fr > 0) {
+    RSVD_HIP(hipMemcpyAsync(u, bo.U.p, sizeof(double) * (size_t)m * fr, hipMemcpyDeviceToDevice, c->stream));
+    RSVD_HIP(hipMemcpyAsync(sv, bo.S.p, sizeof(double) * (size_t)fr, hipMemcpyDeviceToDevice, c->stream));
+    RSVD_HIP(hipMemcpyAsync(v, bo.V.p, sizeof(double) * (size_t)n * fr, hipMemcpyDeviceToDevice, c->stream));
+    RSVD_TRY(sumsq_f64(c, fr, 1, sv, fr, nrm.p));
+    RSVD_HIP(hipMemcpyAsync(&s2, nrm.p, sizeof s2, hipMemcpyDeviceToHost, c->stream));
+  } else {
+    s2 = 0;
+  }
+  RSVD_TRY(stream_sync(c));
+  *frank = fr;
+  *dU = u;
+  *dS = sv;
+  *dV = v;
+  if (fell_back) *fell_back = 1;
+  if (resid_rel) *resid_rel = sqrt(a2 > s2 ? a2 - s2 : 0.0) / sqrt(a2);
+  return 0;
+}
+
+int rsvd_dev_copy(const void* src, void* dst, long long bytes) {
+  Api api;
+  if (api.rc) return api.rc;
+  if (bytes < 0) return set_error("rsvd_dev_copy: negative size");
+  RSVD_TRY(stream_copy(api.c, src, dst, bytes));
+  return stream_sync(api.c);
+}
+
 int rsvd_dev_lowrank_test_matrix(long long m, long long n, int r, const double* sigma,
                                  double noise_std, unsigned long long seed, double* dA64,
                                  long long lda64, float* dA32, long long lda32) {
@@ -854,8 +1027,7 @@ int rsvd_dev_sym_eig_psd(long long n, const double* dG, long long ldg, double* d
   RSVD_TRY(copy2d_f64(c, n, n, dG, ldg, W.p, n));
   RSVD_TRY(symmetrize_upper_f64(c, n, W.p));
   int sw = 0;
-  RSVD_TRY(jacobi_f64(c, n, n, W.p, n, dEvecs, ldv, 60, &sw));
-  RSVD_TRY(colnorms_f64(c, n, n, W.p, n, dEvals, 0));
+  RSVD_TRY(symeig_jacobi_f64(c, n, W.p, n, dEvecs, ldv, dEvals, 60, &sw));
   RSVD_TRY(sort_pairs_f64(c, n, dEvals, 0, dEvecs, n, ldv, nullptr, 0, 0));
   if (sweeps) *sweeps = sw;
   return stream_sync(c);

@@ -290,6 +290,19 @@ int gemm_bf16(Context* c, const char* tag, char ta, char tb, int64_t M, int64_t 
               float alpha, const float* A, int64_t lda, const float* B, int64_t ldb, float beta,
               float* C, int64_t ldc, int split) {
   if (M < 0 || N < 0 || K < 0) return set_error("gemm_bf16: negative dimension");
+  if (split != 1 && split != 3) return set_error("gemm_bf16: split must be 1 or 3");
+  const bool tA = (ta == 'T' || ta == 't'), tB = (tb == 'T' || tb == 't');
+  // skinny right operand and enough row blocks to fill the GPU: the
+  // A-streaming kernel, which reads op(A) once
+  if (M > 0 && N > 0 && K >= 32 && !tB && apass_suitable(c, M, N))
+    return apass_bf16(c, tag, tA, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, split);
+  return gemm_bf16_general(c, tag, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, split);
+}
+
+int gemm_bf16_general(Context* c, const char* tag, char ta, char tb, int64_t M, int64_t N,
+                      int64_t K, float alpha, const float* A, int64_t lda, const float* B,
+                      int64_t ldb, float beta, float* C, int64_t ldc, int split) {
+  if (M < 0 || N < 0 || K < 0) return set_error("gemm_bf16: negative dimension");
   if (M == 0 || N == 0) return 0;
   if (K == 0) return gemm_f32(c, ta, tb, M, N, 0, alpha, A, lda, B, ldb, beta, C, ldc);
   const bool tA = (ta == 'T' || ta == 't'), tB = (tb == 'T' || tb == 't');

@@ -52,7 +52,7 @@ int symmetrize_upper_f64(Context* c, int64_t n, double* G) {
 __global__ void sqrt_kernel(int64_t n, const double* __restrict__ in, double* __restrict__ out) {
   for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n;
        j += (int64_t)gridDim.x * blockDim.x)
-    out[j] = sqrt(in[j]);
+    out[j] = sqrt(fmax(in[j], 0.0));  // a PSD eigenvalue may round to -eps
 }
 
 static int grid1(int64_t n) {
@@ -108,8 +108,7 @@ int svd_from_Q_f64(Context* c, const double* A, int64_t m, int64_t n, int64_t ld
   hipLaunchKernelGGL(symmetrize_upper_kernel, dim3(grid1(l * l)), dim3(256), 0, c->stream, l, G.p);
   RSVD_LAUNCH_CHECK();
   int sw = 0;
-  RSVD_TRY(jacobi_f64(c, l, l, G.p, l, Uh.p, l, 60, &sw));
-  RSVD_TRY(colnorms_f64(c, l, l, G.p, l, lam.p, 0));  // lambda_j = ||G v_j||
+  RSVD_TRY(symeig_jacobi_f64(c, l, G.p, l, Uh.p, l, lam.p, 60, &sw));
   RSVD_TRY(sort_pairs_f64(c, l, lam.p, 0, Uh.p, l, l, nullptr, 0, 0));
   hipLaunchKernelGGL(sqrt_kernel, dim3(grid1(l)), dim3(256), 0, c->stream, l,
                      (const double*)lam.p, sig.p);

@@ -137,6 +137,17 @@ int gemm_f32(Context* c, char ta, char tb, int64_t M, int64_t N, int64_t K, floa
 // rounded to bf16 while staged (split = 1) or split into bf16 hi + lo with
 // three products (split = 3, ~fp32 accuracy).  `tag` names the launch for
 // kernel statistics.
+// C = alpha op(A) S + beta C for a skinny fp32 S (K x N, k-contiguous), A
+// read once from HBM; bf16 (split 1) or split-bf16 (split 3) matrix cores.
+int apass_bf16(Context* c, const char* tag, bool ta, int64_t M, int64_t N, int64_t K, float alpha,
+               const float* A, int64_t lda, const float* S, int64_t lds, float beta, float* C,
+               int64_t ldc, int split);
+// true when apass_bf16 has enough row blocks to fill the GPU.
+bool apass_suitable(Context* c, int64_t M, int64_t N);
+// The tiled bf16 GEMM without the A-streaming dispatch (any shape).
+int gemm_bf16_general(Context* c, const char* tag, char ta, char tb, int64_t M, int64_t N,
+                      int64_t K, float alpha, const float* A, int64_t lda, const float* B,
+                      int64_t ldb, float beta, float* C, int64_t ldc, int split);
 int gemm_bf16(Context* c, const char* tag, char ta, char tb, int64_t M, int64_t N, int64_t K,
               float alpha, const float* A, int64_t lda, const float* B, int64_t ldb, float beta,
               float* C, int64_t ldc, int split);
@@ -175,6 +186,8 @@ int f32_to_f64(Context* c, int64_t rows, int64_t cols, const float* src, int64_t
                int64_t ldd);
 int f64_to_f32(Context* c, int64_t rows, int64_t cols, const double* src, int64_t lds, float* dst,
                int64_t ldd);
+// Plain device copy of `bytes` (multiple of 16, 16-byte aligned pointers).
+int stream_copy(Context* c, const void* src, void* dst, int64_t bytes);
 // G(i,j) = G(j,i) for i > j (upper triangle mirrored).
 int symmetrize_upper_f64(Context* c, int64_t n, double* G);
 int add_diag_f64(Context* c, int64_t n, double v, double* a, int64_t lda);
@@ -205,6 +218,29 @@ int trtri_upper_f64(Context* c, int64_t n, const double* R, double* Rinv);
 constexpr double kJacobiTol = 1e-12;
 int jacobi_f64(Context* c, int64_t L, int64_t n, double* W, int64_t ldw, double* V, int64_t ldv,
                int max_sweeps, int* sweeps_out, double rel_tol = kJacobiTol);
+// Batched fixed-rank rSVD (batched.hip); arguments as rsvd_dev_fixed_rank_batched.
+int fixed_rank_batched(Context* c, int count, const int* m, const int* n, const int* k, int p,
+                       int q, uint64_t seed, const float* dA, const long long* a_off, float* dU,
+                       const long long* u_off, double* dS, const long long* s_off, float* dV,
+                       const long long* v_off, int* status);
+const char* batched_last_config();
+void batched_release_plans();
+// Learned (q, rank) front end (learned.hip).
+int mlp_num_params();
+int mlp_init(Context* c, uint64_t seed, float* theta);
+int mlp_forward(Context* c, const float* theta, const float* X, int B, float* out);
+int mlp_grad(Context* c, const float* theta, const float* X, const int* idx, const int* q,
+             const float* r, float lam, int B, float* grad, float* loss);
+int mlp_train(Context* c, float* theta, const float* X, const int* q, const float* r, int n,
+              float lam, int epochs, int batch, float lr, uint64_t seed, float* loss_hist);
+int sketch_features(Context* c, const float* A, int64_t m, int64_t n, int64_t lda, uint64_t seed,
+                    float* feat);
+// Symmetric eigensolver (two-sided block Jacobi).  G (n x n, ld ldg, both
+// triangles, symmetric) is diagonalised in place: on return G ~ diag(lambda)
+// and V (optional, n x n) holds the eigenvectors (V^T G_in V = diag);
+// evals (optional) = diag(G), unsorted.  *sweeps_out = sweeps used.
+int symeig_jacobi_f64(Context* c, int64_t n, double* G, int64_t ldg, double* V, int64_t ldv,
+                      double* evals, int max_sweeps, int* sweeps_out);
 // Column norms of W into nrm[n]; optional normalisation of the columns.
 int colnorms_f64(Context* c, int64_t L, int64_t n, double* W, int64_t ldw, double* nrm,
                  int normalise);

@@ -1,0 +1,391 @@
+// symeig.hip -- symmetric eigensolver of the rSVD pipeline on the GPU, the
+// device replacement of LAPACKE_dsyev on BB^T (MKL/mvf.c:1206-1209, vnum = 2,
+// called from rra.c:191) and of compute_evals_and_evecs_of_symm_matrix.
+//
+// Two-sided cyclic block Jacobi.  The n columns are cut into blocks of B = 16;
+// a round pairs the blocks with the circle method (every block meets every
+// other once per sweep).  A round is two launches:
+//   solve : one workgroup per block pair loads the pair's S x S (S = 2B)
+//           diagonal tile H of G and runs one cyclic inner sweep of two-sided
+//           Jacobi on it (B disjoint rotations per step, S - 1 steps),
+//           accumulating the S x S orthogonal J with H <- J^T H J;
+//   apply : every S x S tile of G in pair order becomes J_a^T T J_b (the
+//           upper tiles are computed and mirrored, so G stays exactly
+//           symmetric) and every S-row strip of V's pair columns becomes
+//           V J -- hundreds of independent workgroups.
+// Unlike one-sided Jacobi no Gram matrix is formed: the tile is read from G
+// itself, and the O(n^2) work of a round is spread over the whole GPU.
+// A sweep in which no pair met had |g_pq| > 1e-10 sqrt(|g_pp g_qq|) ends the
+// loop: Jacobi converges quadratically, so that sweep's rotations leave the
+// off-diagonals near the rounding floor.
+// Off-diagonals below 4 eps ||G||_F are not rotated (they are below the
+// absolute accuracy of any backward-stable eigensolver, LAPACK's included).
+#include <cmath>
+#include <cstring>
+
+#include "rsvd_internal.h"
+
+namespace rsvd {
+
+namespace {
+
+constexpr double kRotTol = 1e-15;
+constexpr double kStopTol = 1e-10;
+
+__device__ __forceinline__ int player(int pos, int r, int N) {
+  return pos == 0 ? 0 : 1 + ((pos - 1 + r) % (N - 1));
+}
+
+// Global column of slot s (0..2B-1) of block pair w in round r; -1 when the
+// slot is padding (dummy block or past n).
+template <int B>
+__device__ __forceinline__ int64_t pair_col(int w, int s, int r, int N, int nblocks, int64_t n) {
+  const int blk = (s < B) ? player(w, r, N) : player(N - 1 - w, r, N);
+  const int64_t col = (int64_t)blk * B + (s & (B - 1));
+  return (blk < nblocks && col < n) ? col : -1;
+}
+
+// 1/x and 1/sqrt(x) from the hardware estimates plus two Newton steps
+// (error a few ulp; a rotation only needs c^2 + s^2 = 1 to rounding).
+__device__ __forceinline__ double rcp_nr(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, y, 1.0);
+  y = fma(y, e, y);
+  e = fma(-x, y, 1.0);
+  return fma(y, e, y);
+}
+__device__ __forceinline__ double rsq_nr(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  const double h = 0.5 * x;
+  double e = fma(-h * y, y, 0.5);
+  y = fma(y, e, y);
+  e = fma(-h * y, y, 0.5);
+  return fma(y, e, y);
+}
+
+// Rotation (c, s) with [x' y'] = [x y] [[c, s], [-s, c]] zeroing the (p, q)
+// entry of a symmetric 2x2 [[a, g], [g, b]]: t = tan theta is the root of
+// g t^2 + (b - a) t - g = 0 of smaller modulus, t = sgn(d) 2g / (|d| +
+// sqrt(d^2 + 4g^2)), d = b - a (Rutishauser's choice).  No rotation when
+// |g| <= flo or |g| <= kRotTol sqrt|ab|; *sig reports |g| > kStopTol sqrt|ab|.
+__device__ __forceinline__ void rotation(double a, double b, double g, double flo, double* cs,
+                                         double* sn, int* sig) {
+  *cs = 1.0;
+  *sn = 0.0;
+  *sig = 0;
+  const double ag = fabs(g);
+  if (!(ag > flo)) return;
+  const double ab = fabs(a * b);
+  const double g2 = g * g;
+  *sig = g2 > kStopTol * kStopTol * ab;
+  if (!(g2 > kRotTol * kRotTol * ab)) return;
+  const double d = b - a;
+  double t;
+  if (fabs(d) < 1e100 && ag < 1e100) {
+    const double r2 = fma(d, d, 4.0 * g2);
+    const double den = fabs(d) + r2 * rsq_nr(r2);
+    t = 2.0 * g * rcp_nr(den);
+    if (d < 0.0) t = -t;
+  } else {
+    const double zeta = d / (2.0 * g);
+    t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+  }
+  const double c = rsq_nr(fma(t, t, 1.0));
+  *cs = c;
+  *sn = c * t;
+}
+
+// One workgroup per block pair: H = G(cols, cols), one inner sweep, J out.
+// Thread t owns, in every step, the 2 x 2 blocks e = t + 256 u of H (rows of
+// local pair e / B, columns of local pair e % B; the blocks partition H) and,
+// for each, rows 2 (e % B), 2 (e % B) + 1 of J's columns of pair e / B.  H
+// is double-buffered, so a step needs a single barrier.
+template <int B>
+__global__ void __launch_bounds__(256)
+symj_solve_kernel(int64_t n, const double* __restrict__ G, int64_t ldg, int nblocks, int N,
+                  int round, const double* __restrict__ fro2, double* __restrict__ Jout,
+                  unsigned long long* __restrict__ offmax) {
+  constexpr int S = 2 * B, HP = S + 1;
+  constexpr int BPT = (B * B) / 256;       // 2x2 blocks per thread
+  static_assert(BPT >= 1 && BPT * 256 == B * B, "B * B must be a multiple of 256");
+  __shared__ double Ha[S * HP], Hb[S * HP], Jl[S * HP];
+  __shared__ int64_t cols[S];
+  const int w = blockIdx.x;
+  const int tid = threadIdx.x;
+  if (tid < S) cols[tid] = pair_col<B>(w, tid, round, N, nblocks, n);
+  __syncthreads();
+  for (int e = tid; e < S * S; e += 256) {
+    const int i = e % S, j = e / S;
+    const int64_t ci = cols[i], cj = cols[j];
+    Ha[i * HP + j] = (ci >= 0 && cj >= 0) ? G[ci + cj * ldg] : 0.0;
+    Jl[i * HP + j] = (i == j) ? 1.0 : 0.0;
+  }
+  const double flo = 4.0 * 2.220446049250313e-16 * sqrt(*fro2);
+  __syncthreads();
+
+  int any_sig = 0;
+  double* Hc = Ha;
+  double* Hn = Hb;
+  for (int t = 0; t < S - 1; ++t) {
+#pragma unroll
+    for (int u = 0; u < BPT; ++u) {
+      const int e = tid + 256 * u;
+      const int bi = e / B, bj = e % B;
+      const int pi = player(bi, t, S), qi = player(S - 1 - bi, t, S);
+      const int pj = player(bj, t, S), qj = player(S - 1 - bj, t, S);
+      double ci, si, cj, sj;
+      int gi, gj;
+      rotation(Hc[pi * HP + pi], Hc[qi * HP + qi], Hc[pi * HP + qi], flo, &ci, &si, &gi);
+      rotation(Hc[pj * HP + pj], Hc[qj * HP + qj], Hc[pj * HP + qj], flo, &cj, &sj, &gj);
+      any_sig |= gi;
+      const double h00 = Hc[pi * HP + pj], h01 = Hc[pi * HP + qj];
+      const double h10 = Hc[qi * HP + pj], h11 = Hc[qi * HP + qj];
+      // X = H_blk R_j, then R_i^T X
+      const double x00 = cj * h00 - sj * h01, x01 = sj * h00 + cj * h01;
+      const double x10 = cj * h10 - sj * h11, x11 = sj * h10 + cj * h11;
+      double y00 = ci * x00 - si * x10, y01 = ci * x01 - si * x11;
+      double y10 = si * x00 + ci * x10, y11 = si * x01 + ci * x11;
+      if (bi == bj && si != 0.0) y01 = y10 = 0.0;  // the annihilated pair
+      Hn[pi * HP + pj] = y00;
+      Hn[pi * HP + qj] = y01;
+      Hn[qi * HP + pj] = y10;
+      Hn[qi * HP + qj] = y11;
+      // J <- J R_i on the columns of pair bi, rows 2 bj and 2 bj + 1 (over bj
+      // these cover every row once)
+#pragma unroll
+      for (int x = 2 * bj; x < 2 * bj + 2; ++x) {
+        const double jp = Jl[x * HP + pi], jq = Jl[x * HP + qi];
+        Jl[x * HP + pi] = ci * jp - si * jq;
+        Jl[x * HP + qi] = si * jp + ci * jq;
+      }
+    }
+    __syncthreads();
+    double* tmp = Hc;
+    Hc = Hn;
+    Hn = tmp;
+  }
+  // J out, column-major S x S per pair
+  double* __restrict__ Jw = Jout + (size_t)w * S * S;
+  for (int e = tid; e < S * S; e += 256) {
+    const int i = e % S, j = e / S;
+    Jw[i + j * S] = Jl[i * HP + j];
+  }
+  if (__any(any_sig) && (tid & 63) == 0) atomicOr(offmax, 1ULL);
+}
+
+// out (S x S) = op(A) * Bm with A, Bm in LDS (row-major, padded HP); thread
+// owns the R x R output block (rows R (tid / 16) .., columns R (tid % 16) ..),
+// R = S / 16.
+template <int S, bool TA>
+__device__ __forceinline__ void tile_mm(const double* __restrict__ A, const double* __restrict__ Bm,
+                                        double (&o)[S / 16][S / 16]) {
+  constexpr int R = S / 16, HP = S + 1;
+  const int i0 = (threadIdx.x >> 4) * R, j0 = (threadIdx.x & 15) * R;
+#pragma unroll
+  for (int a = 0; a < R; ++a)
+#pragma unroll
+    for (int b = 0; b < R; ++b) o[a][b] = 0.0;
+#pragma unroll 4
+  for (int u = 0; u < S; ++u) {
+    double av[R], bv[R];
+#pragma unroll
+    for (int a = 0; a < R; ++a) av[a] = TA ? A[u * HP + i0 + a] : A[(i0 + a) * HP + u];
+#pragma unroll
+    for (int b = 0; b < R; ++b) bv[b] = Bm[u * HP + j0 + b];
+#pragma unroll
+    for (int a = 0; a < R; ++a)
+#pragma unroll
+      for (int b = 0; b < R; ++b) o[a][b] += av[a] * bv[b];
+  }
+}
+
+// Apply the round's rotations.  Blocks [0, ntg) take the upper pair tiles
+// (w1 <= w2) of G: T <- J_w1^T T J_w2, written to both triangles; the rest
+// take S-row strips of V: V(rows, cols(w)) <- V(rows, cols(w)) J_w.  Every
+// workgroup reads and writes only its own tile, so the update is in place.
+template <int B>
+__global__ void __launch_bounds__(256)
+symj_apply_kernel(int64_t n, double* __restrict__ G, int64_t ldg, double* __restrict__ V,
+                  int64_t ldv, int nblocks, int N, int round, const double* __restrict__ Jall,
+                  int ntg) {
+  constexpr int S = 2 * B, HP = S + 1, R = S / 16;
+  __shared__ double T[S * HP], Ja[S * HP], Jb[S * HP], X[S * HP];
+  __shared__ int64_t ra[S], cb[S];
+  const int pairs = N / 2;
+  const int tid = threadIdx.x;
+  const int i0 = (tid >> 4) * R, j0 = (tid & 15) * R;
+  int bid = blockIdx.x;
+  double o[R][R];
+  if (bid < ntg) {
+    int w1 = 0;
+    while (bid >= pairs - w1) {
+      bid -= pairs - w1;
+      ++w1;
+    }
+    const int w2 = w1 + bid;
+    if (tid < S) {
+      ra[tid] = pair_col<B>(w1, tid, round, N, nblocks, n);
+      cb[tid] = pair_col<B>(w2, tid, round, N, nblocks, n);
+    }
+    __syncthreads();
+    const double* J1 = Jall + (size_t)w1 * S * S;
+    const double* J2 = Jall + (size_t)w2 * S * S;
+    for (int e = tid; e < S * S; e += 256) {
+      const int r = e % S, c = e / S;
+      const int64_t gr = ra[r], gc = cb[c];
+      T[r * HP + c] = (gr >= 0 && gc >= 0) ? G[gr + gc * ldg] : 0.0;
+      Ja[r * HP + c] = J1[r + c * S];
+      Jb[r * HP + c] = J2[r + c * S];
+    }
+    __syncthreads();
+    tile_mm<S, false>(T, Jb, o);  // X = T J_w2
+#pragma unroll
+    for (int a = 0; a < R; ++a)
+#pragma unroll
+      for (int b = 0; b < R; ++b) X[(i0 + a) * HP + j0 + b] = o[a][b];
+    __syncthreads();
+    tile_mm<S, true>(Ja, X, o);   // Y = J_w1^T X
+    __syncthreads();
+#pragma unroll
+    for (int a = 0; a < R; ++a)
+#pragma unroll
+      for (int b = 0; b < R; ++b) T[(i0 + a) * HP + j0 + b] = o[a][b];
+    __syncthreads();
+    if (w1 == w2) {
+      // diagonal pair tile: the upper triangle, mirrored
+      for (int e = tid; e < S * S; e += 256) {
+        const int r = e % S, c = e / S;
+        const int64_t gr = ra[r], gc = cb[c];
+        if (gr < 0 || gc < 0) continue;
+        G[gr + gc * ldg] = (r <= c) ? T[r * HP + c] : T[c * HP + r];
+      }
+    } else {
+      for (int e = tid; e < S * S; e += 256) {
+        const int r = e % S, c = e / S;
+        const int64_t gr = ra[r], gc = cb[c];
+        if (gr >= 0 && gc >= 0) G[gr + gc * ldg] = T[r * HP + c];
+      }
+      for (int e = tid; e < S * S; e += 256) {
+        const int r = e % S, c = e / S;  // mirror: row cb[r], column ra[c]
+        const int64_t gr = cb[r], gc = ra[c];
+        if (gr >= 0 && gc >= 0) G[gr + gc * ldg] = T[c * HP + r];
+      }
+    }
+    return;
+  }
+  if (V == nullptr) return;
+  bid -= ntg;
+  const int w = bid % pairs;
+  const int64_t r0 = (int64_t)(bid / pairs) * S;
+  if (tid < S) cb[tid] = pair_col<B>(w, tid, round, N, nblocks, n);
+  __syncthreads();
+  const double* Jw = Jall + (size_t)w * S * S;
+  for (int e = tid; e < S * S; e += 256) {
+    const int r = e % S, c = e / S;
+    const int64_t gc = cb[c];
+    T[r * HP + c] = (gc >= 0 && r0 + r < n) ? V[(r0 + r) + gc * ldv] : 0.0;
+    Jb[r * HP + c] = Jw[r + c * S];
+  }
+  __syncthreads();
+  tile_mm<S, false>(T, Jb, o);
+  __syncthreads();
+#pragma unroll
+  for (int a = 0; a < R; ++a)
+#pragma unroll
+    for (int b = 0; b < R; ++b) T[(i0 + a) * HP + j0 + b] = o[a][b];
+  __syncthreads();
+  for (int e = tid; e < S * S; e += 256) {
+    const int r = e % S, c = e / S;
+    const int64_t gc = cb[c];
+    if (gc >= 0 && r0 + r < n) V[(r0 + r) + gc * ldv] = T[r * HP + c];
+  }
+}
+
+__global__ void symj_identity_kernel(int64_t n, double* __restrict__ V, int64_t ldv) {
+  const int64_t total = n * n;
+  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = idx % n, j = idx / n;
+    V[i + j * ldv] = (i == j) ? 1.0 : 0.0;
+  }
+}
+
+__global__ void symj_diag_kernel(int64_t n, const double* __restrict__ G, int64_t ldg,
+                                 double* __restrict__ d) {
+  const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (j < n) d[j] = G[j + j * ldg];
+}
+
+template <int B>
+int run_sweeps(Context* c, int64_t n, double* G, int64_t ldg, double* V, int64_t ldv,
+               int max_sweeps, int* sweep_out) {
+  constexpr int S = 2 * B;
+  const int nblocks = (int)((n + B - 1) / B);
+  const int N = nblocks + (nblocks & 1);
+  const int pairs = N / 2;
+  const int ntg = pairs * (pairs + 1) / 2;
+  const int ntv = V ? pairs * (int)((n + S - 1) / S) : 0;
+  DBuf<double> J, fro2;
+  DBuf<unsigned long long> off;
+  RSVD_TRY(J.alloc((size_t)pairs * S * S));
+  RSVD_TRY(fro2.alloc(1));
+  RSVD_TRY(off.alloc(1));
+  RSVD_TRY(sumsq_f64(c, n, n, G, ldg, fro2.p));
+  int sweep = 0;
+  if (n > 1) {
+    for (; sweep < max_sweeps; ++sweep) {
+      RSVD_HIP(hipMemsetAsync(off.p, 0, sizeof(unsigned long long), c->stream));
+      for (int r = 0; r < N - 1; ++r) {
+        {
+          KTimer kt(c, "symeig_solve");
+          hipLaunchKernelGGL(symj_solve_kernel<B>, dim3(pairs), dim3(256), 0, c->stream, n,
+                             (const double*)G, ldg, nblocks, N, r, (const double*)fro2.p, J.p,
+                             off.p);
+          RSVD_LAUNCH_CHECK();
+        }
+        KTimer kt(c, "symeig_apply");
+        hipLaunchKernelGGL(symj_apply_kernel<B>, dim3(ntg + ntv), dim3(256), 0, c->stream, n, G,
+                           ldg, V, ldv, nblocks, N, r, (const double*)J.p, ntg);
+        RSVD_LAUNCH_CHECK();
+      }
+      unsigned long long h = 0;
+      RSVD_HIP(hipMemcpyAsync(&h, off.p, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+      RSVD_HIP(hipStreamSynchronize(c->stream));
+      if (h == 0) {  // no pair met in this sweep was above kStopTol
+        ++sweep;
+        break;
+      }
+    }
+  }
+  *sweep_out = sweep;
+  return 0;
+}
+
+}  // namespace
+
+int symeig_jacobi_f64(Context* c, int64_t n, double* G, int64_t ldg, double* V, int64_t ldv,
+                      double* evals, int max_sweeps, int* sweeps_out) {
+  if (sweeps_out) *sweeps_out = 0;
+  if (n <= 0) return 0;
+  if (ldg < n || (V && ldv < n)) return set_error("symeig: leading dimension < n");
+  if (V) {
+    int blocks = (int)((n * n + 255) / 256);
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(symj_identity_kernel, dim3(blocks), dim3(256), 0, c->stream, n, V, ldv);
+    RSVD_LAUNCH_CHECK();
+  }
+  int sweep = 0;
+  // B = 16: each thread owns one 2x2 block and computes two rotations per
+  // step; a larger B trades launches for a slower, LDS-bound inner sweep
+  const int rc = run_sweeps<16>(c, n, G, ldg, V, ldv, max_sweeps, &sweep);
+  if (rc) return rc;
+  if (evals) {
+    hipLaunchKernelGGL(symj_diag_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       c->stream, n, (const double*)G, ldg, evals);
+    RSVD_LAUNCH_CHECK();
+  }
+  if (sweeps_out) *sweeps_out = sweep;
+  return 0;
+}
+
+}  // namespace rsvd

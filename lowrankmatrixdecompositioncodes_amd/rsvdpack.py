@@ -123,6 +123,31 @@ _SIGS = {
                   C.c_double, C.c_int, C.c_int, C.c_int, C.c_int, C.c_ulonglong, C.c_void_p,
                   C.POINTER(C.c_int), C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
                   C.POINTER(C.c_void_p)]),
+    "rsvd_dev_copy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_longlong]),
+    "rsvd_dev_fixed_rank_batched": (C.c_int, [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+                                              C.c_int, C.c_ulonglong, C.c_void_p, C.c_void_p,
+                                              C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                              C.c_void_p, C.c_void_p, C.POINTER(C.c_int)]),
+    "rsvd_batched_last_config": (C.c_char_p, []),
+    "rsvd_mlp_num_params": (C.c_int, []),
+    "rsvd_dev_mlp_init": (C.c_int, [C.c_ulonglong, C.c_void_p]),
+    "rsvd_dev_mlp_forward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
+    "rsvd_dev_mlp_grad": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_float,
+                                    C.c_int, C.c_void_p, C.c_void_p]),
+    "rsvd_dev_mlp_train": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+                                     C.c_float, C.c_int, C.c_int, C.c_float, C.c_ulonglong,
+                                     C.c_void_p]),
+    "rsvd_dev_sketch_features": (C.c_int, [C.c_void_p, C.c_longlong, C.c_longlong, C.c_longlong,
+                                           C.c_ulonglong, C.c_void_p]),
+    "rsvd_dev_learned_lowrank": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_longlong,
+                                           C.c_longlong, C.c_longlong, C.c_double, C.c_int,
+                                           C.c_double, C.c_int, C.c_int, C.c_ulonglong,
+                                           C.POINTER(C.c_int), C.POINTER(C.c_void_p),
+                                           C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
+                                           C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                           C.POINTER(C.c_int), C.POINTER(C.c_double)]),
+    "rsvd_dev_batch_test_matrices": (C.c_int, [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                                               C.c_ulonglong, C.c_void_p, C.c_void_p]),
     "rsvd_dev_gemm": (C.c_int, [C.c_int, C.c_char, C.c_char, C.c_longlong, C.c_longlong,
                                 C.c_longlong, C.c_double, C.c_void_p, C.c_longlong, C.c_void_p,
                                 C.c_longlong, C.c_double, C.c_void_p, C.c_longlong]),
@@ -485,3 +510,62 @@ def kstats(name: str) -> tuple[float, int]:
     t, c = C.c_double(0), C.c_longlong(0)
     rc = load().rsvd_kstats_get(name.encode(), C.byref(t), C.byref(c))
     return (t.value, c.value) if rc == 0 else (0.0, 0)
+
+
+class Batch:
+    """A batch of independent float32 matrices packed in one device buffer,
+    with packed device outputs for rsvd_dev_fixed_rank_batched."""
+
+    def __init__(self, m, n, k, p: int):
+        self.m = np.ascontiguousarray(m, dtype=np.int32)
+        self.n = np.ascontiguousarray(n, dtype=np.int32)
+        self.k = np.ascontiguousarray(k, dtype=np.int32)
+        self.p = int(p)
+        self.count = len(self.m)
+        mn = self.m.astype(np.int64) * self.n
+        self.a_off = np.concatenate([[0], np.cumsum(mn)[:-1]]).astype(np.int64)
+        mk = self.m.astype(np.int64) * self.k
+        nk = self.n.astype(np.int64) * self.k
+        self.u_off = np.concatenate([[0], np.cumsum(mk)[:-1]]).astype(np.int64)
+        self.v_off = np.concatenate([[0], np.cumsum(nk)[:-1]]).astype(np.int64)
+        self.s_off = np.concatenate([[0], np.cumsum(self.k)[:-1]]).astype(np.int64)
+        self.A = DeviceMatrix(int(mn.sum()), 1, np.float32)
+        self.U = DeviceMatrix(int(mk.sum()), 1, np.float32)
+        self.V = DeviceMatrix(int(nk.sum()), 1, np.float32)
+        self.S = DeviceMatrix(int(self.k.sum()), 1, np.float64)
+
+    def fill_test_matrices(self, tau, seed: int) -> None:
+        tau = np.ascontiguousarray(tau, dtype=np.float64)
+        _check(load().rsvd_dev_batch_test_matrices(
+            self.count, self.m.ctypes.data, self.n.ctypes.data, tau.ctypes.data, seed,
+            self.A.ptr, self.a_off.ctypes.data), "rsvd_dev_batch_test_matrices")
+
+    def run(self, q: int, seed: int) -> int:
+        st = C.c_int(0)
+        _check(load().rsvd_dev_fixed_rank_batched(
+            self.count, self.m.ctypes.data, self.n.ctypes.data, self.k.ctypes.data, self.p, q,
+            seed, self.A.ptr, self.a_off.ctypes.data, self.U.ptr, self.u_off.ctypes.data,
+            self.S.ptr, self.s_off.ctypes.data, self.V.ptr, self.v_off.ctypes.data,
+            C.byref(st)), "rsvd_dev_fixed_rank_batched")
+        return st.value
+
+    def matrix(self, i: int) -> np.ndarray:
+        """A_i copied to the host (float32, m x n)."""
+        a = self.A.to_host()[:, 0]
+        o, m, n = self.a_off[i], self.m[i], self.n[i]
+        return a[o:o + m * n].reshape(n, m).T
+
+    def results(self):
+        """Per matrix (U_i, S_i, V_i) on the host."""
+        U, S, V = self.U.to_host()[:, 0], self.S.to_host()[:, 0], self.V.to_host()[:, 0]
+        out = []
+        for i in range(self.count):
+            m, n, k = int(self.m[i]), int(self.n[i]), int(self.k[i])
+            u = U[self.u_off[i]:self.u_off[i] + m * k].reshape(k, m).T
+            v = V[self.v_off[i]:self.v_off[i] + n * k].reshape(k, n).T
+            out.append((u, S[self.s_off[i]:self.s_off[i] + k].copy(), v))
+        return out
+
+
+def batched_last_config() -> str:
+    return load().rsvd_batched_last_config().decode()

@@ -134,6 +134,42 @@ def test_symmetric_eig_uses_upper_triangle(gpu_lib):
     assert np.abs(vals - np.linalg.eigvalsh(S)).max() < 1e-12 * np.abs(vals).max()
 
 
+@pytest.mark.parametrize("n,kind", [(16, "spread"), (17, "spread"), (200, "clustered"),
+                                    (544, "clustered"), (300, "rank_deficient")])
+def test_dev_sym_eig_psd(gpu_lib, n, kind):
+    """The vnum = 2 eigensolver on device: eigenvalues ascending to 1e-12 of the
+    largest (LAPACK's own accuracy class), orthonormal eigenvectors, small
+    residual; a clustered spectrum and an exactly rank-deficient G (zero
+    eigenvalues) must still terminate."""
+    import ctypes as C
+    from lowrankmatrixdecompositioncodes_amd import rsvdpack as rp
+    rng = np.random.default_rng(n)
+    Qm, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    if kind == "spread":
+        lam = np.exp(-np.arange(n) / 3.0)
+    elif kind == "clustered":
+        lam = np.concatenate([1.0 / (1 + np.arange(30.0)) ** 2,
+                              np.full(n - 30, 1e-3) * (1 + 0.2 * rng.random(n - 30))])
+    else:
+        lam = np.concatenate([rng.random(n // 2) + 0.5, np.zeros(n - n // 2)])
+    G = (Qm * lam) @ Qm.T
+    G = (G + G.T) / 2
+    dG = rp.DeviceMatrix.from_host(G)
+    ev, V = rp.DeviceMatrix(n, 1), rp.DeviceMatrix(n, n)
+    sw = C.c_int(0)
+    lib = rp.load()
+    assert lib.rsvd_dev_sym_eig_psd(n, dG.ptr, n, ev.ptr, V.ptr, n, C.byref(sw)) == 0, \
+        rp.last_error()
+    w = ev.to_host()[:, 0]
+    X = V.to_host()
+    ref = np.linalg.eigvalsh(G)
+    assert np.all(np.diff(w) >= 0)
+    assert np.abs(w - ref).max() < 1e-12 * ref.max()
+    assert np.abs(X.T @ X - np.eye(n)).max() < 1e-12
+    assert np.linalg.norm(G @ X - X * w) < 1e-12 * ref.max() * np.sqrt(n)
+    assert 0 < sw.value < 60
+
+
 @pytest.mark.parametrize("m,n", [(110, 110), (60, 40), (40, 60), (544, 544)])
 def test_svd(gpu_lib, m, n):
     rng = np.random.default_rng(m * n)
@@ -220,3 +256,31 @@ def test_gemm_bf16x3_is_fp32_accurate(gpu_lib, ta, tb):
     gpu_lib.dev_gemm(ta, tb, dA, dB, dC2, mode="bf16")
     err1 = np.abs(dC2.to_host() - want).max() / scale
     assert err < err1 / 20
+
+
+@pytest.mark.parametrize("ta", ["N", "T"])
+@pytest.mark.parametrize("mode", ["bf16", "bf16x3"])
+@pytest.mark.parametrize("M,N,K", [(16400, 544, 300), (16384, 40, 1000), (16448, 600, 77)])
+def test_gemm_bf16_streaming_path(gpu_lib, ta, mode, M, N, K):
+    """Shapes with a skinny right operand and >= 256 row blocks take the
+    A-streaming kernel (A read once, all N columns per workgroup, more than one
+    column block when N > 576, unaligned rows when K % 4 != 0).  bf16: equal to
+    the exact product of the bf16-rounded operands up to fp32 accumulation;
+    bf16x3: fp32-accurate."""
+    rng = np.random.default_rng(M + N + K)
+    A = rng.standard_normal((K, M) if ta == "T" else (M, K)).astype(np.float32)
+    B = rng.standard_normal((K, N)).astype(np.float32)
+    C0 = rng.standard_normal((M, N)).astype(np.float32)
+    dA, dB, dC = (gpu_lib.DeviceMatrix.from_host(x) for x in (A, B, C0))
+    gpu_lib.dev_gemm(ta, "N", dA, dB, dC, alpha=0.75, beta=0.5, mode=mode)
+    got = dC.to_host().astype(np.float64)
+    opA = A.T if ta == "T" else A
+    if mode == "bf16":
+        ea, eb = bf16_round(opA), bf16_round(B)
+        want = 0.75 * (ea @ eb) + 0.5 * C0.astype(np.float64)
+        bound = 2 * K * 2.0**-24 * (np.abs(ea) @ np.abs(eb)).max() + 1e-6 * np.abs(C0).max()
+    else:
+        ea, eb = opA.astype(np.float64), B.astype(np.float64)
+        want = 0.75 * (ea @ eb) + 0.5 * C0.astype(np.float64)
+        bound = 8 * K * 2.0**-24 * (np.abs(ea) @ np.abs(eb)).max() + 1e-6 * np.abs(C0).max()
+    assert np.abs(got - want).max() <= bound

@@ -1,0 +1,108 @@
+"""Learned (q, rank) front end on the GPU.
+
+* the HIP forward pass and gradients of the perceptron against the NumPy
+  oracle (oracle/mlp.py, itself pinned by finite differences): outputs to
+  1e-4 of their scale, gradients to 1e-4 relative (float32 kernels vs float64);
+* GPU training (Adam, the library's kernels) learns a synthetic rule;
+* the learned low-rank path on matrices of the power-law family: every
+  accepted result meets TOL (FP64 residual, checked here explicitly too), and
+  the fallback makes the path TOL-safe whatever the network predicts.
+"""
+import numpy as np
+import pytest
+
+from oracle import mlp
+
+pytestmark = pytest.mark.gpu
+
+
+def _theta(seed):
+    rng = np.random.default_rng(seed)
+    return (mlp.init_params(seed) + 0.05 * rng.standard_normal(mlp.n_params())).astype(np.float32)
+
+
+def test_mlp_param_count(gpu_lib):
+    assert gpu_lib.load().rsvd_mlp_num_params() == mlp.n_params()
+
+
+def test_mlp_forward_matches_oracle(gpu_lib):
+    rp = gpu_lib
+    lib = rp.load()
+    th = _theta(1)
+    X = np.random.default_rng(2).standard_normal((37, 64)).astype(np.float32)
+    dth = rp.DeviceMatrix.from_host(th.reshape(-1, 1))
+    dX = rp.DeviceMatrix.from_host(X.reshape(-1, 1))
+    dO = rp.DeviceMatrix(37 * 6, 1, np.float32)
+    rp._check(lib.rsvd_dev_mlp_forward(dth.ptr, dX.ptr, 37, dO.ptr))
+    got = dO.to_host().reshape(37, 6)
+    want, _ = mlp.forward(th.astype(np.float64), X.astype(np.float64))
+    assert np.abs(got - want).max() <= 1e-4 * (1 + np.abs(want).max())
+
+
+def test_mlp_gradient_matches_oracle(gpu_lib):
+    rp = gpu_lib
+    lib = rp.load()
+    th = _theta(3)
+    rng = np.random.default_rng(4)
+    B = 45
+    X = rng.standard_normal((B, 64)).astype(np.float32)
+    q = rng.integers(0, 5, B).astype(np.int32)
+    r = rng.random(B).astype(np.float32)
+    dth = rp.DeviceMatrix.from_host(th.reshape(-1, 1))
+    dX = rp.DeviceMatrix.from_host(X.reshape(-1, 1))
+    dq = rp.DeviceMatrix.from_host(q.view(np.float32).reshape(-1, 1))
+    dr = rp.DeviceMatrix.from_host(r.reshape(-1, 1))
+    dg = rp.DeviceMatrix(mlp.n_params(), 1, np.float32)
+    dl = rp.DeviceMatrix(B, 1, np.float32)
+    rp._check(lib.rsvd_dev_mlp_grad(dth.ptr, dX.ptr, dq.ptr, dr.ptr, 3.0, B, dg.ptr, dl.ptr))
+    g = dg.to_host()[:, 0].astype(np.float64)
+    loss, gref = mlp.loss_and_grad(th.astype(np.float64), X.astype(np.float64), q, r, 3.0)
+    assert abs(dl.to_host()[:, 0].mean() - loss) < 1e-4 * (1 + abs(loss))
+    assert np.linalg.norm(g - gref) <= 1e-4 * np.linalg.norm(gref)
+    # every parameter block is exercised (no silently zero layer)
+    for o, i in mlp.SIZES:
+        pass
+    assert np.count_nonzero(g) > 0.9 * mlp.n_params()
+
+
+def test_mlp_training_learns_a_rule(gpu_lib):
+    rp = gpu_lib
+    lib = rp.load()
+    rng = np.random.default_rng(5)
+    n = 512
+    X = rng.standard_normal((n, 64)).astype(np.float32)
+    Wq = rng.standard_normal((5, 64))
+    q = np.argmax(X @ Wq.T, axis=1).astype(np.int32)
+    r = (1 / (1 + np.exp(-X[:, :4].sum(axis=1)))).astype(np.float32)
+    dth = rp.DeviceMatrix(mlp.n_params(), 1, np.float32)
+    rp._check(lib.rsvd_dev_mlp_init(11, dth.ptr))
+    dX = rp.DeviceMatrix.from_host(X.reshape(-1, 1))
+    dq = rp.DeviceMatrix.from_host(q.view(np.float32).reshape(-1, 1))
+    dr = rp.DeviceMatrix.from_host(r.reshape(-1, 1))
+    hist = np.zeros(150, np.float32)
+    rp._check(lib.rsvd_dev_mlp_train(dth.ptr, dX.ptr, dq.ptr, dr.ptr, n, 5.0, 150, 64, 2e-3, 3,
+                                     hist.ctypes.data))
+    assert hist[-1] < 0.25 * hist[0]
+    th = dth.to_host()[:, 0].astype(np.float64)
+    out, _ = mlp.forward(th, X.astype(np.float64))
+    assert np.mean(np.argmax(out[:, :5], axis=1) == q) > 0.9
+    assert np.sqrt(np.mean((out[:, 5] - r) ** 2)) < 0.1
+
+
+def test_learned_lowrank_meets_tol(gpu_lib):
+    from lowrankmatrixdecompositioncodes_amd import learned
+    ds = learned.build_dataset(24, seed=77)
+    assert set(np.unique(ds.q)) <= set(range(5))
+    theta, hist = learned.train(ds, epochs=300, batch=24)
+    assert hist[-1] < hist[0]
+    res = learned.evaluate(theta, 100, 6, seed=77)
+    for r in res:
+        assert r["resid_rel"] <= learned.TOL_DEF * (1 + 1e-9), r
+    print("learned vs plain:", [(round(r["plain_seconds"] / r["seconds"], 2), r["fell_back"])
+                                for r in res])
+    # the FP64 residual the path reports equals the explicit ||A - U S V^T||_F
+    fm = learned.family(100, 77)
+    out = learned.learned_lowrank(theta, fm, keep=True)
+    A = fm.A64.to_host()
+    expl = np.linalg.norm(A - (out["U"] * out["sigma"]) @ out["V"].T) / np.linalg.norm(A)
+    assert abs(expl - out["resid_rel"]) <= 1e-6 * learned.TOL_DEF + 1e-3 * expl
