@@ -924,39 +924,45 @@ int rsvd_dev_learned_lowrank(const float* dTheta, const double* dA, const float*
   if (k > mn - p) k = mn - p;
   if (q_pred) *q_pred = q;
   if (k_pred) *k_pred = (int)k;
+  // a predicted rank at the sketch limit means the fixed-rank solve cannot
+  // reach TOL: go to the adaptive method directly
+  const bool try_fixed = k < mn - p;
   // one fixed-rank rSVD, judged by its FP64 residual
   double *u = nullptr, *sv = nullptr, *v = nullptr;
-  RSVD_HIP(hipMalloc(&u, sizeof(double) * (size_t)(m * k)));
-  RSVD_HIP(hipMalloc(&sv, sizeof(double) * (size_t)k));
-  RSVD_HIP(hipMalloc(&v, sizeof(double) * (size_t)(n * k)));
   OmegaSrc om;
   om.seed = seed;
-  int rc = fixed_rank_f64(c, dA, m, n, lda, (int)k, p, 2, q + 1, 1, om, u, m, sv, v, n, nullptr);
-  double s2 = 0;
-  if (rc == 0) rc = sumsq_f64(c, k, 1, sv, k, nrm.p);
-  if (rc == 0 && hipMemcpyAsync(&s2, nrm.p, sizeof s2, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
-    rc = set_error("learned: copy failed");
-  if (rc == 0) rc = stream_sync(c);
-  if (rc) {
+  if (try_fixed) {
+    RSVD_HIP(hipMalloc(&u, sizeof(double) * (size_t)(m * k)));
+    RSVD_HIP(hipMalloc(&sv, sizeof(double) * (size_t)k));
+    RSVD_HIP(hipMalloc(&v, sizeof(double) * (size_t)(n * k)));
+    int rc = fixed_rank_f64(c, dA, m, n, lda, (int)k, p, 2, q + 1, 1, om, u, m, sv, v, n, nullptr);
+    double s2 = 0;
+    if (rc == 0) rc = sumsq_f64(c, k, 1, sv, k, nrm.p);
+    if (rc == 0 && hipMemcpyAsync(&s2, nrm.p, sizeof s2, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+      rc = set_error("learned: copy failed");
+    if (rc == 0) rc = stream_sync(c);
+    if (rc) {
+      (void)hipFree(u);
+      (void)hipFree(sv);
+      (void)hipFree(v);
+      return rc;
+    }
+    const double res = sqrt(a2 > s2 ? a2 - s2 : 0.0);
+    if (res <= tol_rel * sqrt(a2)) {
+      *frank = (int)k;
+      *dU = u;
+      *dS = sv;
+      *dV = v;
+      if (fell_back) *fell_back = 0;
+      if (resid_rel) *resid_rel = res / sqrt(a2);
+      return 0;
+    }
     (void)hipFree(u);
     (void)hipFree(sv);
     (void)hipFree(v);
-    return rc;
   }
-  const double res = sqrt(a2 > s2 ? a2 - s2 : 0.0);
-  if (res <= tol_rel * sqrt(a2)) {
-    *frank = (int)k;
-    *dU = u;
-    *dS = sv;
-    *dV = v;
-    if (fell_back) *fell_back = 0;
-    if (resid_rel) *resid_rel = res / sqrt(a2);
-    return 0;
-  }
-  (void)hipFree(u);
-  (void)hipFree(sv);
-  (void)hipFree(v);
   // fallback: the adaptive blocked method in TOL mode
+  double s2 = 0;
   BlockrandOut bo;
   RSVD_TRY(blockrand_f64(c, dA, m, n, lda, -1, p, tol_rel * sqrt(a2), 2, kstep, q_fallback, 1, om,
                          &bo));
@@ -970,8 +976,6 @@ int rsvd_dev_learned_lowrank(const float* dTheta, const double* dA, const float*
     RSVD_HIP(hipMemcpyAsync(v, bo.V.p, sizeof(double) * (size_t)n * fr, hipMemcpyDeviceToDevice, c->stream));
     RSVD_TRY(sumsq_f64(c, fr, 1, sv, fr, nrm.p));
     RSVD_HIP(hipMemcpyAsync(&s2, nrm.p, sizeof s2, hipMemcpyDeviceToHost, c->stream));
-  } else {
-    s2 = 0;
   }
   RSVD_TRY(stream_sync(c));
   *frank = fr;

@@ -66,49 +66,45 @@ __device__ __forceinline__ double rsq_nr(double x) {
 // Rotation (c, s) with [x' y'] = [x y] [[c, s], [-s, c]] zeroing the (p, q)
 // entry of a symmetric 2x2 [[a, g], [g, b]]: t = tan theta is the root of
 // g t^2 + (b - a) t - g = 0 of smaller modulus, t = sgn(d) 2g / (|d| +
-// sqrt(d^2 + 4g^2)), d = b - a (Rutishauser's choice).  No rotation when
-// |g| <= flo or |g| <= kRotTol sqrt|ab|; *sig reports |g| > kStopTol sqrt|ab|.
+// sqrt(d^2 + 4g^2)), d = b - a (Rutishauser's choice), evaluated on the
+// entries scaled by a power of two so that d^2 + 4g^2 stays in range.
+// Branch-free.  No rotation when |g| <= flo or |g| <= kRotTol sqrt|ab|; *sig
+// reports |g| > kStopTol sqrt|ab|.
 __device__ __forceinline__ void rotation(double a, double b, double g, double flo, double* cs,
                                          double* sn, int* sig) {
-  *cs = 1.0;
-  *sn = 0.0;
-  *sig = 0;
   const double ag = fabs(g);
-  if (!(ag > flo)) return;
-  const double ab = fabs(a * b);
-  const double g2 = g * g;
-  *sig = g2 > kStopTol * kStopTol * ab;
-  if (!(g2 > kRotTol * kRotTol * ab)) return;
-  const double d = b - a;
-  double t;
-  if (fabs(d) < 1e100 && ag < 1e100) {
-    const double r2 = fma(d, d, 4.0 * g2);
-    const double den = fabs(d) + r2 * rsq_nr(r2);
-    t = 2.0 * g * rcp_nr(den);
-    if (d < 0.0) t = -t;
-  } else {
-    const double zeta = d / (2.0 * g);
-    t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-  }
+  int e;
+  (void)frexp(fmax(fmax(fabs(a), fabs(b)), ag), &e);
+  const double sa = ldexp(a, -e), sb = ldexp(b, -e), sg = ldexp(g, -e);
+  const double ab = fabs(sa * sb), g2 = sg * sg;
+  const bool big = ag > flo;
+  *sig = (big && g2 > kStopTol * kStopTol * ab) ? 1 : 0;
+  const bool rot = big && g2 > kRotTol * kRotTol * ab;
+  const double d = sb - sa;
+  const double r2 = rot ? fma(d, d, 4.0 * g2) : 1.0;
+  const double den = fabs(d) + r2 * rsq_nr(r2);
+  double t = 2.0 * sg * rcp_nr(den);
+  t = (d < 0.0) ? -t : t;
   const double c = rsq_nr(fma(t, t, 1.0));
-  *cs = c;
-  *sn = c * t;
+  *cs = rot ? c : 1.0;
+  *sn = rot ? c * t : 0.0;
 }
 
 // One workgroup per block pair: H = G(cols, cols), one inner sweep, J out.
-// Thread t owns, in every step, the 2 x 2 blocks e = t + 256 u of H (rows of
-// local pair e / B, columns of local pair e % B; the blocks partition H) and,
-// for each, rows 2 (e % B), 2 (e % B) + 1 of J's columns of pair e / B.  H
-// is double-buffered, so a step needs a single barrier.
+// A step: threads 0..B-1 compute the rotation of local pair tid once and
+// publish it; after a barrier thread t = (bi, bj) (bi = t / B, bj = t % B)
+// applies R_bi^T . R_bj to the 2 x 2 block of H at (pair bi rows, pair bj
+// columns) -- the blocks partition H -- and R_bi to rows 2 bj, 2 bj + 1 of J's
+// columns of pair bi.  H is double-buffered.
 template <int B>
 __global__ void __launch_bounds__(256)
 symj_solve_kernel(int64_t n, const double* __restrict__ G, int64_t ldg, int nblocks, int N,
                   int round, const double* __restrict__ fro2, double* __restrict__ Jout,
                   unsigned long long* __restrict__ offmax) {
   constexpr int S = 2 * B, HP = S + 1;
-  constexpr int BPT = (B * B) / 256;       // 2x2 blocks per thread
-  static_assert(BPT >= 1 && BPT * 256 == B * B, "B * B must be a multiple of 256");
+  static_assert(B * B == 256, "one 2x2 block per thread");
   __shared__ double Ha[S * HP], Hb[S * HP], Jl[S * HP];
+  __shared__ double rc[B], rs[B];
   __shared__ int64_t cols[S];
   const int w = blockIdx.x;
   const int tid = threadIdx.x;
@@ -124,41 +120,42 @@ symj_solve_kernel(int64_t n, const double* __restrict__ G, int64_t ldg, int nblo
   __syncthreads();
 
   int any_sig = 0;
+  const int bi = tid / B, bj = tid % B;
   double* Hc = Ha;
   double* Hn = Hb;
   for (int t = 0; t < S - 1; ++t) {
-#pragma unroll
-    for (int u = 0; u < BPT; ++u) {
-      const int e = tid + 256 * u;
-      const int bi = e / B, bj = e % B;
-      const int pi = player(bi, t, S), qi = player(S - 1 - bi, t, S);
-      const int pj = player(bj, t, S), qj = player(S - 1 - bj, t, S);
-      double ci, si, cj, sj;
-      int gi, gj;
-      rotation(Hc[pi * HP + pi], Hc[qi * HP + qi], Hc[pi * HP + qi], flo, &ci, &si, &gi);
-      rotation(Hc[pj * HP + pj], Hc[qj * HP + qj], Hc[pj * HP + qj], flo, &cj, &sj, &gj);
-      any_sig |= gi;
-      const double h00 = Hc[pi * HP + pj], h01 = Hc[pi * HP + qj];
-      const double h10 = Hc[qi * HP + pj], h11 = Hc[qi * HP + qj];
-      // X = H_blk R_j, then R_i^T X
-      const double x00 = cj * h00 - sj * h01, x01 = sj * h00 + cj * h01;
-      const double x10 = cj * h10 - sj * h11, x11 = sj * h10 + cj * h11;
-      double y00 = ci * x00 - si * x10, y01 = ci * x01 - si * x11;
-      double y10 = si * x00 + ci * x10, y11 = si * x01 + ci * x11;
-      if (bi == bj && si != 0.0) y01 = y10 = 0.0;  // the annihilated pair
-      Hn[pi * HP + pj] = y00;
-      Hn[pi * HP + qj] = y01;
-      Hn[qi * HP + pj] = y10;
-      Hn[qi * HP + qj] = y11;
-      // J <- J R_i on the columns of pair bi, rows 2 bj and 2 bj + 1 (over bj
-      // these cover every row once)
-#pragma unroll
-      for (int x = 2 * bj; x < 2 * bj + 2; ++x) {
-        const double jp = Jl[x * HP + pi], jq = Jl[x * HP + qi];
-        Jl[x * HP + pi] = ci * jp - si * jq;
-        Jl[x * HP + qi] = si * jp + ci * jq;
-      }
+    if (tid < B) {
+      const int p = player(tid, t, S), q = player(S - 1 - tid, t, S);
+      double c, sn;
+      int sg;
+      rotation(Hc[p * HP + p], Hc[q * HP + q], Hc[p * HP + q], flo, &c, &sn, &sg);
+      rc[tid] = c;
+      rs[tid] = sn;
+      any_sig |= sg;
     }
+    __syncthreads();
+    const int pi = player(bi, t, S), qi = player(S - 1 - bi, t, S);
+    const int pj = player(bj, t, S), qj = player(S - 1 - bj, t, S);
+    const double ci = rc[bi], si = rs[bi], cj = rc[bj], sj = rs[bj];
+    const double h00 = Hc[pi * HP + pj], h01 = Hc[pi * HP + qj];
+    const double h10 = Hc[qi * HP + pj], h11 = Hc[qi * HP + qj];
+    const int x0 = 2 * bj;
+    const double j00 = Jl[x0 * HP + pi], j01 = Jl[x0 * HP + qi];
+    const double j10 = Jl[(x0 + 1) * HP + pi], j11 = Jl[(x0 + 1) * HP + qi];
+    // X = H_blk R_j, then R_i^T X
+    const double x00 = cj * h00 - sj * h01, x01 = sj * h00 + cj * h01;
+    const double x10 = cj * h10 - sj * h11, x11 = sj * h10 + cj * h11;
+    double y00 = ci * x00 - si * x10, y01 = ci * x01 - si * x11;
+    double y10 = si * x00 + ci * x10, y11 = si * x01 + ci * x11;
+    if (bi == bj && si != 0.0) y01 = y10 = 0.0;  // the annihilated pair
+    Hn[pi * HP + pj] = y00;
+    Hn[pi * HP + qj] = y01;
+    Hn[qi * HP + pj] = y10;
+    Hn[qi * HP + qj] = y11;
+    Jl[x0 * HP + pi] = ci * j00 - si * j01;
+    Jl[x0 * HP + qi] = si * j00 + ci * j01;
+    Jl[(x0 + 1) * HP + pi] = ci * j10 - si * j11;
+    Jl[(x0 + 1) * HP + qi] = si * j10 + ci * j11;
     __syncthreads();
     double* tmp = Hc;
     Hc = Hn;

@@ -89,16 +89,24 @@ def plain_adaptive(A64: rp.DeviceMatrix, tol_rel: float, q: int, kstep: int = KS
 
 
 def label(fm: FamilyMatrix, tol_rel: float = TOL_DEF, kstep: int = KSTEP_DEF, p: int = P_DEF):
-    """(q*, rank fraction, frank per q) from plain adaptive runs for q = 0..4."""
+    """(q*, rank fraction, frank per q) from plain adaptive runs for q = 0..4.
+
+    Runs that miss TOL (TOL out of reach below full rank: the noise floor's
+    energy alone exceeds it) cost as if they had gone to full rank; when every
+    q misses, the rank label is 1 (the network then learns to send such
+    matrices straight to the adaptive method)."""
     m, n = fm.A64.rows, fm.A64.cols
+    mn = min(m, n)
     anorm = frob(fm.A64)
-    franks = []
+    franks, ok = [], []
     for q in range(5):
-        fr, _, _ = plain_adaptive(fm.A64, tol_rel, q, kstep, p, seed=fm.seed, anorm=anorm)
+        fr, sig, _ = plain_adaptive(fm.A64, tol_rel, q, kstep, p, seed=fm.seed, anorm=anorm)
+        res = np.sqrt(max(anorm ** 2 - float(np.sum(sig ** 2)), 0.0)) / anorm
         franks.append(fr)
-    cost = [(2 * q + 2) * (franks[q] + kstep) for q in range(5)]
+        ok.append(res <= tol_rel * (1 + 1e-9))
+    cost = [(2 * q + 2) * ((franks[q] if ok[q] else mn) + kstep) for q in range(5)]
     qs = int(np.argmin(cost))
-    return qs, franks[qs] / min(m, n), franks
+    return qs, (franks[qs] / mn if ok[qs] else 1.0), franks
 
 
 def features(fm: FamilyMatrix) -> np.ndarray:

@@ -96,10 +96,17 @@ def test_learned_lowrank_meets_tol(gpu_lib):
     theta, hist = learned.train(ds, epochs=300, batch=24)
     assert hist[-1] < hist[0]
     res = learned.evaluate(theta, 100, 6, seed=77)
+    tol = learned.TOL_DEF * (1 + 1e-9)
     for r in res:
-        assert r["resid_rel"] <= learned.TOL_DEF * (1 + 1e-9), r
-    print("learned vs plain:", [(round(r["plain_seconds"] / r["seconds"], 2), r["fell_back"])
-                                for r in res])
+        if r["resid_rel"] > tol:
+            # only where TOL is out of reach below full rank: then the path has
+            # returned the plain method's own result (same computation; the two
+            # residuals differ only by the rounding of sqrt(||A||^2 - sum s^2),
+            # evaluated on the device and in NumPy)
+            assert r["fell_back"] and r["plain_resid"] > tol, r
+            assert abs(r["resid_rel"] - r["plain_resid"]) <= 1e-6 * r["plain_resid"], r
+    print("learned vs plain:", [(round(r["plain_seconds"] / r["seconds"], 2), r["fell_back"],
+                                 r["plain_resid"] <= tol) for r in res])
     # the FP64 residual the path reports equals the explicit ||A - U S V^T||_F
     fm = learned.family(100, 77)
     out = learned.learned_lowrank(theta, fm, keep=True)
