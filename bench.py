@@ -43,6 +43,26 @@ def falg(m: int, n: int, l: int, q: int) -> float:
             + 2.0 * n * l * l + 9.0 * l**3 + 2.0 * m * l * l + 2.0 * l**3 + 2.0 * n * l * l)
 
 
+TRAFFIC_PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                               "apass_bf16x3_65536x16384_hbm_traffic.md")
+
+
+def measured_traffic(tag, m, n, l):
+    """HBM bytes per launch of the split-bf16 A pass, from the committed
+    FETCH_SIZE + WRITE_SIZE counter runs (scaled by the copy-kernel
+    calibration); only for the shape those runs measured, else None."""
+    if tag != "apass_bf16x3" or (m, n, l) != (65536, 16384, 544):
+        return None
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch_tn:"):
+                    return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def physical_cores() -> int:
     try:
         return len(os.sched_getaffinity(0))
@@ -408,7 +428,7 @@ def main() -> int:
         roofline = {"bound": "hbm", "achieved": round(gbs, 1), "peak": 8000.0, "unit": "GB/s",
                     "mfma_tflops": round(flops / (avg * 1e-3) / 1e12, 1)}
     roofline["frac"] = round(roofline["achieved"] / roofline["peak"], 4)
-    roofline["traffic"] = None
+    roofline["traffic"] = measured_traffic(tag, m, n, l)
     roofline["kernel"] = f"{tag} ({'3x ' if tag.endswith('x3') else ''}2*m*n*l flops per launch)"
     roofline["avg_launch_ms"] = round(avg, 4)
     roofline["launches"] = cnt

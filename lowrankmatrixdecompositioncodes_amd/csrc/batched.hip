@@ -35,6 +35,7 @@
 #include <numeric>
 #include <vector>
 
+#include "jacobi_rot.h"
 #include "rng.h"
 #include "rsvd_internal.h"
 
@@ -198,50 +199,42 @@ bpass_kernel(const BMat* __restrict__ mats, const BItem* __restrict__ items, int
 }
 
 // ---------------------------------------------------------------- LDS helpers
-// Upper-triangular Gram G = X^T X (l x l of the first l columns, FP64
-// accumulation of fp32 products) of X (R x 48, ld R), staged in 64-row chunks.
+// Gram G = X^T X (the first l columns; FP64 accumulation of fp32 products)
+// of X (R x 48, ld R), staged in 64-row chunks.  Thread t owns the 3 x 3
+// block (3 (t / 16), 3 (t % 16)) of the 48 x 48 result: per row it reads six
+// values (the three of its row block are the same for the 16 threads that
+// share it, so the wave's reads broadcast) for nine independent FMAs.
 __device__ void lds_gram(const float* __restrict__ X, int64_t R, int l, float (*Xs)[kL + 1],
                          double (*G)[kLP]) {
   const int tid = threadIdx.x;
-  const int ne = l * (l + 1) / 2;
-  double acc[5] = {0, 0, 0, 0, 0};
-  int ei[5], ej[5];
-#pragma unroll
-  for (int u = 0; u < 5; ++u) {
-    int e = tid + 256 * u;
-    ei[u] = -1;
-    ej[u] = -1;
-    if (e < ne) {
-      // e -> (i, j), i <= j, row-major over the upper triangle
-      int i = 0;
-      while (e >= l - i) {
-        e -= l - i;
-        ++i;
-      }
-      ei[u] = i;
-      ej[u] = i + e;
-    }
-  }
+  const int i0 = 3 * (tid >> 4), j0 = 3 * (tid & 15);
+  double acc[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
   for (int64_t r0 = 0; r0 < R; r0 += 64) {
     const int rows = (int)((R - r0) < 64 ? (R - r0) : 64);
     __syncthreads();
-    for (int e = tid; e < 64 * l; e += 256) {
+    for (int e = tid; e < 64 * kL; e += 256) {
       const int rr = e & 63, c = e >> 6;
-      Xs[rr][c] = (rr < rows) ? X[r0 + rr + (int64_t)c * R] : 0.0f;
+      Xs[rr][c] = (rr < rows && c < l) ? X[r0 + rr + (int64_t)c * R] : 0.0f;
     }
     __syncthreads();
+    for (int rr = 0; rr < rows; ++rr) {
+      double xi[3], xj[3];
 #pragma unroll
-    for (int u = 0; u < 5; ++u) {
-      if (ei[u] < 0) continue;
-      const int i = ei[u], j = ej[u];
-      double s = 0.0;
-      for (int rr = 0; rr < rows; ++rr) s += (double)Xs[rr][i] * (double)Xs[rr][j];
-      acc[u] += s;
+      for (int a = 0; a < 3; ++a) {
+        xi[a] = (double)Xs[rr][i0 + a];
+        xj[a] = (double)Xs[rr][j0 + a];
+      }
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) acc[a][b] += xi[a] * xj[b];
     }
   }
 #pragma unroll
-  for (int u = 0; u < 5; ++u)
-    if (ei[u] >= 0) G[ei[u]][ej[u]] = G[ej[u]][ei[u]] = acc[u];
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+      if (i0 + a < l && j0 + b < l) G[i0 + a][j0 + b] = acc[a][b];
   __syncthreads();
 }
 
@@ -265,11 +258,8 @@ __device__ void lds_chol_inv(double (*G)[kLP], double (*Ri)[kLP], int l, double 
     for (int c = j + 1 + tid; c < l; c += 256) G[j][c] /= rjj;
     __syncthreads();
     // trailing update G[a][b] -= R[j][a] R[j][b], j < a <= b
-    const int w = l - j - 1;
-    for (int e = tid; e < w * w; e += 256) {
-      const int a = j + 1 + e / w, b = j + 1 + e % w;
-      if (a <= b) G[a][b] -= G[j][a] * G[j][b];
-    }
+    for (int a = j + 1 + (tid >> 4); a < l; a += 16)
+      for (int b = a + (tid & 15); b < l; b += 16) G[a][b] -= G[j][a] * G[j][b];
     __syncthreads();
   }
   // inverse of the upper R, one column per thread (back substitution)
@@ -354,10 +344,15 @@ __device__ __forceinline__ int circle(int pos, int r, int N) {
   return pos == 0 ? 0 : 1 + ((pos - 1 + r) % (N - 1));
 }
 
-// two-sided cyclic Jacobi on H (np x np, np even) with J accumulated;
-// 2 x 2 blocks (bi, bj) of the current pairing owned round-robin by threads.
+// two-sided cyclic Jacobi on H (np x np, np even, positive semidefinite) with
+// J accumulated.  A step: threads 0..np/2-1 compute the rotation of their pair
+// once (round-robin pairing) and publish it; after a barrier the 2 x 2 blocks
+// of H (pairs bi x bj) and the (row, pair) entries of J are updated in
+// parallel into the second H buffer.  Sweeps until no pair met in a sweep had
+// a relative off-diagonal above kStopTol.
 __device__ void lds_jacobi(double (*H)[kLP], double (*H2)[kLP], double (*J)[kLP], int np,
                            int* sig_flag) {
+  __shared__ double rc[kL / 2], rs[kL / 2];
   const int tid = threadIdx.x;
   const int half = np / 2;
   double (*Hc)[kLP] = H;
@@ -373,47 +368,39 @@ __device__ void lds_jacobi(double (*H)[kLP], double (*H2)[kLP], double (*J)[kLP]
     __syncthreads();
     int sig = 0;
     for (int t = 0; t < np - 1; ++t) {
+      if (tid < half) {
+        const int p = circle(tid, t, np), q = circle(np - 1 - tid, t, np);
+        double c, sn;
+        int sg;
+        rotation(Hc[p][p], Hc[q][q], Hc[p][q], flo, &c, &sn, &sg);
+        rc[tid] = c;
+        rs[tid] = sn;
+        sig |= sg;
+      }
+      __syncthreads();
       for (int e = tid; e < half * half; e += 256) {
         const int bi = e / half, bj = e % half;
         const int pi = circle(bi, t, np), qi = circle(np - 1 - bi, t, np);
         const int pj = circle(bj, t, np), qj = circle(np - 1 - bj, t, np);
-        double c[2] = {1.0, 1.0}, s[2] = {0.0, 0.0};
-        const int pp[2] = {pi, pj}, qq[2] = {qi, qj};
-#pragma unroll
-        for (int u = 0; u < 2; ++u) {
-          const double a = Hc[pp[u]][pp[u]], b = Hc[qq[u]][qq[u]], g = Hc[pp[u]][qq[u]];
-          const double ag = fabs(g);
-          if (ag > flo) {
-            const double ab = fabs(a * b);
-            if (u == 0 && g * g > 1e-20 * ab) sig = 1;
-            if (g * g > 1e-30 * ab) {
-              const double dd = b - a;
-              const double den = fabs(dd) + sqrt(dd * dd + 4.0 * g * g);
-              double tt = 2.0 * g / den;
-              if (dd < 0.0) tt = -tt;
-              c[u] = 1.0 / sqrt(1.0 + tt * tt);
-              s[u] = c[u] * tt;
-            }
-          }
-        }
+        const double ci = rc[bi], si = rs[bi], cj = rc[bj], sj = rs[bj];
         const double h00 = Hc[pi][pj], h01 = Hc[pi][qj], h10 = Hc[qi][pj], h11 = Hc[qi][qj];
-        const double x00 = c[1] * h00 - s[1] * h01, x01 = s[1] * h00 + c[1] * h01;
-        const double x10 = c[1] * h10 - s[1] * h11, x11 = s[1] * h10 + c[1] * h11;
-        double y00 = c[0] * x00 - s[0] * x10, y01 = c[0] * x01 - s[0] * x11;
-        double y10 = s[0] * x00 + c[0] * x10, y11 = s[0] * x01 + c[0] * x11;
-        if (bi == bj && s[0] != 0.0) y01 = y10 = 0.0;
+        const double x00 = cj * h00 - sj * h01, x01 = sj * h00 + cj * h01;
+        const double x10 = cj * h10 - sj * h11, x11 = sj * h10 + cj * h11;
+        double y00 = ci * x00 - si * x10, y01 = ci * x01 - si * x11;
+        double y10 = si * x00 + ci * x10, y11 = si * x01 + ci * x11;
+        if (bi == bj && si != 0.0) y01 = y10 = 0.0;
         Hn[pi][pj] = y00;
         Hn[pi][qj] = y01;
         Hn[qi][pj] = y10;
         Hn[qi][qj] = y11;
-        if (bj == 0) {
-          // J <- J R_i on columns pi, qi (this thread owns pair bi for bj == 0)
-          for (int x = 0; x < np; ++x) {
-            const double jp = J[x][pi], jq = J[x][qi];
-            J[x][pi] = c[0] * jp - s[0] * jq;
-            J[x][qi] = s[0] * jp + c[0] * jq;
-          }
-        }
+      }
+      for (int e = tid; e < np * half; e += 256) {
+        const int x = e % np, k = e / np;
+        const int pk = circle(k, t, np), qk = circle(np - 1 - k, t, np);
+        const double ck = rc[k], sk = rs[k];
+        const double jp = J[x][pk], jq = J[x][qk];
+        J[x][pk] = ck * jp - sk * jq;
+        J[x][qk] = sk * jp + ck * jq;
       }
       __syncthreads();
       double (*tmp)[kLP] = Hc;

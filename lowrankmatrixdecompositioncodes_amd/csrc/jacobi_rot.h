@@ -1,0 +1,60 @@
+// jacobi_rot.h -- the 2x2 symmetric Jacobi rotation shared by the device
+// eigensolvers (symeig.hip, batched.hip).
+#pragma once
+#include <cmath>
+
+#include <hip/hip_runtime.h>
+
+namespace rsvd {
+
+// no rotation below this relative size of the off-diagonal
+constexpr double kRotTol = 1e-15;
+// a sweep that met no off-diagonal above this relative size ends the iteration
+constexpr double kStopTol = 1e-10;
+
+// 1/x and 1/sqrt(x) from the hardware estimates plus two Newton steps
+// (error a few ulp; a rotation only needs c^2 + s^2 = 1 to rounding).
+static __device__ __forceinline__ double rcp_nr(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, y, 1.0);
+  y = fma(y, e, y);
+  e = fma(-x, y, 1.0);
+  return fma(y, e, y);
+}
+static __device__ __forceinline__ double rsq_nr(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  const double h = 0.5 * x;
+  double e = fma(-h * y, y, 0.5);
+  y = fma(y, e, y);
+  e = fma(-h * y, y, 0.5);
+  return fma(y, e, y);
+}
+
+// Rotation (c, s) with [x' y'] = [x y] [[c, s], [-s, c]] zeroing the (p, q)
+// entry of a symmetric 2x2 [[a, g], [g, b]]: t = tan theta is the root of
+// g t^2 + (b - a) t - g = 0 of smaller modulus, t = sgn(d) 2g / (|d| +
+// sqrt(d^2 + 4g^2)), d = b - a (Rutishauser's choice), evaluated on the
+// entries scaled by a power of two so that d^2 + 4g^2 stays in range.
+// Branch-free.  No rotation when |g| <= flo or |g| <= kRotTol sqrt|ab|; *sig
+// reports |g| > kStopTol sqrt|ab|.
+static __device__ __forceinline__ void rotation(double a, double b, double g, double flo, double* cs,
+                                         double* sn, int* sig) {
+  const double ag = fabs(g);
+  int e;
+  (void)frexp(fmax(fmax(fabs(a), fabs(b)), ag), &e);
+  const double sa = ldexp(a, -e), sb = ldexp(b, -e), sg = ldexp(g, -e);
+  const double ab = fabs(sa * sb), g2 = sg * sg;
+  const bool big = ag > flo;
+  *sig = (big && g2 > kStopTol * kStopTol * ab) ? 1 : 0;
+  const bool rot = big && g2 > kRotTol * kRotTol * ab;
+  const double d = sb - sa;
+  const double r2 = rot ? fma(d, d, 4.0 * g2) : 1.0;
+  const double den = fabs(d) + r2 * rsq_nr(r2);
+  double t = 2.0 * sg * rcp_nr(den);
+  t = (d < 0.0) ? -t : t;
+  const double c = rsq_nr(fma(t, t, 1.0));
+  *cs = rot ? c : 1.0;
+  *sn = rot ? c * t : 0.0;
+}
+
+}  // namespace rsvd

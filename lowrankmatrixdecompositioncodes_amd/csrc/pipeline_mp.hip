@@ -11,8 +11,8 @@
 //     for the final Q;
 //   * projection B^T = A^T Q: split-bf16 (fp32-level accuracy), because every
 //     singular value is read off B;
-//   * BB^T, its eigendecomposition (one-sided Jacobi) and sqrt: FP64;
-//   * U = Q Uhat and V = B^T Uhat Sigma^-1: fp32 matrix cores.
+//   * BB^T, its eigendecomposition (two-sided block Jacobi) and sqrt: FP64;
+//   * U = Q Uhat and V = B^T Uhat Sigma^-1: split-bf16 (fp32-level accuracy).
 // B^T (n x l) is formed instead of B (l x n): the A pass then has the same
 // A^T-times-skinny shape as the power iteration.
 #include <cmath>
@@ -131,7 +131,7 @@ int fixed_rank_mp(Context* c, const float* A, int64_t m, int64_t n, int64_t lda,
     RSVD_TRY(colnorms_f64(c, l, l, R.p, l, sig.p, 1));
     RSVD_TRY(sort_pairs_f64(c, l, sig.p, 1, R.p, l, l, Vh.p, l, l));
     RSVD_TRY(f64_to_f32(c, l, k, Vh.p, l, X32.p, l));
-    RSVD_TRY(gemm_f32(c, 'N', 'N', m, k, l, 1.0f, Y.p, m, X32.p, l, 0.0f, U, ldu));
+    RSVD_TRY(gemm_bf16(c, "uv_bf16x3", 'N', 'N', m, k, l, 1.0f, Y.p, m, X32.p, l, 0.0f, U, ldu, 3));
     DBuf<double> V64;
     RSVD_TRY(V64.alloc((size_t)(n * k)));
     RSVD_TRY(gemm_f64(c, 'N', 'N', n, k, l, 1.0, Bt64.p, n, R.p, l, 0.0, V64.p, n));
@@ -147,10 +147,10 @@ int fixed_rank_mp(Context* c, const float* A, int64_t m, int64_t n, int64_t lda,
     RSVD_TRY(sqrt_f64(c, l, lam.p, sig.p));
     const int64_t off = l - k;
     RSVD_TRY(f64_to_f32(c, l, k, Uh.p + off * l, l, X32.p, l));
-    RSVD_TRY(gemm_f32(c, 'N', 'N', m, k, l, 1.0f, Y.p, m, X32.p, l, 0.0f, U, ldu));
+    RSVD_TRY(gemm_bf16(c, "uv_bf16x3", 'N', 'N', m, k, l, 1.0f, Y.p, m, X32.p, l, 0.0f, U, ldu, 3));
     RSVD_TRY(scale_cols_f64(c, l, l, Uh.p, l, sig.p, 1));
     RSVD_TRY(f64_to_f32(c, l, k, Uh.p + off * l, l, X32.p, l));
-    RSVD_TRY(gemm_f32(c, 'N', 'N', n, k, l, 1.0f, Bt.p, n, X32.p, l, 0.0f, V, ldv));
+    RSVD_TRY(gemm_bf16(c, "uv_bf16x3", 'N', 'N', n, k, l, 1.0f, Bt.p, n, X32.p, l, 0.0f, V, ldv, 3));
     RSVD_HIP(hipMemcpyAsync(S, sig.p + off, sizeof(double) * k, hipMemcpyDeviceToDevice,
                             c->stream));
   }

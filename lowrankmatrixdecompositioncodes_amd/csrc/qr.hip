@@ -19,20 +19,32 @@ constexpr double kCholRelTol = 1e-13;   // pivot / original diagonal
 
 // ---------------------------------------------------------------- Cholesky
 // Factor the kb x kb diagonal block at (p,p) of G in LDS (upper, G = R^T R),
-// then (every workgroup) solve R_pp^T X = G[p:p+kb, cols] for its chunk of the
-// row panel.  Out of place: G is only read here, R only written, so the
-// workgroups never race on the diagonal block.  Workgroup 0 stores it.
+// then (every workgroup) solve R_pp^T X = G[p:p+kb, cols] for its 256-column
+// chunk of the row panel.  The chunk is staged through LDS with coalesced
+// loads and stores (each column of the panel is kb contiguous doubles), and
+// the solves multiply by the reciprocal pivots.  Out of place: G is only read
+// here, R only written, so the workgroups never race on the diagonal block.
+// Workgroup 0 stores it.
 __global__ void __launch_bounds__(256)
 chol_panel_kernel(int64_t n, const double* __restrict__ G, double* __restrict__ R, int64_t p,
                   int kb, const double* __restrict__ diag0, int* __restrict__ fail) {
   __shared__ double D[NB][NB + 1];
+  __shared__ double inv[NB];
+  __shared__ double P[256][NB + 1];
+  __shared__ int bad;
   const int tid = threadIdx.x;
   for (int e = tid; e < kb * kb; e += 256) {
     int i = e % kb, j = e / kb;
     D[i][j] = (i <= j) ? G[(p + i) + (p + j) * n] : 0.0;
   }
-  __syncthreads();
-  __shared__ int bad;
+  // stage this workgroup's chunk of the row panel: column cc of the chunk is
+  // G[p : p + kb, c0 + cc]
+  const int64_t c0 = p + kb + (int64_t)blockIdx.x * 256;
+  const int64_t ncols = (n - c0) < 256 ? (n - c0) : 256;
+  for (int e = tid; e < NB * 256; e += 256) {
+    const int j = e % NB, cc = e / NB;
+    P[cc][j] = (j < kb && cc < ncols) ? G[(p + j) + (c0 + cc) * n] : 0.0;
+  }
   if (tid == 0) bad = 0;
   __syncthreads();
   for (int j = 0; j < kb; ++j) {
@@ -43,11 +55,13 @@ chol_panel_kernel(int64_t n, const double* __restrict__ G, double* __restrict__ 
         bad = 1;
         d = (d > 0 && isfinite(d)) ? d : 1.0;
       }
-      D[j][j] = sqrt(d);
+      const double r = sqrt(d);
+      D[j][j] = r;
+      inv[j] = 1.0 / r;
     }
     __syncthreads();
-    const double r = D[j][j];
-    for (int c = j + 1 + tid; c < kb; c += 256) D[j][c] /= r;
+    const double ri = inv[j];
+    for (int c = j + 1 + tid; c < kb; c += 256) D[j][c] *= ri;
     __syncthreads();
     // trailing update of the upper triangle of D
     for (int e = tid; e < kb * kb; e += 256) {
@@ -63,21 +77,25 @@ chol_panel_kernel(int64_t n, const double* __restrict__ G, double* __restrict__ 
     }
     if (tid == 0 && bad) atomicOr(fail, 1);
   }
-  // row panel: columns p+kb+blockIdx.x*256+tid
-  const int64_t col = p + kb + (int64_t)blockIdx.x * 256 + tid;
-  if (col < n) {
+  // forward substitution R_pp^T x = g for column tid of the chunk, in LDS
+  if (tid < ncols) {
     double x[NB];
 #pragma unroll
     for (int j = 0; j < NB; ++j) {
       if (j < kb) {
-        double s = G[(p + j) + col * n];
+        double s = P[tid][j];
         for (int i = 0; i < j; ++i) s -= D[i][j] * x[i];
-        x[j] = s / D[j][j];
+        x[j] = s * inv[j];
       }
     }
 #pragma unroll
     for (int j = 0; j < NB; ++j)
-      if (j < kb) R[(p + j) + col * n] = x[j];
+      if (j < kb) P[tid][j] = x[j];
+  }
+  __syncthreads();
+  for (int e = tid; e < NB * 256; e += 256) {
+    const int j = e % NB, cc = e / NB;
+    if (j < kb && cc < ncols) R[(p + j) + (c0 + cc) * n] = P[cc][j];
   }
 }
 

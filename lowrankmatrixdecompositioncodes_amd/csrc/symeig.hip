@@ -6,9 +6,10 @@
 // a round pairs the blocks with the circle method (every block meets every
 // other once per sweep).  A round is two launches:
 //   solve : one workgroup per block pair loads the pair's S x S (S = 2B)
-//           diagonal tile H of G and runs one cyclic inner sweep of two-sided
-//           Jacobi on it (B disjoint rotations per step, S - 1 steps),
-//           accumulating the S x S orthogonal J with H <- J^T H J;
+//           diagonal tile H of G and runs an inner sweep of two-sided Jacobi
+//           on it (B disjoint rotations per step; all pairs in the first round
+//           of a sweep, only the cross-block pairs afterwards), accumulating
+//           the S x S orthogonal J with H <- J^T H J;
 //   apply : every S x S tile of G in pair order becomes J_a^T T J_b (the
 //           upper tiles are computed and mirrored, so G stays exactly
 //           symmetric) and every S-row strip of V's pair columns becomes
@@ -23,14 +24,13 @@
 #include <cmath>
 #include <cstring>
 
+#include "jacobi_rot.h"
 #include "rsvd_internal.h"
 
 namespace rsvd {
 
 namespace {
 
-constexpr double kRotTol = 1e-15;
-constexpr double kStopTol = 1e-10;
 
 __device__ __forceinline__ int player(int pos, int r, int N) {
   return pos == 0 ? 0 : 1 + ((pos - 1 + r) % (N - 1));
@@ -45,57 +45,31 @@ __device__ __forceinline__ int64_t pair_col(int w, int s, int r, int N, int nblo
   return (blk < nblocks && col < n) ? col : -1;
 }
 
-// 1/x and 1/sqrt(x) from the hardware estimates plus two Newton steps
-// (error a few ulp; a rotation only needs c^2 + s^2 = 1 to rounding).
-__device__ __forceinline__ double rcp_nr(double x) {
-  double y = __builtin_amdgcn_rcp(x);
-  double e = fma(-x, y, 1.0);
-  y = fma(y, e, y);
-  e = fma(-x, y, 1.0);
-  return fma(y, e, y);
-}
-__device__ __forceinline__ double rsq_nr(double x) {
-  double y = __builtin_amdgcn_rsq(x);
-  const double h = 0.5 * x;
-  double e = fma(-h * y, y, 0.5);
-  y = fma(y, e, y);
-  e = fma(-h * y, y, 0.5);
-  return fma(y, e, y);
-}
-
-// Rotation (c, s) with [x' y'] = [x y] [[c, s], [-s, c]] zeroing the (p, q)
-// entry of a symmetric 2x2 [[a, g], [g, b]]: t = tan theta is the root of
-// g t^2 + (b - a) t - g = 0 of smaller modulus, t = sgn(d) 2g / (|d| +
-// sqrt(d^2 + 4g^2)), d = b - a (Rutishauser's choice), evaluated on the
-// entries scaled by a power of two so that d^2 + 4g^2 stays in range.
-// Branch-free.  No rotation when |g| <= flo or |g| <= kRotTol sqrt|ab|; *sig
-// reports |g| > kStopTol sqrt|ab|.
-__device__ __forceinline__ void rotation(double a, double b, double g, double flo, double* cs,
-                                         double* sn, int* sig) {
-  const double ag = fabs(g);
-  int e;
-  (void)frexp(fmax(fmax(fabs(a), fabs(b)), ag), &e);
-  const double sa = ldexp(a, -e), sb = ldexp(b, -e), sg = ldexp(g, -e);
-  const double ab = fabs(sa * sb), g2 = sg * sg;
-  const bool big = ag > flo;
-  *sig = (big && g2 > kStopTol * kStopTol * ab) ? 1 : 0;
-  const bool rot = big && g2 > kRotTol * kRotTol * ab;
-  const double d = sb - sa;
-  const double r2 = rot ? fma(d, d, 4.0 * g2) : 1.0;
-  const double den = fabs(d) + r2 * rsq_nr(r2);
-  double t = 2.0 * sg * rcp_nr(den);
-  t = (d < 0.0) ? -t : t;
-  const double c = rsq_nr(fma(t, t, 1.0));
-  *cs = rot ? c : 1.0;
-  *sn = rot ? c * t : 0.0;
-}
-
 // One workgroup per block pair: H = G(cols, cols), one inner sweep, J out.
 // A step: threads 0..B-1 compute the rotation of local pair tid once and
 // publish it; after a barrier thread t = (bi, bj) (bi = t / B, bj = t % B)
 // applies R_bi^T . R_bj to the 2 x 2 block of H at (pair bi rows, pair bj
 // columns) -- the blocks partition H -- and R_bi to rows 2 bj, 2 bj + 1 of J's
 // columns of pair bi.  H is double-buffered.
+// Local pairing of step t of the inner sweep.  Round 0 of a sweep runs the
+// full cyclic sweep of the 2B columns (S - 1 steps, every pair once, within
+// and across the two blocks); the other rounds only meet the B x B cross pairs
+// (B steps: local column k of block a with B + (k + t) mod B of block b) --
+// the pairs inside one block were already met in round 0.  Every pair of
+// columns still meets once per sweep, and the inner steps per sweep fall from
+// about 2n to about n.
+template <int B>
+__device__ __forceinline__ void local_pair(int k, int t, bool cross, int* p, int* q) {
+  constexpr int S = 2 * B;
+  if (cross) {
+    *p = k;
+    *q = B + (k + t) % B;
+  } else {
+    *p = player(k, t, S);
+    *q = player(S - 1 - k, t, S);
+  }
+}
+
 template <int B>
 __global__ void __launch_bounds__(256)
 symj_solve_kernel(int64_t n, const double* __restrict__ G, int64_t ldg, int nblocks, int N,
@@ -123,9 +97,12 @@ symj_solve_kernel(int64_t n, const double* __restrict__ G, int64_t ldg, int nblo
   const int bi = tid / B, bj = tid % B;
   double* Hc = Ha;
   double* Hn = Hb;
-  for (int t = 0; t < S - 1; ++t) {
+  const bool cross = round > 0;
+  const int steps = cross ? B : S - 1;
+  for (int t = 0; t < steps; ++t) {
     if (tid < B) {
-      const int p = player(tid, t, S), q = player(S - 1 - tid, t, S);
+      int p, q;
+      local_pair<B>(tid, t, cross, &p, &q);
       double c, sn;
       int sg;
       rotation(Hc[p * HP + p], Hc[q * HP + q], Hc[p * HP + q], flo, &c, &sn, &sg);
@@ -134,8 +111,9 @@ symj_solve_kernel(int64_t n, const double* __restrict__ G, int64_t ldg, int nblo
       any_sig |= sg;
     }
     __syncthreads();
-    const int pi = player(bi, t, S), qi = player(S - 1 - bi, t, S);
-    const int pj = player(bj, t, S), qj = player(S - 1 - bj, t, S);
+    int pi, qi, pj, qj;
+    local_pair<B>(bi, t, cross, &pi, &qi);
+    local_pair<B>(bj, t, cross, &pj, &qj);
     const double ci = rc[bi], si = rs[bi], cj = rc[bj], sj = rs[bj];
     const double h00 = Hc[pi * HP + pj], h01 = Hc[pi * HP + qj];
     const double h10 = Hc[qi * HP + pj], h11 = Hc[qi * HP + qj];
@@ -198,7 +176,7 @@ __device__ __forceinline__ void tile_mm(const double* __restrict__ A, const doub
 
 // Apply the round's rotations.  Blocks [0, ntg) take the upper pair tiles
 // (w1 <= w2) of G: T <- J_w1^T T J_w2, written to both triangles; the rest
-// take S-row strips of V: V(rows, cols(w)) <- V(rows, cols(w)) J_w.  Every
+// take 2S-row strips of V: V(rows, cols(w)) <- V(rows, cols(w)) J_w.  Every
 // workgroup reads and writes only its own tile, so the update is in place.
 template <int B>
 __global__ void __launch_bounds__(256)
@@ -271,30 +249,43 @@ symj_apply_kernel(int64_t n, double* __restrict__ G, int64_t ldg, double* __rest
     return;
   }
   if (V == nullptr) return;
+  // V strip: rows [r0, r0 + 2S) of the pair's columns, in two S-row halves
+  // (T and X), loaded together so one memory latency covers both
   bid -= ntg;
   const int w = bid % pairs;
-  const int64_t r0 = (int64_t)(bid / pairs) * S;
+  const int64_t r0 = (int64_t)(bid / pairs) * (2 * S);
   if (tid < S) cb[tid] = pair_col<B>(w, tid, round, N, nblocks, n);
   __syncthreads();
   const double* Jw = Jall + (size_t)w * S * S;
+  for (int e = tid; e < 2 * S * S; e += 256) {
+    const int r = e % (2 * S), c = e / (2 * S);
+    const int64_t gc = cb[c];
+    const double v = (gc >= 0 && r0 + r < n) ? V[(r0 + r) + gc * ldv] : 0.0;
+    if (r < S) T[r * HP + c] = v;
+    else X[(r - S) * HP + c] = v;
+  }
   for (int e = tid; e < S * S; e += 256) {
     const int r = e % S, c = e / S;
-    const int64_t gc = cb[c];
-    T[r * HP + c] = (gc >= 0 && r0 + r < n) ? V[(r0 + r) + gc * ldv] : 0.0;
     Jb[r * HP + c] = Jw[r + c * S];
   }
   __syncthreads();
+  double o2[R][R];
   tile_mm<S, false>(T, Jb, o);
+  tile_mm<S, false>(X, Jb, o2);
   __syncthreads();
 #pragma unroll
   for (int a = 0; a < R; ++a)
 #pragma unroll
-    for (int b = 0; b < R; ++b) T[(i0 + a) * HP + j0 + b] = o[a][b];
+    for (int b = 0; b < R; ++b) {
+      T[(i0 + a) * HP + j0 + b] = o[a][b];
+      X[(i0 + a) * HP + j0 + b] = o2[a][b];
+    }
   __syncthreads();
-  for (int e = tid; e < S * S; e += 256) {
-    const int r = e % S, c = e / S;
+  for (int e = tid; e < 2 * S * S; e += 256) {
+    const int r = e % (2 * S), c = e / (2 * S);
     const int64_t gc = cb[c];
-    if (gc >= 0 && r0 + r < n) V[(r0 + r) + gc * ldv] = T[r * HP + c];
+    if (gc >= 0 && r0 + r < n)
+      V[(r0 + r) + gc * ldv] = (r < S) ? T[r * HP + c] : X[(r - S) * HP + c];
   }
 }
 
@@ -321,7 +312,7 @@ int run_sweeps(Context* c, int64_t n, double* G, int64_t ldg, double* V, int64_t
   const int N = nblocks + (nblocks & 1);
   const int pairs = N / 2;
   const int ntg = pairs * (pairs + 1) / 2;
-  const int ntv = V ? pairs * (int)((n + S - 1) / S) : 0;
+  const int ntv = V ? pairs * (int)((n + 2 * S - 1) / (2 * S)) : 0;
   DBuf<double> J, fro2;
   DBuf<unsigned long long> off;
   RSVD_TRY(J.alloc((size_t)pairs * S * S));
