@@ -207,6 +207,10 @@ double rsvd_timer_stop_ms(void);
 /* Per-kernel event timing (off by default). */
 void rsvd_kstats_enable(int on);
 void rsvd_kstats_reset(void);
+/* Restrict the per-kernel timing to the comma-separated families in `names`
+ * (NULL or "": all), so a timed region carries events only around the
+ * launches it reports. */
+void rsvd_kstats_select(const char *names);
 /* Total milliseconds and launch count of kernel family `name` (e.g.
  * "gemm_f64", "jacobi_round"); returns nonzero if never launched. */
 int rsvd_kstats_get(const char *name, double *total_ms, long long *count);

@@ -225,6 +225,238 @@ apass_kernel(int64_t M, int64_t N, int64_t K, float alpha, const float* __restri
   }
 }
 
+// ---------------------------------------------------------------- apass2
+// Eight waves (two per SIMD) and a 192 x 288 tile per workgroup: waves
+// (wm, wn) = (wave % 4, wave / 4) own 48 rows x 144 columns (3 x 9 tiles of
+// 16 x 16, 108 accumulators; two waves per SIMD leave 256 registers each).
+// Per k block of 32 the workgroup needs 288 columns of the skinny operand (hi
+// and lo planes, 36 KiB) and 192 rows of A (24 KiB): 2x fewer bytes per
+// output from L2 than the 64 x 576 tile.
+//   * The skinny operand goes HBM/L2 -> LDS with direct loads
+//     (global_load_lds_dwordx4, no registers) into a ring of three stages,
+//     issued two k blocks ahead; each wave reads its fragments from LDS one
+//     column tile ahead of the MFMAs that use them.
+//   * A goes straight into registers in fragment order (lane l: row l % 16,
+//     k 8 (l / 16) .. + 7), one k block ahead, and is split into bf16 hi / lo
+//     at the end of the block.  The two waves that share rows (wn = 0, 1)
+//     load the same lines at the same time, so the second read hits L1.
+//   * One barrier per k block: after it, the next stage is complete for every
+//     wave and the stage read in this block may be overwritten.
+// The k range can be split over gridDim.z-like slices (ks): slice partials go
+// to a workspace and a second kernel adds them.  Block ids are mapped so that
+// the column blocks of one row block and k slice run on the same XCD next to
+// each other (they read the same A lines).
+typedef __attribute__((address_space(3))) void* lds_ptr_t;
+typedef __attribute__((address_space(1))) void* gbl_ptr_t;
+
+constexpr int kW2 = 8;
+constexpr int kT2 = 64 * kW2;
+constexpr int kMT2 = 3;               // row tiles per wave
+constexpr int kBM2 = 4 * 16 * kMT2;   // rows per workgroup
+constexpr int kNTW2 = 9;
+constexpr int kBNT2 = 2 * kNTW2;
+constexpr int kST2 = 3;
+
+template <bool TA, int SPLIT, bool VEC>
+__global__ void __launch_bounds__(kT2)
+apass2_kernel(int64_t M, int64_t N, int64_t KBtot, int64_t KBper, float alpha,
+              const float* __restrict__ A, int64_t lda, const bf16x8_t* __restrict__ St,
+              int64_t NTP, float beta, float* __restrict__ C, int64_t ldc,
+              float* __restrict__ Wk, int gx, int gy, int ks) {
+  constexpr int NP = (SPLIT == 3) ? 2 : 1;
+  constexpr int NCH = NP * kBNT2;             // 1 KiB chunks per stage
+  constexpr int NLD = (NCH + kW2 - 1) / kW2;  // chunk loads per wave per stage
+  __shared__ bf16x8_t Ss[kST2][NP][kBNT2][64];
+  __shared__ bf16x8_t Sdummy[64];
+
+  const int tid = threadIdx.x, lane = tid & 63;
+  // wave-uniform values in scalar registers
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int wm = wave & 3, wn = wave >> 2;
+  const int id = blockIdx.x;
+  int bx, by, kz;
+  if ((gx & 7) == 0) {
+    const int xcd = id & 7, slot = id >> 3;
+    by = slot % gy;
+    const int t = slot / gy;
+    kz = t % ks;
+    bx = (t / ks) * 8 + xcd;
+  } else {
+    bx = id % gx;
+    const int t = id / gx;
+    by = t % gy;
+    kz = t / gy;
+  }
+  bx = __builtin_amdgcn_readfirstlane(bx);
+  by = __builtin_amdgcn_readfirstlane(by);
+  kz = __builtin_amdgcn_readfirstlane(kz);
+  if ((int64_t)bx * kBM2 >= M) return;  // padding of the row-block count to 8
+  const int64_t kb0 = (int64_t)kz * KBper;
+  const int64_t kbe = (kb0 + KBper < KBtot) ? kb0 + KBper : KBtot;
+  const int64_t nk = kbe - kb0;
+  const int64_t klast = kbe - 1;
+  const int64_t mw = (int64_t)bx * kBM2 + wm * 16 * kMT2;  // this wave's first row
+  const int64_t ntw = (int64_t)by * kBNT2;          // workgroup's first n tile
+
+  // ---- S stage loads: chunk c = (plane c / 18, tile c % 18), wave-strided;
+  // the padding slots reload chunk 0 into Sdummy so every wave issues NLD
+  auto loadS = [&](int64_t kb, bf16x8_t (*dst)[kBNT2][64]) {
+    kb = kb < klast ? kb : klast;
+#pragma unroll
+    for (int q = 0; q < NLD; ++q) {
+      const int c = wave + kW2 * q;
+      const bool real = c < NCH;
+      const int pl = real ? c / kBNT2 : 0, t = real ? c % kBNT2 : 0;
+      const bf16x8_t* g = St + ((pl * KBtot + kb) * NTP + ntw + t) * 64 + lane;
+      lds_ptr_t d = real ? (lds_ptr_t)&dst[pl][t][0] : (lds_ptr_t)&Sdummy[0];
+      __builtin_amdgcn_global_load_lds((gbl_ptr_t)g, d, 16, 0, 0);
+    }
+  };
+
+  // ---- A: lane -> row mw + 16 i + lane % 16, k 8 (lane / 16) .. + 7
+  const float* Ar[kMT2];
+#pragma unroll
+  for (int i = 0; i < kMT2; ++i) {
+    int64_t r = mw + 16 * i + (lane & 15);
+    r = r < M ? r : M - 1;
+    Ar[i] = TA ? A + r * lda + 8 * (lane >> 4) : A + r + (int64_t)(8 * (lane >> 4)) * lda;
+  }
+  float raw[kMT2][8];
+  auto loadA = [&](int64_t kb) {
+    kb = kb < klast ? kb : klast;
+#pragma unroll
+    for (int i = 0; i < kMT2; ++i) {
+      if (TA) {
+        const float* p = Ar[i] + kb * 32;
+        if (VEC) {
+          const f4s_t x = *reinterpret_cast<const f4s_t*>(p);
+          const f4s_t y = *reinterpret_cast<const f4s_t*>(p + 4);
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            raw[i][e] = x[e];
+            raw[i][4 + e] = y[e];
+          }
+        } else {
+#pragma unroll
+          for (int e = 0; e < 8; ++e) raw[i][e] = p[e];
+        }
+      } else {
+        const float* p = Ar[i] + kb * 32 * lda;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) raw[i][e] = p[e * lda];
+      }
+    }
+  };
+  bf16x8_t ah[kMT2], al[kMT2];
+  auto split = [&]() {
+#pragma unroll
+    for (int i = 0; i < kMT2; ++i)
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const __bf16 hb = (__bf16)raw[i][e];
+        ah[i][e] = hb;
+        if (SPLIT == 3) al[i][e] = (__bf16)(raw[i][e] - (float)hb);
+      }
+  };
+
+  f4v_t acc[kNTW2][kMT2];
+#pragma unroll
+  for (int j = 0; j < kNTW2; ++j)
+#pragma unroll
+    for (int i = 0; i < kMT2; ++i) acc[j][i] = f4v_t{0.f, 0.f, 0.f, 0.f};
+
+  // one k block on stage `cur`: A(kb + 1) and S(kb + 2) loads first, then the
+  // MFMAs with the S fragments read one tile ahead, then the A split for kb + 1
+  auto step = [&](int64_t kb, bf16x8_t (*cur)[kBNT2][64], bf16x8_t (*nx2)[kBNT2][64]) {
+    loadA(kb + 1);
+    loadS(kb + 2, nx2);
+    __builtin_amdgcn_sched_barrier(0);
+    bf16x8_t sh[2], sl[2];
+    sh[0] = cur[0][wn * kNTW2][lane];
+    if (SPLIT == 3) sl[0] = cur[NP - 1][wn * kNTW2][lane];
+#pragma unroll
+    for (int j = 0; j < kNTW2; ++j) {
+      const int b = j & 1;
+      if (j + 1 < kNTW2) {
+        sh[b ^ 1] = cur[0][wn * kNTW2 + j + 1][lane];
+        if (SPLIT == 3) sl[b ^ 1] = cur[NP - 1][wn * kNTW2 + j + 1][lane];
+      }
+#pragma unroll
+      for (int i = 0; i < kMT2; ++i)
+        acc[j][i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(sh[b], ah[i], acc[j][i], 0, 0, 0);
+      if (SPLIT == 3) {
+#pragma unroll
+        for (int i = 0; i < kMT2; ++i)
+          acc[j][i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(sh[b], al[i], acc[j][i], 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < kMT2; ++i)
+          acc[j][i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(sl[b], ah[i], acc[j][i], 0, 0, 0);
+      }
+      // keep the fragment reads one tile ahead (not all hoisted: registers)
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    split();
+    __syncthreads();
+  };
+
+  loadS(kb0, Ss[0]);
+  loadS(kb0 + 1, Ss[1]);
+  loadA(kb0);
+  split();
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  int st = 0;
+  for (int64_t s = 0; s < nk; ++s) {
+    const int st2 = (st >= 1) ? st - 1 : st + 2;  // (s + 2) % 3
+    step(kb0 + s, Ss[st], Ss[st2]);
+    st = (st == 2) ? 0 : st + 1;
+  }
+  // drain the trailing direct loads before the workgroup exits
+  __builtin_amdgcn_s_waitcnt(0);
+
+  // D[n][m] of tile (j, i): lane holds n = 4 (lane / 16) + rr, m = lane % 16
+  float* out = (ks > 1) ? Wk + (int64_t)kz * M * (NTP * 16) : C;
+  const int64_t ldo = (ks > 1) ? M : ldc;
+#pragma unroll
+  for (int j = 0; j < kNTW2; ++j) {
+#pragma unroll
+    for (int i = 0; i < kMT2; ++i) {
+      const int64_t m = mw + i * 16 + (lane & 15);
+#pragma unroll
+      for (int rr = 0; rr < 4; ++rr) {
+        const int64_t n = (ntw + wn * kNTW2 + j) * 16 + 4 * (lane >> 4) + rr;
+        if (m < M && n < N) {
+          float* cp = out + m + n * ldo;
+          if (ks > 1) {
+            *cp = acc[j][i][rr];
+          } else {
+            float v = alpha * acc[j][i][rr];
+            if (beta != 0.0f) v += beta * (*cp);
+            *cp = v;
+          }
+        }
+      }
+    }
+  }
+}
+
+// C = alpha * sum_z Wk[z] + beta * C over the M x N output.
+__global__ void apass2_reduce_kernel(int64_t M, int64_t N, int ks, const float* __restrict__ Wk,
+                                     int64_t ldw, float alpha, float beta, float* __restrict__ C,
+                                     int64_t ldc) {
+  const int64_t total = M * N;
+  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t m = idx % M, n = idx / M;
+    float v = 0.0f;
+    for (int z = 0; z < ks; ++z) v += Wk[(int64_t)z * M * ldw + m + n * M];
+    v *= alpha;
+    float* cp = C + m + n * ldc;
+    if (beta != 0.0f) v += beta * (*cp);
+    *cp = v;
+  }
+}
+
 }  // namespace
 
 bool apass_suitable(Context* c, int64_t M, int64_t N) {
@@ -245,10 +477,21 @@ int apass_bf16(Context* c, const char* tag, bool ta, int64_t M, int64_t N, int64
     return gemm_bf16_general(c, tag, ta ? 'T' : 'N', 'N', M, N, K, alpha, A, lda, S, lds, beta, C,
                              ldc, split);
   const int64_t KB = K32 / 32;
-  const int64_t gx = (M + kBM - 1) / kBM;
-  const int64_t gy = ((N + 15) / 16 + kBNT - 1) / kBNT;
+  const int64_t cus = c->num_cus > 0 ? c->num_cus : 256;
+  // the 256 x 288 kernel, with the k range split while the grid is short of
+  // the CU count (keeping at least 8 k blocks per slice)
+  const int64_t gx2 = (M + kBM2 - 1) / kBM2;
+  const int64_t gy2 = ((N + 15) / 16 + kBNT2 - 1) / kBNT2;
+  // (op(A) = A only: with op(A) = A^T the 192 rows of a workgroup are 192
+  // separate 128-byte streams per k block, and the 64-row tile keeps the
+  // number of open streams -- and so DRAM pages and TLB entries -- 3x lower)
+  int ks = 1;
+  while (gx2 * gy2 * ks < cus && KB / (2 * ks) >= 8) ks *= 2;
+  const bool use2 = !ta && gx2 * gy2 * ks >= cus && gx2 * gy2 * ks <= 0x7fffffff;
+  const int64_t gx = use2 ? gx2 : (M + kBM - 1) / kBM;
+  const int64_t gy = use2 ? gy2 : ((N + 15) / 16 + kBNT - 1) / kBNT;
   if (gy > 65535 || gx > 0x7fffffff) return set_error("apass: grid too large");
-  const int64_t NTP = gy * kBNT;
+  const int64_t NTP = gy * (use2 ? kBNT2 : kBNT);
   const int planes = (split == 3) ? 2 : 1;
   DBuf<bf16x8_t> St;
   RSVD_TRY(St.alloc((size_t)(planes * KB * NTP * 64)));
@@ -261,6 +504,43 @@ int apass_bf16(Context* c, const char* tag, bool ta, int64_t M, int64_t N, int64
     RSVD_LAUNCH_CHECK();
   }
   const bool vec = ta && (reinterpret_cast<uintptr_t>(A) % 16 == 0) && (lda % 4 == 0);
+  if (use2) {
+    DBuf<float> Wk;
+    if (ks > 1) RSVD_TRY(Wk.alloc((size_t)(ks * M * NTP * 16)));
+    const int64_t KBper = (KB + ks - 1) / ks;
+    // row blocks padded to a multiple of 8 so that the XCD-grouped block
+    // mapping applies (the padding blocks exit at once)
+    const int64_t gxp = (gx + 7) / 8 * 8;
+    const unsigned blocks = (unsigned)(gxp * gy * ks);
+    {
+      KTimer kt(c, tag);
+#define RSVD_APASS2(TA_, S_, V_)                                                               \
+  hipLaunchKernelGGL((apass2_kernel<TA_, S_, V_>), dim3(blocks), dim3(kT2), 0, c->stream, M, N, KB, \
+                     KBper, alpha, A, lda, (const bf16x8_t*)St.p, NTP, beta, C, ldc, Wk.p,        \
+                     (int)gxp, (int)gy, ks)
+      if (ta) {
+        if (split == 3) {
+          if (vec) RSVD_APASS2(true, 3, true);
+          else RSVD_APASS2(true, 3, false);
+        } else {
+          if (vec) RSVD_APASS2(true, 1, true);
+          else RSVD_APASS2(true, 1, false);
+        }
+      } else {
+        if (split == 3) RSVD_APASS2(false, 3, false);
+        else RSVD_APASS2(false, 1, false);
+      }
+#undef RSVD_APASS2
+      RSVD_LAUNCH_CHECK();
+      if (ks > 1) {
+        int64_t rb = (M * N + 255) / 256;
+        if (rb > 8192) rb = 8192;
+        hipLaunchKernelGGL(apass2_reduce_kernel, dim3((unsigned)rb), dim3(256), 0, c->stream, M, N,
+                           ks, (const float*)Wk.p, NTP * 16, alpha, beta, C, ldc);
+        RSVD_LAUNCH_CHECK();
+      }
+    }
+  } else {
   dim3 grid((unsigned)gx, (unsigned)gy);
   {
     KTimer kt(c, tag);
@@ -281,6 +561,7 @@ int apass_bf16(Context* c, const char* tag, bool ta, int64_t M, int64_t N, int64
     }
 #undef RSVD_APASS
     RSVD_LAUNCH_CHECK();
+  }
   }
   if (K32 < K) {
     // the last K % 32 columns of op(A): C += alpha op(A)(:, K32:) S(K32:, :)

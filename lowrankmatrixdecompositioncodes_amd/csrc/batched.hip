@@ -238,40 +238,80 @@ __device__ void lds_gram(const float* __restrict__ X, int64_t R, int l, float (*
   __syncthreads();
 }
 
-// In-place upper Cholesky of G (l x l, both triangles) -> R in the upper
-// triangle (G = R^T R), then Ri = R^-1 (upper).  shift is added to the
-// diagonal first.  Returns through *ok (shared) whether every pivot was
-// positive.
+// 1 / d to about one ulp: the hardware reciprocal estimate and two Newton
+// steps (shorter than a correctly rounded division on the serial path).
+__device__ __forceinline__ double rcp_nr2(double d) {
+  double r = __builtin_amdgcn_rcp(d);
+  r = fma(r, fma(-d, r, 1.0), r);
+  r = fma(r, fma(-d, r, 1.0), r);
+  return r;
+}
+
+// In-place upper Cholesky of G (l x l, upper triangle used) -> R in the upper
+// triangle (G = R^T R, zero below), then Ri = R^-1 (upper).  shift is added
+// to the diagonal first.  Returns through *ok (shared) whether every pivot
+// was positive (a non-positive one is replaced by 1).
+// Factorisation: unnormalised right-looking elimination, one barrier per
+// step: thread (row a = tid % 64, columns 12 (tid / 64) ..) keeps its 12
+// entries in registers, subtracts U_j,a U_j,c / U_j,j at step j and publishes
+// row j + 1 when it owns it.  Inverse: one thread per column, fully unrolled
+// back substitution in registers.
 __device__ void lds_chol_inv(double (*G)[kLP], double (*Ri)[kLP], int l, double shift, int* ok) {
-  const int tid = threadIdx.x;
+  __shared__ double piv[kL], inv[kL];
+  const int tid = threadIdx.x, a = tid & 63, c0 = (tid >> 6) * 12;
   if (tid == 0) *ok = 1;
   if (tid < l) G[tid][tid] += shift;
   __syncthreads();
+  double own[12];
+#pragma unroll
+  for (int q = 0; q < 12; ++q) own[q] = (a < l && c0 + q < l) ? G[a][c0 + q] : 0.0;
   for (int j = 0; j < l; ++j) {
+    double d = G[j][j];
+    const bool good = d > 0.0;
+    if (!good) d = 1.0;
     if (tid == 0) {
-      const double dj = G[j][j];
-      if (!(dj > 0.0)) *ok = 0;
-      G[j][j] = dj > 0.0 ? sqrt(dj) : 1.0;
+      if (!good) *ok = 0;
+      piv[j] = d;
     }
-    __syncthreads();
-    const double rjj = G[j][j];
-    for (int c = j + 1 + tid; c < l; c += 256) G[j][c] /= rjj;
-    __syncthreads();
-    // trailing update G[a][b] -= R[j][a] R[j][b], j < a <= b
-    for (int a = j + 1 + (tid >> 4); a < l; a += 16)
-      for (int b = a + (tid & 15); b < l; b += 16) G[a][b] -= G[j][a] * G[j][b];
+    if (a > j && a < l) {
+      const double u = G[j][a] * rcp_nr2(d);
+#pragma unroll
+      for (int q = 0; q < 12; ++q)
+        if (c0 + q >= a && c0 + q < l) own[q] -= u * G[j][c0 + q];
+      if (a == j + 1) {
+#pragma unroll
+        for (int q = 0; q < 12; ++q)
+          if (c0 + q >= a && c0 + q < l) G[a][c0 + q] = own[q];
+      }
+    }
     __syncthreads();
   }
-  // inverse of the upper R, one column per thread (back substitution)
   if (tid < l) {
+    const double sj = sqrt(piv[tid]);
+    piv[tid] = sj;
+    inv[tid] = 1.0 / sj;
+  }
+  __syncthreads();
+  for (int e = tid; e < kL * kL; e += 256) {
+    const int i = e % kL, c = e / kL;
+    double v = 0.0;
+    if (i < l && c < l) v = (c > i) ? G[i][c] * inv[i] : (c == i ? piv[i] : 0.0);
+    G[i][c] = v;
+  }
+  __syncthreads();
+  if (tid < kL) {
     const int c = tid;
-    for (int r = 0; r < l; ++r) Ri[r][c] = 0.0;
-    Ri[c][c] = 1.0 / G[c][c];
-    for (int r = c - 1; r >= 0; --r) {
+    double x[kL];
+#pragma unroll
+    for (int r = kL - 1; r >= 0; --r) {
       double s = 0.0;
-      for (int u = r + 1; u <= c; ++u) s += G[r][u] * Ri[u][c];
-      Ri[r][c] = -s / G[r][r];
+#pragma unroll
+      for (int u = kL - 1; u > r; --u) s = fma(G[r][u], x[u], s);
+      const double ir = (r < l) ? inv[r] : 0.0;
+      x[r] = (r > c) ? 0.0 : ((r == c) ? ir : -s * ir);
     }
+#pragma unroll
+    for (int r = 0; r < kL; ++r) Ri[r][c] = (c < l) ? x[r] : 0.0;
   }
   __syncthreads();
 }

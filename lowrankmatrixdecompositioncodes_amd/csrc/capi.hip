@@ -1082,6 +1082,23 @@ void rsvd_kstats_enable(int on) {
   kstats_resolve(api.c);
   api.c->kstats = on != 0;
 }
+void rsvd_kstats_select(const char* names) {
+  Api api;
+  if (api.rc) return;
+  kstats_resolve(api.c);
+  api.c->kstats_only.clear();
+  if (!names) return;
+  std::string cur;
+  for (const char* q = names;; ++q) {
+    if (*q == ',' || *q == 0) {
+      if (!cur.empty()) api.c->kstats_only.push_back(cur);
+      cur.clear();
+      if (*q == 0) break;
+    } else {
+      cur.push_back(*q);
+    }
+  }
+}
 void rsvd_kstats_reset(void) {
   Api api;
   if (api.rc) return;

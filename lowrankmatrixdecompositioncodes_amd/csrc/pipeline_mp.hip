@@ -48,7 +48,7 @@ static int orth_f32(Context* c, int64_t m, int64_t n, float* Y, int64_t ldy, int
                          G32.p, n, 3));
     RSVD_TRY(f32_to_f64(c, n, n, G32.p, n, G64.p, n));
     int ok = 1;
-    RSVD_TRY(cholesky_upper_f64(c, n, G64.p, R64.p, &ok));
+    RSVD_TRY(chol_inv_upper_f64(c, n, G64.p, R64.p, Rinv64.p, &ok));
     if (!ok) {
       // FP64 Householder on the current basis (same range as Y)
       if (fallback) *fallback += 1;
@@ -59,7 +59,6 @@ static int orth_f32(Context* c, int64_t m, int64_t n, float* Y, int64_t ldy, int
       RSVD_TRY(f64_to_f32(c, m, n, Y64.p, m, Y, ldy));
       return 0;
     }
-    RSVD_TRY(trtri_upper_f64(c, n, R64.p, Rinv64.p));
     RSVD_TRY(f64_to_f32(c, n, n, Rinv64.p, n, Rinv32.p, n));
     float* dst = (src == Y) ? W.p : Y;
     const int64_t ldd = (src == Y) ? m : ldy;

@@ -53,6 +53,7 @@ struct Context {
   size_t pooled_bytes = 0;
   // per-kernel timing (enabled by rsvd_kstats_enable)
   bool kstats = false;
+  std::vector<std::string> kstats_only;  // empty: every timer records
   std::vector<KernelStat> stats;
   // pending event pairs, resolved on rsvd_kstats_get / sync
   struct Pending { int idx; hipEvent_t a, b; };
@@ -204,12 +205,10 @@ int orth_f64(Context* c, int64_t m, int64_t n, double* Y, int64_t ldy, double* R
              int* used_fallback);
 // Householder QR (geqr2 + org2r) in place, R optional.
 int householder_qr_f64(Context* c, int64_t m, int64_t n, double* Y, int64_t ldy, double* R);
-// Cholesky of SPD G (n x n, ld n) -> upper R (n x n, ld n, lower part zero);
-// G's trailing part is overwritten.  *ok = 0 when a pivot is not safely
-// positive.
-int cholesky_upper_f64(Context* c, int64_t n, double* G, double* R, int* ok);
-// Inverse of an upper-triangular R (n x n, ld n) into Rinv (n x n, ld n).
-int trtri_upper_f64(Context* c, int64_t n, const double* R, double* Rinv);
+// R = chol(G) (upper) and Rinv = R^-1 in one blocked pass (n x n, ld n, zero
+// below the diagonal).  G's trailing part is overwritten.  *ok = 0 when a
+// pivot is not safely positive.
+int chol_inv_upper_f64(Context* c, int64_t n, double* G, double* R, double* Rinv, int* ok);
 
 // One-sided Jacobi.  W (L x n, ld L) is rotated in place until its columns
 // are mutually orthogonal to rel_tol (0: the rounding floor eps*sqrt(L));

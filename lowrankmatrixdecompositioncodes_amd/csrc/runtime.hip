@@ -120,6 +120,11 @@ void pool_trim() {
 // ---------------------------------------------------------------- kstats
 KTimer::KTimer(Context* ctx, const char* name) : c(ctx) {
   if (!c->kstats) return;
+  if (!c->kstats_only.empty()) {
+    bool hit = false;
+    for (const auto& s : c->kstats_only) hit = hit || s == name;
+    if (!hit) return;
+  }
   int i = 0;
   for (; i < (int)c->stats.size(); ++i)
     if (c->stats[i].name == name) break;

@@ -320,18 +320,15 @@ int run_sweeps(Context* c, int64_t n, double* G, int64_t ldg, double* V, int64_t
   RSVD_TRY(off.alloc(1));
   RSVD_TRY(sumsq_f64(c, n, n, G, ldg, fro2.p));
   int sweep = 0;
+  KTimer kt(c, "symeig");
   if (n > 1) {
     for (; sweep < max_sweeps; ++sweep) {
       RSVD_HIP(hipMemsetAsync(off.p, 0, sizeof(unsigned long long), c->stream));
       for (int r = 0; r < N - 1; ++r) {
-        {
-          KTimer kt(c, "symeig_solve");
-          hipLaunchKernelGGL(symj_solve_kernel<B>, dim3(pairs), dim3(256), 0, c->stream, n,
-                             (const double*)G, ldg, nblocks, N, r, (const double*)fro2.p, J.p,
-                             off.p);
-          RSVD_LAUNCH_CHECK();
-        }
-        KTimer kt(c, "symeig_apply");
+        hipLaunchKernelGGL(symj_solve_kernel<B>, dim3(pairs), dim3(256), 0, c->stream, n,
+                           (const double*)G, ldg, nblocks, N, r, (const double*)fro2.p, J.p,
+                           off.p);
+        RSVD_LAUNCH_CHECK();
         hipLaunchKernelGGL(symj_apply_kernel<B>, dim3(ntg + ntv), dim3(256), 0, c->stream, n, G,
                            ldg, V, ldv, nblocks, N, r, (const double*)J.p, ntg);
         RSVD_LAUNCH_CHECK();

@@ -260,13 +260,18 @@ def test_gemm_bf16x3_is_fp32_accurate(gpu_lib, ta, tb):
 
 @pytest.mark.parametrize("ta", ["N", "T"])
 @pytest.mark.parametrize("mode", ["bf16", "bf16x3"])
-@pytest.mark.parametrize("M,N,K", [(16400, 544, 300), (16384, 40, 1000), (16448, 600, 77)])
+@pytest.mark.parametrize("M,N,K", [(16400, 544, 300), (16384, 40, 1000), (16448, 600, 77),
+                                   (50000, 300, 1000), (49999, 40, 1001), (16384, 544, 4096),
+                                   (8192, 64, 16384)])
 def test_gemm_bf16_streaming_path(gpu_lib, ta, mode, M, N, K):
-    """Shapes with a skinny right operand and >= 256 row blocks take the
-    A-streaming kernel (A read once, all N columns per workgroup, more than one
-    column block when N > 576, unaligned rows when K % 4 != 0).  bf16: equal to
-    the exact product of the bf16-rounded operands up to fp32 accumulation;
-    bf16x3: fp32-accurate."""
+    """Shapes with a skinny right operand take the A-streaming kernels: the
+    64 x 576 tile (A read once, more than one column block when N > 576,
+    unaligned rows when K % 4 != 0; every op(A) = A^T case), and for op(A) = A
+    with at least a CU's worth of 192-row blocks the 192 x 288 tile with its
+    LDS ring (two column blocks for N = 300, a K % 32 remainder for K = 1001,
+    the k range split over eight slices for 8192 x 64 x 16384).
+    bf16: equal to the exact product of the bf16-rounded operands up to fp32
+    accumulation; bf16x3: fp32-accurate."""
     rng = np.random.default_rng(M + N + K)
     A = rng.standard_normal((K, M) if ta == "T" else (M, K)).astype(np.float32)
     B = rng.standard_normal((K, N)).astype(np.float32)
