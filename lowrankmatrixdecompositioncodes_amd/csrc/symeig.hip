@@ -70,34 +70,23 @@ __device__ __forceinline__ void local_pair(int k, int t, bool cross, int* p, int
   }
 }
 
+// The inner sweep of one block pair on its S x S tile (S = 2B), in LDS: H in
+// Ha (double-buffered with Hb), J accumulated in Jl (row-major, padded HP).
+// A step: threads 0..B-1 compute the rotation of local pair tid once and
+// publish it; after a barrier thread t = (bi, bj) (bi = t / B, bj = t % B)
+// applies R_bi^T . R_bj to the 2 x 2 block of H at (pair bi rows, pair bj
+// columns) -- the blocks partition H -- and R_bi to rows 2 bj, 2 bj + 1 of J's
+// columns of pair bi.  Returns the buffer holding the final H.
 template <int B>
-__global__ void __launch_bounds__(256)
-symj_solve_kernel(int64_t n, const double* __restrict__ G, int64_t ldg, int nblocks, int N,
-                  int round, const double* __restrict__ fro2, double* __restrict__ Jout,
-                  unsigned long long* __restrict__ offmax) {
+__device__ double* jac_inner(double* Ha, double* Hb, double* Jl, double* rc, double* rs,
+                             bool cross, double flo, int* any_sig) {
   constexpr int S = 2 * B, HP = S + 1;
   static_assert(B * B == 256, "one 2x2 block per thread");
-  __shared__ double Ha[S * HP], Hb[S * HP], Jl[S * HP];
-  __shared__ double rc[B], rs[B];
-  __shared__ int64_t cols[S];
-  const int w = blockIdx.x;
   const int tid = threadIdx.x;
-  if (tid < S) cols[tid] = pair_col<B>(w, tid, round, N, nblocks, n);
-  __syncthreads();
-  for (int e = tid; e < S * S; e += 256) {
-    const int i = e % S, j = e / S;
-    const int64_t ci = cols[i], cj = cols[j];
-    Ha[i * HP + j] = (ci >= 0 && cj >= 0) ? G[ci + cj * ldg] : 0.0;
-    Jl[i * HP + j] = (i == j) ? 1.0 : 0.0;
-  }
-  const double flo = 4.0 * 2.220446049250313e-16 * sqrt(*fro2);
-  __syncthreads();
-
-  int any_sig = 0;
   const int bi = tid / B, bj = tid % B;
   double* Hc = Ha;
   double* Hn = Hb;
-  const bool cross = round > 0;
+  int sig = 0;
   const int steps = cross ? B : S - 1;
   for (int t = 0; t < steps; ++t) {
     if (tid < B) {
@@ -108,7 +97,7 @@ symj_solve_kernel(int64_t n, const double* __restrict__ G, int64_t ldg, int nblo
       rotation(Hc[p * HP + p], Hc[q * HP + q], Hc[p * HP + q], flo, &c, &sn, &sg);
       rc[tid] = c;
       rs[tid] = sn;
-      any_sig |= sg;
+      sig |= sg;
     }
     __syncthreads();
     int pi, qi, pj, qj;
@@ -139,13 +128,53 @@ symj_solve_kernel(int64_t n, const double* __restrict__ G, int64_t ldg, int nblo
     Hc = Hn;
     Hn = tmp;
   }
-  // J out, column-major S x S per pair
+  *any_sig = sig;
+  return Hc;
+}
+
+// J and the final tile H of pair w out (column-major S x S each); the flag
+// records whether any rotation was above kStopTol.
+template <int B>
+__device__ void jac_out(const double* Hf, const double* Jl, int w, double* __restrict__ Jout,
+                        double* __restrict__ Hout, int any_sig,
+                        unsigned long long* __restrict__ flag) {
+  constexpr int S = 2 * B, HP = S + 1;
+  const int tid = threadIdx.x;
   double* __restrict__ Jw = Jout + (size_t)w * S * S;
+  double* __restrict__ Hw = Hout + (size_t)w * S * S;
   for (int e = tid; e < S * S; e += 256) {
     const int i = e % S, j = e / S;
     Jw[i + j * S] = Jl[i * HP + j];
+    Hw[i + j * S] = Hf[i * HP + j];
   }
-  if (__any(any_sig) && (tid & 63) == 0) atomicOr(offmax, 1ULL);
+  if (__any(any_sig) && (tid & 63) == 0) atomicOr(flag, 1ULL);
+}
+
+// First round of the iteration: the pair tiles are read from G.
+template <int B>
+__global__ void __launch_bounds__(256)
+symj_solve_kernel(int64_t n, const double* __restrict__ G, int64_t ldg, int nblocks, int N,
+                  int round, const double* __restrict__ fro2, double* __restrict__ Jout,
+                  double* __restrict__ Hout, unsigned long long* __restrict__ flag) {
+  constexpr int S = 2 * B, HP = S + 1;
+  __shared__ double Ha[S * HP], Hb[S * HP], Jl[S * HP];
+  __shared__ double rc[B], rs[B];
+  __shared__ int64_t cols[S];
+  const int w = blockIdx.x;
+  const int tid = threadIdx.x;
+  if (tid < S) cols[tid] = pair_col<B>(w, tid, round, N, nblocks, n);
+  __syncthreads();
+  for (int e = tid; e < S * S; e += 256) {
+    const int i = e % S, j = e / S;
+    const int64_t ci = cols[i], cj = cols[j];
+    Ha[i * HP + j] = (ci >= 0 && cj >= 0) ? G[ci + cj * ldg] : 0.0;
+    Jl[i * HP + j] = (i == j) ? 1.0 : 0.0;
+  }
+  const double flo = 4.0 * 2.220446049250313e-16 * sqrt(*fro2);
+  __syncthreads();
+  int sig = 0;
+  const double* Hf = jac_inner<B>(Ha, Hb, Jl, rc, rs, round > 0, flo, &sig);
+  jac_out<B>(Hf, Jl, w, Jout, Hout, sig, flag);
 }
 
 // out (S x S) = op(A) * Bm with A, Bm in LDS (row-major, padded HP); thread
@@ -174,22 +203,21 @@ __device__ __forceinline__ void tile_mm(const double* __restrict__ A, const doub
   }
 }
 
-// Apply the round's rotations.  Blocks [0, ntg) take the upper pair tiles
-// (w1 <= w2) of G: T <- J_w1^T T J_w2, written to both triangles; the rest
-// take 2S-row strips of V: V(rows, cols(w)) <- V(rows, cols(w)) J_w.  Every
-// workgroup reads and writes only its own tile, so the update is in place.
+// Apply round r's rotations (the apply role of symj_step_kernel): blocks
+// [0, ntg) take the upper pair tiles (w1 <= w2) of Gin and write J_w1^T T J_w2
+// to both triangles of Gout; the rest take 2S-row strips of V (in place):
+// V(rows, cols(w)) <- V(rows, cols(w)) J_w.  Every element of Gout and V is
+// written by exactly one workgroup.
 template <int B>
-__global__ void __launch_bounds__(256)
-symj_apply_kernel(int64_t n, double* __restrict__ G, int64_t ldg, double* __restrict__ V,
-                  int64_t ldv, int nblocks, int N, int round, const double* __restrict__ Jall,
-                  int ntg) {
+__device__ void jac_apply(int bid, int64_t n, const double* __restrict__ Gin,
+                          double* __restrict__ Gout, int64_t ldg, double* __restrict__ V,
+                          int64_t ldv, int nblocks, int N, int round,
+                          const double* __restrict__ Jall, int ntg, double* T, double* Ja,
+                          double* Jb, double* X, int64_t* ra, int64_t* cb) {
   constexpr int S = 2 * B, HP = S + 1, R = S / 16;
-  __shared__ double T[S * HP], Ja[S * HP], Jb[S * HP], X[S * HP];
-  __shared__ int64_t ra[S], cb[S];
   const int pairs = N / 2;
   const int tid = threadIdx.x;
   const int i0 = (tid >> 4) * R, j0 = (tid & 15) * R;
-  int bid = blockIdx.x;
   double o[R][R];
   if (bid < ntg) {
     int w1 = 0;
@@ -208,7 +236,7 @@ symj_apply_kernel(int64_t n, double* __restrict__ G, int64_t ldg, double* __rest
     for (int e = tid; e < S * S; e += 256) {
       const int r = e % S, c = e / S;
       const int64_t gr = ra[r], gc = cb[c];
-      T[r * HP + c] = (gr >= 0 && gc >= 0) ? G[gr + gc * ldg] : 0.0;
+      T[r * HP + c] = (gr >= 0 && gc >= 0) ? Gin[gr + gc * ldg] : 0.0;
       Ja[r * HP + c] = J1[r + c * S];
       Jb[r * HP + c] = J2[r + c * S];
     }
@@ -232,18 +260,18 @@ symj_apply_kernel(int64_t n, double* __restrict__ G, int64_t ldg, double* __rest
         const int r = e % S, c = e / S;
         const int64_t gr = ra[r], gc = cb[c];
         if (gr < 0 || gc < 0) continue;
-        G[gr + gc * ldg] = (r <= c) ? T[r * HP + c] : T[c * HP + r];
+        Gout[gr + gc * ldg] = (r <= c) ? T[r * HP + c] : T[c * HP + r];
       }
     } else {
       for (int e = tid; e < S * S; e += 256) {
         const int r = e % S, c = e / S;
         const int64_t gr = ra[r], gc = cb[c];
-        if (gr >= 0 && gc >= 0) G[gr + gc * ldg] = T[r * HP + c];
+        if (gr >= 0 && gc >= 0) Gout[gr + gc * ldg] = T[r * HP + c];
       }
       for (int e = tid; e < S * S; e += 256) {
         const int r = e % S, c = e / S;  // mirror: row cb[r], column ra[c]
         const int64_t gr = cb[r], gc = ra[c];
-        if (gr >= 0 && gc >= 0) G[gr + gc * ldg] = T[c * HP + r];
+        if (gr >= 0 && gc >= 0) Gout[gr + gc * ldg] = T[c * HP + r];
       }
     }
     return;
@@ -289,6 +317,115 @@ symj_apply_kernel(int64_t n, double* __restrict__ G, int64_t ldg, double* __rest
   }
 }
 
+// One launch per round: the apply of round r (blocks nsolve..) and, in the
+// same launch, the inner sweeps of round r1 = the next round (blocks
+// 0..nsolve-1), so a round costs max(apply, solve) instead of their sum.  A
+// solve workgroup builds its pair's tile of G_{r+1} = J_r^T G_r J_r itself:
+// its two blocks a and b sat in pairs wa and wb of round r, so the diagonal
+// blocks are sub-blocks of those pairs' final tiles (Hin, kept by round r's
+// solves) and the off-diagonal block is Ja^T G_r(cols(wa), cols(wb)) Jb with
+// Ja, Jb the columns of J_wa, J_wb that belong to a and b.  G is read from
+// Gin and written to Gout (double-buffered), J and the tiles alternate too.
+template <int B>
+__global__ void __launch_bounds__(256)
+symj_step_kernel(int64_t n, const double* __restrict__ Gin, double* __restrict__ Gout,
+                 int64_t ldg, double* __restrict__ V, int64_t ldv, int nblocks, int N, int round,
+                 int round1, int nsolve, const double* __restrict__ fro2,
+                 const double* __restrict__ Jin, const double* __restrict__ Hin,
+                 double* __restrict__ Jout, double* __restrict__ Hout, int ntg,
+                 unsigned long long* __restrict__ flag) {
+  constexpr int S = 2 * B, HP = S + 1, BP = B + 1;
+  // solve role: Ha, Hb, Jl, Tab (S x HP), JAh, JBh, X3 (S x BP); apply role:
+  // T, Ja, Jb, X (S x HP)
+  __shared__ double sm[4 * S * HP + 3 * S * BP];
+  __shared__ double rc[B], rs[B];
+  __shared__ int64_t c0[S], c1[S];
+  const int tid = threadIdx.x;
+  if ((int)blockIdx.x >= nsolve) {
+    jac_apply<B>((int)blockIdx.x - nsolve, n, Gin, Gout, ldg, V, ldv, nblocks, N, round, Jin, ntg,
+                 sm, sm + S * HP, sm + 2 * S * HP, sm + 3 * S * HP, c0, c1);
+    return;
+  }
+  double* Ha = sm;
+  double* Hb = sm + S * HP;
+  double* Jl = sm + 2 * S * HP;
+  double* Tab = sm + 3 * S * HP;
+  double* JAh = sm + 4 * S * HP;
+  double* JBh = JAh + S * BP;
+  double* X3 = JBh + S * BP;
+  const int w = blockIdx.x;
+  // the blocks of pair w in round1 and where they sat in round r
+  const int xa = player(w, round1, N), xb = player(N - 1 - w, round1, N);
+  auto locate = [&](int x, int* wp, int* h) {
+    const int pos = (x == 0) ? 0 : 1 + (((x - 1 - round) % (N - 1)) + (N - 1)) % (N - 1);
+    if (pos < N / 2) {
+      *wp = pos;
+      *h = 0;
+    } else {
+      *wp = N - 1 - pos;
+      *h = 1;
+    }
+  };
+  int wa, ha, wb, hb;
+  locate(xa, &wa, &ha);
+  locate(xb, &wb, &hb);
+  if (tid < S) {
+    c0[tid] = pair_col<B>(wa, tid, round, N, nblocks, n);
+    c1[tid] = pair_col<B>(wb, tid, round, N, nblocks, n);
+  }
+  __syncthreads();
+  const double* JA = Jin + (size_t)wa * S * S;
+  const double* JB = Jin + (size_t)wb * S * S;
+  const double* HA = Hin + (size_t)wa * S * S;
+  const double* HB = Hin + (size_t)wb * S * S;
+  for (int e = tid; e < S * S; e += 256) {
+    const int i = e % S, j = e / S;
+    const int64_t gi = c0[i], gj = c1[j];
+    Tab[i * HP + j] = (gi >= 0 && gj >= 0) ? Gin[gi + gj * ldg] : 0.0;
+    Jl[i * HP + j] = (i == j) ? 1.0 : 0.0;
+  }
+  for (int e = tid; e < S * B; e += 256) {
+    const int i = e % S, c = e / S;
+    JAh[i * BP + c] = JA[i + (ha * B + c) * S];
+    JBh[i * BP + c] = JB[i + (hb * B + c) * S];
+  }
+  {
+    // diagonal blocks of the new tile from round r's final tiles
+    const int c = tid % B, d = tid / B;
+    Ha[c * HP + d] = HA[(ha * B + c) + (ha * B + d) * S];
+    Ha[(B + c) * HP + B + d] = HB[(hb * B + c) + (hb * B + d) * S];
+  }
+  const double flo = 4.0 * 2.220446049250313e-16 * sqrt(*fro2);
+  __syncthreads();
+  {
+    // X3 = Tab JBh (S x B): thread (row i, columns 2 k, 2 k + 1)
+    const int i = tid % S, cc = 2 * (tid / S);
+    double s0 = 0.0, s1 = 0.0;
+#pragma unroll 8
+    for (int j = 0; j < S; ++j) {
+      const double t = Tab[i * HP + j];
+      s0 = fma(t, JBh[j * BP + cc], s0);
+      s1 = fma(t, JBh[j * BP + cc + 1], s1);
+    }
+    X3[i * BP + cc] = s0;
+    X3[i * BP + cc + 1] = s1;
+  }
+  __syncthreads();
+  {
+    // H_ab = JAh^T X3 (B x B), mirrored into H_ba
+    const int c = tid % B, d = tid / B;
+    double acc = 0.0;
+#pragma unroll 8
+    for (int i = 0; i < S; ++i) acc = fma(JAh[i * BP + c], X3[i * BP + d], acc);
+    Ha[c * HP + B + d] = acc;
+    Ha[(B + d) * HP + c] = acc;
+  }
+  __syncthreads();
+  int sig = 0;
+  const double* Hf = jac_inner<B>(Ha, Hb, Jl, rc, rs, round1 > 0, flo, &sig);
+  jac_out<B>(Hf, Jl, w, Jout, Hout, sig, flag);
+}
+
 __global__ void symj_identity_kernel(int64_t n, double* __restrict__ V, int64_t ldv) {
   const int64_t total = n * n;
   for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
@@ -313,28 +450,46 @@ int run_sweeps(Context* c, int64_t n, double* G, int64_t ldg, double* V, int64_t
   const int pairs = N / 2;
   const int ntg = pairs * (pairs + 1) / 2;
   const int ntv = V ? pairs * (int)((n + 2 * S - 1) / (2 * S)) : 0;
-  DBuf<double> J, fro2;
-  DBuf<unsigned long long> off;
-  RSVD_TRY(J.alloc((size_t)pairs * S * S));
+  DBuf<double> G2, Jb, Hb, fro2;
+  DBuf<unsigned long long> flags;
+  const size_t js = (size_t)pairs * S * S;
+  RSVD_TRY(G2.alloc((size_t)(ldg * n)));
+  RSVD_TRY(Jb.alloc(2 * js));
+  RSVD_TRY(Hb.alloc(2 * js));
   RSVD_TRY(fro2.alloc(1));
-  RSVD_TRY(off.alloc(1));
+  RSVD_TRY(flags.alloc((size_t)max_sweeps + 2));
   RSVD_TRY(sumsq_f64(c, n, n, G, ldg, fro2.p));
+  RSVD_HIP(hipMemsetAsync(flags.p, 0, sizeof(unsigned long long) * (max_sweeps + 2), c->stream));
+  double* cur = G;
+  double* nxt = G2.p;
+  int jc = 0;
   int sweep = 0;
   KTimer kt(c, "symeig");
   if (n > 1) {
+    // round 0 of sweep 0 from G itself; afterwards every launch applies
+    // round r and solves the next round
+    hipLaunchKernelGGL(symj_solve_kernel<B>, dim3(pairs), dim3(256), 0, c->stream, n,
+                       (const double*)G, ldg, nblocks, N, 0, (const double*)fro2.p, Jb.p, Hb.p,
+                       flags.p);
+    RSVD_LAUNCH_CHECK();
     for (; sweep < max_sweeps; ++sweep) {
-      RSVD_HIP(hipMemsetAsync(off.p, 0, sizeof(unsigned long long), c->stream));
       for (int r = 0; r < N - 1; ++r) {
-        hipLaunchKernelGGL(symj_solve_kernel<B>, dim3(pairs), dim3(256), 0, c->stream, n,
-                           (const double*)G, ldg, nblocks, N, r, (const double*)fro2.p, J.p,
-                           off.p);
+        const bool wrap = r + 1 == N - 1;
+        const int r1 = wrap ? 0 : r + 1;
+        unsigned long long* fl = flags.p + (wrap ? sweep + 1 : sweep);
+        hipLaunchKernelGGL(symj_step_kernel<B>, dim3(pairs + ntg + ntv), dim3(256), 0, c->stream,
+                           n, (const double*)cur, nxt, ldg, V, ldv, nblocks, N, r, r1, pairs,
+                           (const double*)fro2.p, (const double*)(Jb.p + jc * js),
+                           (const double*)(Hb.p + jc * js), Jb.p + (1 - jc) * js,
+                           Hb.p + (1 - jc) * js, ntg, fl);
         RSVD_LAUNCH_CHECK();
-        hipLaunchKernelGGL(symj_apply_kernel<B>, dim3(ntg + ntv), dim3(256), 0, c->stream, n, G,
-                           ldg, V, ldv, nblocks, N, r, (const double*)J.p, ntg);
-        RSVD_LAUNCH_CHECK();
+        double* tmp = cur;
+        cur = nxt;
+        nxt = tmp;
+        jc = 1 - jc;
       }
       unsigned long long h = 0;
-      RSVD_HIP(hipMemcpyAsync(&h, off.p, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+      RSVD_HIP(hipMemcpyAsync(&h, flags.p + sweep, sizeof(h), hipMemcpyDeviceToHost, c->stream));
       RSVD_HIP(hipStreamSynchronize(c->stream));
       if (h == 0) {  // no pair met in this sweep was above kStopTol
         ++sweep;
@@ -342,6 +497,9 @@ int run_sweeps(Context* c, int64_t n, double* G, int64_t ldg, double* V, int64_t
       }
     }
   }
+  if (cur != G)
+    RSVD_HIP(hipMemcpy2DAsync(G, ldg * sizeof(double), cur, ldg * sizeof(double),
+                              n * sizeof(double), n, hipMemcpyDeviceToDevice, c->stream));
   *sweep_out = sweep;
   return 0;
 }
