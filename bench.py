@@ -298,6 +298,8 @@ def bench_learned(args, rp, lib) -> int:
         "plain_ms_per_solve": round(1e3 * t_plain / len(res), 3),
         "feasible": len(feas), "speedup_feasible": round(tf_plain / max(tf_learn, 1e-12), 3),
         "fallbacks": sum(r["fell_back"] for r in res), "tol_violations": len(viol),
+        "learned_meets_tol": sum(r["resid_rel"] <= tol for r in res),
+        "full_rank_solves": sum((not r["fell_back"]) and r["k_pred"] == 512 for r in res),
         "train_loss": [float(hist[0]), float(hist[-1])],
     }
     print(json.dumps(line))

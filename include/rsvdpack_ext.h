@@ -166,13 +166,15 @@ int rsvd_dev_sketch_features(const float *dA, long long m, long long n, long lon
  * FP64 fixed-rank rSVD of dA (vnum = 2, oversampling p, q_pred + 1, i.e. q_pred
  * power iterations, k = ceil(rank (1 + margin)) + 2), and accepts it when its
  * FP64 residual ||A - U S V^T||_F = sqrt(||A||_F^2 - sum sigma^2) is at most
- * tol_rel ||A||_F; otherwise (and directly when the predicted rank reaches
- * min(m, n) - p, where a fixed-rank solve cannot reach TOL) it falls back to
+ * tol_rel ||A||_F.  A predicted k at the sketch limit (k + p >= min(m, n)) is a
+ * prediction of full rank; for m >= n it is solved as one: k = n, p = 0,
+ * Omega = I (an exact thin QR of A and the small eigenproblem), its residual
+ * formed explicitly.  A result above TOL falls back to
  * low_rank_svd_blockrand_decomp_fixed_rank_or_prec in TOL mode (TOL = tol_rel
  * ||A||_F, kstep, q_fallback) and returns that method's result, which may itself
- * miss TOL when TOL is not reachable below full rank.  Outputs as rsvd_dev_blockrand; *q_pred, *k_pred
- * the prediction, *fell_back 1 when the fallback ran, *resid_rel the relative
- * residual of the accepted result. */
+ * miss TOL when TOL is not reachable below full rank.  Outputs as
+ * rsvd_dev_blockrand; *q_pred, *k_pred the prediction, *fell_back 1 when the
+ * fallback ran, *resid_rel the relative residual of the accepted result. */
 int rsvd_dev_learned_lowrank(const float *dTheta, const double *dA, const float *dA32,
                              long long m, long long n, long long lda, double tol_rel, int p,
                              double margin, int kstep, int q_fallback, unsigned long long seed,

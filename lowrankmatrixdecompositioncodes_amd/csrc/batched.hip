@@ -238,15 +238,6 @@ __device__ void lds_gram(const float* __restrict__ X, int64_t R, int l, float (*
   __syncthreads();
 }
 
-// 1 / d to about one ulp: the hardware reciprocal estimate and two Newton
-// steps (shorter than a correctly rounded division on the serial path).
-__device__ __forceinline__ double rcp_nr2(double d) {
-  double r = __builtin_amdgcn_rcp(d);
-  r = fma(r, fma(-d, r, 1.0), r);
-  r = fma(r, fma(-d, r, 1.0), r);
-  return r;
-}
-
 // In-place upper Cholesky of G (l x l, upper triangle used) -> R in the upper
 // triangle (G = R^T R, zero below), then Ri = R^-1 (upper).  shift is added
 // to the diagonal first.  Returns through *ok (shared) whether every pivot
@@ -274,7 +265,7 @@ __device__ void lds_chol_inv(double (*G)[kLP], double (*Ri)[kLP], int l, double 
       piv[j] = d;
     }
     if (a > j && a < l) {
-      const double u = G[j][a] * rcp_nr2(d);
+      const double u = G[j][a] * rcp_nr(d);  // reciprocal: off the division path
 #pragma unroll
       for (int q = 0; q < 12; ++q)
         if (c0 + q >= a && c0 + q < l) own[q] -= u * G[j][c0 + q];
@@ -540,6 +531,7 @@ struct Plan {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   double tuned_ms[2][3] = {{0, 0, 0}, {0, 0, 0}};
+  double stage_ms[4] = {0, 0, 0, 0};  // orth of Y, orth of Z, final solve, tiling (one launch)
   ~Plan() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
@@ -685,6 +677,26 @@ int autotune(Context* c, Plan& P) {
     if (ta) P.rt_tn = best_rt;
     else P.rt_nn = best_rt;
   }
+  // the other stages, timed once each for the report (the work buffers hold
+  // real operands from the eager call; the graph recomputes everything)
+  for (int st = 0; st < 4 && rc == 0; ++st) {
+    (void)hipEventRecord(e0, c->stream);
+    if (st == 0) rc = launch_orth(c, P, false);
+    else if (st == 1) rc = launch_orth(c, P, true);
+    else if (st == 2) {
+      hipLaunchKernelGGL(bfinal_kernel, dim3((unsigned)P.count), dim3(256), 0, c->stream, P.mats.p,
+                         P.work.p, P.U, P.S, P.V);
+      rc = hipGetLastError() == hipSuccess ? 0 : set_error("batched: final launch failed");
+    } else {
+      rc = launch_tile(c, P, false);
+    }
+    (void)hipEventRecord(e1, c->stream);
+    if (rc) break;
+    (void)hipEventSynchronize(e1);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    P.stage_ms[st] = ms;
+  }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   return rc;
@@ -782,11 +794,13 @@ int fixed_rank_batched(Context* c, int count, const int* m, const int* n, const 
     if (ce != hipSuccess) return set_error("batched: graph capture failed: %s", hipGetErrorString(ce));
     P->graph = g;
     RSVD_HIP(hipGraphInstantiate(&P->exec, g, nullptr, nullptr, 0));
-    char buf[160];
+    char buf[320];
     snprintf(buf, sizeof buf,
-             "rt_nn=%d rt_tn=%d (NN ms %.3f/%.3f/%.3f, TN ms %.3f/%.3f/%.3f for RT 1/2/4)",
+             "rt_nn=%d rt_tn=%d (NN ms %.3f/%.3f/%.3f, TN ms %.3f/%.3f/%.3f for RT 1/2/4); "
+             "one launch each: orth Y %.3f ms, orth Z %.3f ms, final %.3f ms, tile %.3f ms",
              P->rt_nn, P->rt_tn, P->tuned_ms[0][0], P->tuned_ms[0][1], P->tuned_ms[0][2],
-             P->tuned_ms[1][0], P->tuned_ms[1][1], P->tuned_ms[1][2]);
+             P->tuned_ms[1][0], P->tuned_ms[1][1], P->tuned_ms[1][2], P->stage_ms[0],
+             P->stage_ms[1], P->stage_ms[2], P->stage_ms[3]);
     g_last_config = buf;
     g_plans[key] = std::move(np);
   } else {

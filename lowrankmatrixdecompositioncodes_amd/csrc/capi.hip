@@ -921,33 +921,53 @@ int rsvd_dev_learned_lowrank(const float* dTheta, const double* dA, const float*
   double rk = std::isfinite(o[5]) ? o[5] : 1.0;
   long long k = (long long)ceil(rk * (double)mn * (1.0 + margin)) + 2;
   if (k < 1) k = 1;
-  if (k > mn - p) k = mn - p;
+  // A predicted rank at the sketch limit (k + p >= min(m, n)) is a prediction
+  // of full rank: for m >= n it is solved as such -- k = n, no oversampling,
+  // Omega = I, so Y = A and the pipeline is an exact thin QR of A followed by
+  // the small eigenproblem (q is moot: Y already spans range(A)).  Below the
+  // limit: one fixed-rank rSVD with the predicted q and k.
+  const bool full = k >= mn - p && m >= n;
+  if (full) k = mn;
+  else if (k > mn - p) k = mn - p;
+  const int pp = full ? 0 : p, qq = full ? 1 : q + 1;
   if (q_pred) *q_pred = q;
   if (k_pred) *k_pred = (int)k;
-  // a predicted rank at the sketch limit means the fixed-rank solve cannot
-  // reach TOL: go to the adaptive method directly
-  const bool try_fixed = k < mn - p;
+  const bool try_fixed = full || k < mn - p;
   // one fixed-rank rSVD, judged by its FP64 residual
   double *u = nullptr, *sv = nullptr, *v = nullptr;
   OmegaSrc om;
   om.seed = seed;
+  DBuf<double> eye;
+  if (full) {
+    RSVD_TRY(eye.alloc((size_t)(n * n)));
+    RSVD_TRY(fill2d_f64(c, n, n, 0.0, eye.p, n));
+    RSVD_TRY(fill2d_f64(c, 1, n, 1.0, eye.p, n + 1));  // the diagonal
+    om.dev = eye.p;
+    om.ld = n;
+  }
   if (try_fixed) {
     RSVD_HIP(hipMalloc(&u, sizeof(double) * (size_t)(m * k)));
     RSVD_HIP(hipMalloc(&sv, sizeof(double) * (size_t)k));
     RSVD_HIP(hipMalloc(&v, sizeof(double) * (size_t)(n * k)));
-    int rc = fixed_rank_f64(c, dA, m, n, lda, (int)k, p, 2, q + 1, 1, om, u, m, sv, v, n, nullptr);
-    double s2 = 0;
-    if (rc == 0) rc = sumsq_f64(c, k, 1, sv, k, nrm.p);
-    if (rc == 0 && hipMemcpyAsync(&s2, nrm.p, sizeof s2, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
-      rc = set_error("learned: copy failed");
-    if (rc == 0) rc = stream_sync(c);
+    int rc = fixed_rank_f64(c, dA, m, n, lda, (int)k, pp, 2, qq, 1, om, u, m, sv, v, n, nullptr);
+    double s2 = 0, res = 0;
+    if (full) {
+      // ||A||^2 - sum s^2 cancels for an exact decomposition: form the
+      // residual explicitly (one m x n x n product)
+      if (rc == 0) rc = lowrank_residual(c, dA, m, n, lda, k, u, m, sv, v, n, 0, &res);
+    } else {
+      if (rc == 0) rc = sumsq_f64(c, k, 1, sv, k, nrm.p);
+      if (rc == 0 && hipMemcpyAsync(&s2, nrm.p, sizeof s2, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+        rc = set_error("learned: copy failed");
+      if (rc == 0) rc = stream_sync(c);
+      res = sqrt(a2 > s2 ? a2 - s2 : 0.0);
+    }
     if (rc) {
       (void)hipFree(u);
       (void)hipFree(sv);
       (void)hipFree(v);
       return rc;
     }
-    const double res = sqrt(a2 > s2 ? a2 - s2 : 0.0);
     if (res <= tol_rel * sqrt(a2)) {
       *frank = (int)k;
       *dU = u;
@@ -962,6 +982,8 @@ int rsvd_dev_learned_lowrank(const float* dTheta, const double* dA, const float*
     (void)hipFree(v);
   }
   // fallback: the adaptive blocked method in TOL mode
+  om.dev = nullptr;
+  om.ld = 0;
   double s2 = 0;
   BlockrandOut bo;
   RSVD_TRY(blockrand_f64(c, dA, m, n, lda, -1, p, tol_rel * sqrt(a2), 2, kstep, q_fallback, 1, om,

@@ -32,11 +32,14 @@ static __device__ __forceinline__ double rsq_nr(double x) {
 
 // Rotation (c, s) with [x' y'] = [x y] [[c, s], [-s, c]] zeroing the (p, q)
 // entry of a symmetric 2x2 [[a, g], [g, b]]: t = tan theta is the root of
-// g t^2 + (b - a) t - g = 0 of smaller modulus, t = sgn(d) 2g / (|d| +
-// sqrt(d^2 + 4g^2)), d = b - a (Rutishauser's choice), evaluated on the
-// entries scaled by a power of two so that d^2 + 4g^2 stays in range.
-// Branch-free.  No rotation when |g| <= flo or |g| <= kRotTol sqrt|ab|; *sig
-// reports |g| > kStopTol sqrt|ab|.
+// g t^2 + (b - a) t - g = 0 of smaller modulus, t = sgn(d) 2g / (|d| + r),
+// d = b - a, r = sqrt(d^2 + 4g^2) (Rutishauser's choice).  With den = |d| + r,
+// 1 + t^2 = 2r / den, so c = 1 / sqrt(1 + t^2) = den / sqrt(2 r den) and
+// s = c t = sgn(d) 2g / sqrt(2 r den): two reciprocal square roots on the
+// serial path and c^2 + s^2 = (den^2 + 4g^2) / (2 r den) = 1 identically.
+// Evaluated on the entries scaled by a power of two so that the squares stay
+// in range.  Branch-free.  No rotation when |g| <= flo or |g| <= kRotTol
+// sqrt|ab|; *sig reports |g| > kStopTol sqrt|ab|.
 static __device__ __forceinline__ void rotation(double a, double b, double g, double flo, double* cs,
                                          double* sn, int* sig) {
   const double ag = fabs(g);
@@ -49,12 +52,12 @@ static __device__ __forceinline__ void rotation(double a, double b, double g, do
   const bool rot = big && g2 > kRotTol * kRotTol * ab;
   const double d = sb - sa;
   const double r2 = rot ? fma(d, d, 4.0 * g2) : 1.0;
-  const double den = fabs(d) + r2 * rsq_nr(r2);
-  double t = 2.0 * sg * rcp_nr(den);
-  t = (d < 0.0) ? -t : t;
-  const double c = rsq_nr(fma(t, t, 1.0));
-  *cs = rot ? c : 1.0;
-  *sn = rot ? c * t : 0.0;
+  const double r = r2 * rsq_nr(r2);
+  const double den = fabs(d) + r;
+  const double q = rsq_nr(2.0 * r * den);
+  const double t2g = (d < 0.0) ? -2.0 * sg : 2.0 * sg;
+  *cs = rot ? den * q : 1.0;
+  *sn = rot ? t2g * q : 0.0;
 }
 
 }  // namespace rsvd

@@ -11,7 +11,9 @@ this module drives them through the C ABI:
   for q = 0..4 and keeps the cheapest by a pass-count cost model,
   (2 q + 2) (frank_q + kstep): q* and rank* / min(m, n).
 * train(...): Adam on the GPU with the library's forward / backward kernels.
-* evaluate(...): learned path (one fixed-rank rSVD, FP64 residual check,
+* evaluate(...): learned path (one fixed-rank rSVD with the predicted q and
+  rank -- a predicted rank at the sketch limit is solved as full rank, an
+  exact thin QR of A and the small eigenproblem --, FP64 residual check,
   fallback) against the plain adaptive path at the same TOL.
 """
 from __future__ import annotations
