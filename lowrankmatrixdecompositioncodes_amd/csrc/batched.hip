@@ -46,6 +46,7 @@ namespace {
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float f4v_t __attribute__((ext_vector_type(4)));
 typedef float f4s_t __attribute__((ext_vector_type(4)));
+typedef double d4_t __attribute__((ext_vector_type(4)));
 
 constexpr int kL = 48;       // padded sketch width
 constexpr int kNT = kL / 16; // 16-wide column tiles
@@ -199,42 +200,77 @@ bpass_kernel(const BMat* __restrict__ mats, const BItem* __restrict__ items, int
 }
 
 // ---------------------------------------------------------------- LDS helpers
-// Gram G = X^T X (the first l columns; FP64 accumulation of fp32 products)
-// of X (R x 48, ld R), staged in 64-row chunks.  Thread t owns the 3 x 3
-// block (3 (t / 16), 3 (t % 16)) of the 48 x 48 result: per row it reads six
-// values (the three of its row block are the same for the 16 threads that
-// share it, so the wave's reads broadcast) for nine independent FMAs.
+// A 64-row chunk of X (R x 48 fp32, ld R, first l columns) staged through
+// registers: thread t holds entries e = t + 256 q (row e % 64, column e / 64);
+// zeros past R and past l.
+constexpr int kChunkLoads = 64 * kL / 256;
+__device__ __forceinline__ void chunk_fetch(const float* __restrict__ X, int64_t R, int l,
+                                            int64_t r0, float (&pre)[kChunkLoads]) {
+#pragma unroll
+  for (int q = 0; q < kChunkLoads; ++q) {
+    const int e = threadIdx.x + 256 * q, rr = e & 63, c = e >> 6;
+    pre[q] = (r0 + rr < R && c < l) ? X[r0 + rr + (int64_t)c * R] : 0.0f;
+  }
+}
+__device__ __forceinline__ void chunk_store(float (*Xs)[kL + 1], const float (&pre)[kChunkLoads]) {
+#pragma unroll
+  for (int q = 0; q < kChunkLoads; ++q) {
+    const int e = threadIdx.x + 256 * q;
+    Xs[e & 63][e >> 6] = pre[q];
+  }
+}
+
+// FP64 matrix cores (v_mfma_f64_16x16x4f64), per 64-lane wave: srcA lane l
+// holds X[i = l % 16][k = l / 16], srcB lane l holds Y[k = l / 16][j = l % 16],
+// and the accumulator's register r of lane l holds D[4 r + l / 16][l % 16].
+
+// Gram G = X^T X (both triangles; FP64 products of the fp32 entries, exact,
+// and FP64 accumulation) of X (R x 48, ld R, first l columns, zero beyond),
+// staged in 64-row chunks.  Wave w takes rows 16 w .. 16 w + 15 of every
+// chunk (four k steps of 4) for the six upper 16 x 16 tiles; the waves'
+// partial tiles are added in wave order at the end (deterministic).
 __device__ void lds_gram(const float* __restrict__ X, int64_t R, int l, float (*Xs)[kL + 1],
                          double (*G)[kLP]) {
-  const int tid = threadIdx.x;
-  const int i0 = 3 * (tid >> 4), j0 = 3 * (tid & 15);
-  double acc[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  d4_t acc[6];
+#pragma unroll
+  for (int t = 0; t < 6; ++t) acc[t] = d4_t{0.0, 0.0, 0.0, 0.0};
+  float pre[kChunkLoads];
+  chunk_fetch(X, R, l, 0, pre);
   for (int64_t r0 = 0; r0 < R; r0 += 64) {
-    const int rows = (int)((R - r0) < 64 ? (R - r0) : 64);
     __syncthreads();
-    for (int e = tid; e < 64 * kL; e += 256) {
-      const int rr = e & 63, c = e >> 6;
-      Xs[rr][c] = (rr < rows && c < l) ? X[r0 + rr + (int64_t)c * R] : 0.0f;
-    }
+    chunk_store(Xs, pre);
     __syncthreads();
-    for (int rr = 0; rr < rows; ++rr) {
-      double xi[3], xj[3];
+    if (r0 + 64 < R) chunk_fetch(X, R, l, r0 + 64, pre);  // in flight during the MFMAs
 #pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        xi[a] = (double)Xs[rr][i0 + a];
-        xj[a] = (double)Xs[rr][j0 + a];
-      }
-#pragma unroll
-      for (int a = 0; a < 3; ++a)
-#pragma unroll
-        for (int b = 0; b < 3; ++b) acc[a][b] += xi[a] * xj[b];
+    for (int s = 0; s < 4; ++s) {
+      const int row = 16 * wave + 4 * s + lk;
+      const double v0 = (double)Xs[row][li], v1 = (double)Xs[row][16 + li],
+                   v2 = (double)Xs[row][32 + li];
+      acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(v0, v0, acc[0], 0, 0, 0);
+      acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(v0, v1, acc[1], 0, 0, 0);
+      acc[2] = __builtin_amdgcn_mfma_f64_16x16x4f64(v0, v2, acc[2], 0, 0, 0);
+      acc[3] = __builtin_amdgcn_mfma_f64_16x16x4f64(v1, v1, acc[3], 0, 0, 0);
+      acc[4] = __builtin_amdgcn_mfma_f64_16x16x4f64(v1, v2, acc[4], 0, 0, 0);
+      acc[5] = __builtin_amdgcn_mfma_f64_16x16x4f64(v2, v2, acc[5], 0, 0, 0);
     }
   }
+  const int ti[6] = {0, 0, 0, 1, 1, 2}, tj[6] = {0, 1, 2, 1, 2, 2};
+  for (int w = 0; w < 4; ++w) {
+    __syncthreads();
+    if (wave == w) {
 #pragma unroll
-  for (int a = 0; a < 3; ++a)
+      for (int t = 0; t < 6; ++t)
 #pragma unroll
-    for (int b = 0; b < 3; ++b)
-      if (i0 + a < l && j0 + b < l) G[i0 + a][j0 + b] = acc[a][b];
+        for (int r = 0; r < 4; ++r) {
+          const int i = 16 * ti[t] + 4 * r + lk, j = 16 * tj[t] + li;
+          const double v = (w == 0) ? acc[t][r] : G[i][j] + acc[t][r];
+          G[i][j] = v;
+          if (ti[t] != tj[t]) G[j][i] = v;
+        }
+    }
+  }
   __syncthreads();
 }
 
@@ -308,39 +344,50 @@ __device__ void lds_chol_inv(double (*G)[kLP], double (*Ri)[kLP], int l, double 
 }
 
 // out (R x kout, ld ldo) = scale_j * X W for X (R x 48, ld R, first l
-// columns used) and W (l x kout, LDS, FP64), in 64-row chunks staged in LDS:
-// thread (row rr = tid % 64, column group tid / 64) owns 12 outputs of a row.
-// out may alias X when kout == l and W is upper triangular (each chunk is
-// fully read before it is written).
+// columns used) and W (48 x kout, LDS, FP64; rows >= l must be finite), in
+// 64-row chunks staged in LDS, on the FP64 matrix cores: wave w computes rows
+// 16 w .. 16 w + 15 of the chunk against the three 16-column tiles of W
+// (twelve k steps of 4).  The results go back through the chunk's LDS so the
+// stores are coalesced; out may alias X (each chunk is fully read first).
 __device__ void lds_rows_times(const float* X, int64_t R, int l, float (*Xs)[kL + 1],
                                const double (*W)[kLP], int kout, const double* scale, float* out,
                                int64_t ldo) {
-  const int tid = threadIdx.x, rr = tid & 63, cg = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  float pre[kChunkLoads];
+  chunk_fetch(X, R, l, 0, pre);
   for (int64_t r0 = 0; r0 < R; r0 += 64) {
     const int rows = (int)((R - r0) < 64 ? (R - r0) : 64);
     __syncthreads();
-    for (int e = tid; e < 64 * l; e += 256) {
-      const int r = e & 63, c = e >> 6;
-      Xs[r][c] = (r < rows) ? X[r0 + r + (int64_t)c * R] : 0.0f;
-    }
+    chunk_store(Xs, pre);
     __syncthreads();
-    double o[12];
+    // the next chunk's rows are disjoint from this chunk's output rows, so the
+    // prefetch is safe when out aliases X
+    if (r0 + 64 < R) chunk_fetch(X, R, l, r0 + 64, pre);
+    d4_t acc[kNT];
 #pragma unroll
-    for (int j = 0; j < 12; ++j) o[j] = 0.0;
-    for (int u = 0; u < l; ++u) {
-      const double xu = (double)Xs[rr][u];
+    for (int t = 0; t < kNT; ++t) acc[t] = d4_t{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-      for (int j = 0; j < 12; ++j) {
-        const int c = cg * 12 + j;
-        if (c < kout) o[j] += xu * W[u][c];
-      }
+    for (int s = 0; s < kL / 4; ++s) {
+      const double a = (double)Xs[16 * wave + li][4 * s + lk];
+#pragma unroll
+      for (int t = 0; t < kNT; ++t)
+        acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, W[4 * s + lk][16 * t + li], acc[t], 0, 0,
+                                                      0);
     }
-    if (rr < rows) {
+    __syncthreads();  // every wave has read the chunk
 #pragma unroll
-      for (int j = 0; j < 12; ++j) {
-        const int c = cg * 12 + j;
-        if (c < kout) out[r0 + rr + (int64_t)c * ldo] = (float)(scale ? o[j] * scale[c] : o[j]);
+    for (int t = 0; t < kNT; ++t)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int c = 16 * t + li;
+        const double v = acc[t][r];
+        Xs[16 * wave + 4 * r + lk][c] = (float)((scale && c < kout) ? v * scale[c] : v);
       }
+    __syncthreads();
+    for (int e = tid; e < 64 * kout; e += 256) {
+      const int r = e & 63, c = e >> 6;
+      if (r < rows) out[r0 + r + (int64_t)c * ldo] = Xs[r][c];
     }
   }
   __syncthreads();
@@ -349,13 +396,14 @@ __device__ void lds_rows_times(const float* X, int64_t R, int l, float (*Xs)[kL 
 // ---------------------------------------------------------------- orth
 // CholeskyQR2 of Y_i (FROM_Z: Z_i) per matrix, one workgroup each.
 template <bool FROM_Z>
-__global__ void __launch_bounds__(256)
-borth_kernel(const BMat* __restrict__ mats, float* __restrict__ work, int* __restrict__ flags) {
+__global__ void __launch_bounds__(256, 2)
+borth_kernel(const BMat* __restrict__ mats, const int* __restrict__ order, float* __restrict__ work,
+             int* __restrict__ flags) {
   __shared__ float Xs[64][kL + 1];
   __shared__ double G[kL][kLP];
   __shared__ double Ri[kL][kLP];
   __shared__ int ok;
-  const BMat d = mats[blockIdx.x];
+  const BMat d = mats[order[blockIdx.x]];  // largest first
   const int64_t R = FROM_Z ? d.n : d.m;
   float* X = work + (FROM_Z ? d.z : d.y);
   const int l = d.l;
@@ -377,13 +425,14 @@ __device__ __forceinline__ int circle(int pos, int r, int N) {
 
 // two-sided cyclic Jacobi on H (np x np, np even, positive semidefinite) with
 // J accumulated.  A step: threads 0..np/2-1 compute the rotation of their pair
-// once (round-robin pairing) and publish it; after a barrier the 2 x 2 blocks
-// of H (pairs bi x bj) and the (row, pair) entries of J are updated in
-// parallel into the second H buffer.  Sweeps until no pair met in a sweep had
-// a relative off-diagonal above kStopTol.
+// once (round-robin pairing) and publish it with the pair's two indices;
+// after a barrier the 2 x 2 blocks of H (pairs bi x bj) and the (row, pair)
+// entries of J are updated in parallel into the second H buffer.  Sweeps
+// until no pair met in a sweep had a relative off-diagonal above kStopTol.
 __device__ void lds_jacobi(double (*H)[kLP], double (*H2)[kLP], double (*J)[kLP], int np,
                            int* sig_flag) {
   __shared__ double rc[kL / 2], rs[kL / 2];
+  __shared__ int pp[kL / 2], qq[kL / 2];
   const int tid = threadIdx.x;
   const int half = np / 2;
   double (*Hc)[kLP] = H;
@@ -406,13 +455,14 @@ __device__ void lds_jacobi(double (*H)[kLP], double (*H2)[kLP], double (*J)[kLP]
         rotation(Hc[p][p], Hc[q][q], Hc[p][q], flo, &c, &sn, &sg);
         rc[tid] = c;
         rs[tid] = sn;
+        pp[tid] = p;
+        qq[tid] = q;
         sig |= sg;
       }
       __syncthreads();
       for (int e = tid; e < half * half; e += 256) {
-        const int bi = e / half, bj = e % half;
-        const int pi = circle(bi, t, np), qi = circle(np - 1 - bi, t, np);
-        const int pj = circle(bj, t, np), qj = circle(np - 1 - bj, t, np);
+        const int bi = e / half, bj = e - bi * half;
+        const int pi = pp[bi], qi = qq[bi], pj = pp[bj], qj = qq[bj];
         const double ci = rc[bi], si = rs[bi], cj = rc[bj], sj = rs[bj];
         const double h00 = Hc[pi][pj], h01 = Hc[pi][qj], h10 = Hc[qi][pj], h11 = Hc[qi][qj];
         const double x00 = cj * h00 - sj * h01, x01 = sj * h00 + cj * h01;
@@ -426,8 +476,8 @@ __device__ void lds_jacobi(double (*H)[kLP], double (*H2)[kLP], double (*J)[kLP]
         Hn[qi][qj] = y11;
       }
       for (int e = tid; e < np * half; e += 256) {
-        const int x = e % np, k = e / np;
-        const int pk = circle(k, t, np), qk = circle(np - 1 - k, t, np);
+        const int k = e / np, x = e - k * np;
+        const int pk = pp[k], qk = qq[k];
         const double ck = rc[k], sk = rs[k];
         const double jp = J[x][pk], jq = J[x][qk];
         J[x][pk] = ck * jp - sk * jq;
@@ -451,8 +501,9 @@ __device__ void lds_jacobi(double (*H)[kLP], double (*H2)[kLP], double (*J)[kLP]
   __syncthreads();
 }
 
-__global__ void __launch_bounds__(256)
-bfinal_kernel(const BMat* __restrict__ mats, float* __restrict__ work, float* __restrict__ U,
+__global__ void __launch_bounds__(256, 2)
+bfinal_kernel(const BMat* __restrict__ mats, const int* __restrict__ mat_order,
+              float* __restrict__ work, float* __restrict__ U,
               double* __restrict__ S, float* __restrict__ V) {
   __shared__ float Xs[64][kL + 1];
   __shared__ double H[kL][kLP];
@@ -461,7 +512,7 @@ bfinal_kernel(const BMat* __restrict__ mats, float* __restrict__ work, float* __
   __shared__ double sig[kL];
   __shared__ int order[kL];
   __shared__ int flag;
-  const BMat d = mats[blockIdx.x];
+  const BMat d = mats[mat_order[blockIdx.x]];  // largest first
   const int l = d.l, k = d.k;
   const int np = l + (l & 1);
   const float* Q = work + d.y;   // m x 48, orthonormal
@@ -494,9 +545,9 @@ bfinal_kernel(const BMat* __restrict__ mats, float* __restrict__ work, float* __
   }
   __syncthreads();
   // selected eigenvectors (columns order[l - k + j]) into H2, 1 / sigma
-  for (int e = tid; e < l * k; e += 256) {
-    const int c = e % l, j = e / l;
-    H2[c][j] = J[c][order[l - k + j]];
+  for (int e = tid; e < kL * kL; e += 256) {
+    const int c = e % kL, j = e / kL;
+    H2[c][j] = (c < l && j < k) ? J[c][order[l - k + j]] : 0.0;  // zero padding: finite
   }
   if (tid < k) sig[tid] = 1.0 / sig[tid];
   __syncthreads();
@@ -528,6 +579,7 @@ struct Plan {
   float* V = nullptr;
   std::vector<BMat> host;
   std::vector<int> by_cost;  // matrices, largest first
+  DBuf<int> order;           // by_cost on the device (per-matrix kernels)
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   double tuned_ms[2][3] = {{0, 0, 0}, {0, 0, 0}};
@@ -539,7 +591,13 @@ struct Plan {
 };
 
 std::mutex g_plan_mu;
-std::map<std::vector<int64_t>, std::unique_ptr<Plan>> g_plans;
+// Heap-allocated and never destroyed: a Plan's destructor calls into the HIP
+// runtime (graph destruction) and the buffer pool, and at process exit those
+// may already be gone (static destruction order across translation units and
+// libraries is unspecified).  rsvd_release_cache releases the plans while the
+// runtime is alive.
+std::map<std::vector<int64_t>, std::unique_ptr<Plan>>& g_plans =
+    *new std::map<std::vector<int64_t>, std::unique_ptr<Plan>>();
 std::string g_last_config;
 
 std::vector<BItem> pass_items(const Plan& P, bool ta, int rt) {
@@ -609,10 +667,10 @@ int launch_tile(Context* c, Plan& P, bool from_y) {
 int launch_orth(Context* c, Plan& P, bool from_z) {
   if (from_z)
     hipLaunchKernelGGL((borth_kernel<true>), dim3((unsigned)P.count), dim3(256), 0, c->stream,
-                       P.mats.p, P.work.p, P.flags.p);
+                       P.mats.p, (const int*)P.order.p, P.work.p, P.flags.p);
   else
     hipLaunchKernelGGL((borth_kernel<false>), dim3((unsigned)P.count), dim3(256), 0, c->stream,
-                       P.mats.p, P.work.p, P.flags.p);
+                       P.mats.p, (const int*)P.order.p, P.work.p, P.flags.p);
   RSVD_LAUNCH_CHECK();
   return 0;
 }
@@ -638,7 +696,7 @@ int enqueue_all(Context* c, Plan& P) {
   RSVD_TRY(launch_tile(c, P, true));
   RSVD_TRY(launch_pass(c, P, true, P.rt_tn, P.it_tn.p, P.n_tn));  // B^T = A^T Q
   hipLaunchKernelGGL(bfinal_kernel, dim3((unsigned)P.count), dim3(256), 0, c->stream, P.mats.p,
-                     P.work.p, P.U, P.S, P.V);
+                     (const int*)P.order.p, P.work.p, P.U, P.S, P.V);
   RSVD_LAUNCH_CHECK();
   return 0;
 }
@@ -685,7 +743,7 @@ int autotune(Context* c, Plan& P) {
     else if (st == 1) rc = launch_orth(c, P, true);
     else if (st == 2) {
       hipLaunchKernelGGL(bfinal_kernel, dim3((unsigned)P.count), dim3(256), 0, c->stream, P.mats.p,
-                         P.work.p, P.U, P.S, P.V);
+                         (const int*)P.order.p, P.work.p, P.U, P.S, P.V);
       rc = hipGetLastError() == hipSuccess ? 0 : set_error("batched: final launch failed");
     } else {
       rc = launch_tile(c, P, false);
@@ -765,6 +823,9 @@ int fixed_rank_batched(Context* c, int count, const int* m, const int* n, const 
     std::stable_sort(P->by_cost.begin(), P->by_cost.end(), [&](int a, int b) {
       return P->host[a].m * P->host[a].n > P->host[b].m * P->host[b].n;
     });
+    RSVD_TRY(P->order.alloc(count));
+    RSVD_HIP(hipMemcpyAsync(P->order.p, P->by_cost.data(), count * sizeof(int),
+                            hipMemcpyHostToDevice, c->stream));
     RSVD_TRY(P->mats.alloc(count));
     RSVD_HIP(hipMemcpyAsync(P->mats.p, P->host.data(), count * sizeof(BMat),
                             hipMemcpyHostToDevice, c->stream));
