@@ -48,19 +48,24 @@ TRAFFIC_PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "prof
 
 
 def measured_traffic(tag, m, n, l):
-    """HBM bytes per launch of the split-bf16 A pass, from the committed
-    FETCH_SIZE + WRITE_SIZE counter runs (scaled by the copy-kernel
-    calibration); only for the shape those runs measured, else None."""
+    """HBM bytes per launch of the split-bf16 A passes timed under `tag` (two
+    NN passes and one TN pass per call), from the committed FETCH_SIZE +
+    WRITE_SIZE counter runs (scaled by the copy-kernel calibration); only for
+    the shape those runs measured, else None."""
     if tag != "apass_bf16x3" or (m, n, l) != (65536, 16384, 544):
         return None
+    vals = {}
     try:
         with open(TRAFFIC_PROFILE) as f:
             for line in f:
-                if line.startswith("traffic_bytes_per_launch_tn:"):
-                    return int(line.split(":")[1])
+                for kind in ("nn", "tn"):
+                    if line.startswith(f"traffic_bytes_per_launch_{kind}:"):
+                        vals[kind] = int(line.split(":")[1])
     except OSError:
-        pass
-    return None
+        return None
+    if len(vals) != 2:
+        return None
+    return int(round((2 * vals["nn"] + vals["tn"]) / 3))
 
 
 def physical_cores() -> int:
