@@ -316,8 +316,9 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--precision", default="bf16x3", choices=["bf16", "bf16x3", "fp64"],
-                    help="bf16x3 (default) meets the 1e-5 error target; bf16 is faster, ~4e-5")
+    ap.add_argument("--precision", default="bf16x3", choices=["bf16", "bf16x3", "fp8", "fp64"],
+                    help="bf16x3 (default) meets the 1e-5 error target; bf16 (~4e-5) and fp8 "
+                         "e4m3 (per-128-column scales) are faster and coarser")
     ap.add_argument("--m", type=int, default=65536)
     ap.add_argument("--n", type=int, default=16384)
     ap.add_argument("--rank", type=int, default=2048)
@@ -397,7 +398,7 @@ def main() -> int:
         s64 = S64.to_host()[:, 0]
 
     # ---- mixed precision ----
-    prec_id = {"bf16": 1, "bf16x3": 2}.get(args.precision)
+    prec_id = {"bf16": 1, "bf16x3": 2, "fp8": 3}.get(args.precision)
     if prec_id:
         Um, Sm, Vm = (rp.DeviceMatrix(m, k, np.float32), rp.DeviceMatrix(k, 1),
                       rp.DeviceMatrix(n, k, np.float32))
@@ -407,7 +408,7 @@ def main() -> int:
                                           Um.ptr, m, Sm.ptr, Vm.ptr, n):
                 raise SystemExit("rsvd_dev_fixed_rank_mp: " + rp.last_error())
 
-        tag = "apass_bf16" if prec_id == 1 else "apass_bf16x3"
+        tag = {1: "apass_bf16", 2: "apass_bf16x3", 3: "apass_fp8"}[prec_id]
         ms, ks = _timed(lib, rp, stepmp, args.steps, args.warmup,
                         [tag], ["proj_bf16x3", "symeig", "chol_inv", "gram_bf16x3",
                                 "orth_bf16x3", "uv_bf16x3"])
@@ -444,8 +445,9 @@ def main() -> int:
                     "hbm_gbs": round(4.0 * m * n / (avg * 1e-3) / 1e9, 1),
                     "algorithmic_tflops": round(2.0 * m * n * l / (avg * 1e-3) / 1e12, 1)}
     else:
-        # fp32 A streamed and rounded in-kernel: 4 m n bytes per pass, arithmetic
-        # intensity 2 l / 4 = 272 flop/B, below the bf16 ridge (2.5e15 / 8e12)
+        # fp32 A streamed and rounded (bf16) or scaled and quantised (fp8 e4m3)
+        # in-kernel: 4 m n bytes per pass, arithmetic intensity 2 l / 4 = 272
+        # flop/B, below the bf16 (2.5e15 / 8e12) and fp8 (5e15 / 8e12) ridges
         gbs = 4.0 * m * n / (avg * 1e-3) / 1e9
         roofline = {"bound": "hbm", "achieved": round(gbs, 1), "peak": 8000.0, "unit": "GB/s",
                     "mfma_tflops": round(flops / (avg * 1e-3) / 1e12, 1)}
@@ -495,7 +497,8 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"bf16": "bf16", "bf16x3": "bf16x3", "fp64": "f64"}[args.precision],
+        "dtype": {"bf16": "bf16", "bf16x3": "bf16x3", "fp8": "fp8e4m3",
+                  "fp64": "f64"}[args.precision],
         "data": "synthetic",
         "config": {"workload": f"fixed-rank rSVD {m}x{n} fp32, rank-{r} + N(0,1e-4^2) noise, "
                                f"k={k} p={p} q={q} s=1 vnum=2",

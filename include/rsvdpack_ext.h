@@ -64,7 +64,9 @@ int rsvd_dev_fixed_rank(const double *dA, long long m, long long n, long long ld
 
 /* Mixed-precision fixed-rank rSVD of a float32 device matrix A (m x n, lda).
  * precision 1: sketch and power-iteration passes on the bf16 matrix cores
- * (fp32 accumulation); 2: the same passes in split-bf16 (fp32 accuracy).
+ * (fp32 accumulation); 2: the same passes in split-bf16 (fp32 accuracy); 3:
+ * the same passes in fp8 e4m3 (A scaled per 128 columns, the skinny operand
+ * per column; fp32 accumulation).
  * Always: CholeskyQR with FP64 Cholesky, split-bf16 projection B^T = A^T Q,
  * FP64 BB^T and eigen/SVD solve.  Omega: dOmega (n x (k+p) float, ld n) or the
  * seeded stream rounded to float.  Outputs: dU m x k float (ldu), dS k doubles
@@ -101,7 +103,9 @@ int rsvd_dev_blockrand(const double *dA, long long m, long long n, long long lda
 /* C = alpha*op(A)*op(B) + beta*C on device matrices; ta / tb = 'N' or 'T'.
  * dtype 0: double; 1: float (fp32 matrix cores); 2: float storage, products on
  * the bf16 matrix cores with fp32 accumulation; 3: float storage, split-bf16
- * (hi*hi + hi*lo + lo*hi, about fp32 accuracy). */
+ * (hi*hi + hi*lo + lo*hi, about fp32 accuracy); 4: float storage, op(A) in fp8
+ * e4m3 with one scale per 128 columns of A and B in e4m3 with one scale per
+ * column, fp8 matrix cores, fp32 accumulation (tb = 'N' only). */
 int rsvd_dev_gemm(int dtype, char ta, char tb, long long M, long long N, long long K, double alpha,
                   const void *A, long long lda, const void *B, long long ldb, double beta, void *C,
                   long long ldc);

@@ -457,6 +457,206 @@ __global__ void apass2_reduce_kernel(int64_t M, int64_t N, int ks, const float* 
   }
 }
 
+// ---------------------------------------------------------------- fp8
+// FP8 (e4m3) A passes with per-128-column scales.  a_c = max |A| over the
+// 128-column chunk c of A, / 448 (the largest e4m3 value):
+//   * op(A) = A (Y = A S, the chunk runs along K): A is stored as A / a_c and
+//     a_c is folded into the skinny operand, S'[k, :] = a_{k/128} S[k, :];
+//   * op(A) = A^T (Z = A^T S, the chunk runs along the output rows): A / a_c,
+//     and output row j is multiplied by a_{j/128} in the epilogue.
+// The skinny operand is quantised with one scale per output column, b_n =
+// max_k |S'[k, n]| / 448, applied in the epilogue.  Products on
+// v_mfma_f32_16x16x32_fp8_fp8 (fragments: 8 e4m3 values per lane along k,
+// the layout of the bf16 16x16x32 instruction), fp32 accumulation.
+typedef long fp8x8_t;
+
+__device__ __forceinline__ fp8x8_t pack_fp8x8(const float (&v)[8], float scale) {
+  int lo = __builtin_amdgcn_cvt_pk_fp8_f32(v[0] * scale, v[1] * scale, 0, false);
+  lo = __builtin_amdgcn_cvt_pk_fp8_f32(v[2] * scale, v[3] * scale, lo, true);
+  int hi = __builtin_amdgcn_cvt_pk_fp8_f32(v[4] * scale, v[5] * scale, 0, false);
+  hi = __builtin_amdgcn_cvt_pk_fp8_f32(v[6] * scale, v[7] * scale, hi, true);
+  return (fp8x8_t)(((unsigned long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
+// scale_bits[c] = bits of max |A| over columns 128 c .. 128 c + 127 of A
+// (M x N, ld lda); non-negative floats order like their bit patterns.
+__global__ void chunk_absmax_kernel(int64_t M, int64_t N, const float* __restrict__ A,
+                                    int64_t lda, unsigned* __restrict__ scale_bits) {
+  const int64_t c = blockIdx.x;
+  const int64_t j0 = c * 128, j1 = (j0 + 128 < N) ? j0 + 128 : N;
+  const int64_t rs = (M + gridDim.y - 1) / gridDim.y;
+  const int64_t r0 = (int64_t)blockIdx.y * rs, r1 = (r0 + rs < M) ? r0 + rs : M;
+  float mx = 0.0f;
+  for (int64_t j = j0; j < j1; ++j)
+    for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) mx = fmaxf(mx, fabsf(A[r + j * lda]));
+  for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
+  if ((threadIdx.x & 63) == 0) atomicMax(scale_bits + c, __float_as_uint(mx));
+}
+
+// scale_bits -> a_c = max / 448 (1 for an all-zero chunk)
+__global__ void chunk_scale_finish_kernel(int64_t nc, const unsigned* __restrict__ bits,
+                                          float* __restrict__ scale) {
+  const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (c < nc) {
+    const float mx = __uint_as_float(bits[c]);
+    scale[c] = (mx > 0.0f) ? mx / 448.0f : 1.0f;
+  }
+}
+
+// b[n] = max_k |S[k, n] a_{k/128}| / 448 (a optional), one workgroup per column
+__global__ void skinny_colscale_kernel(int64_t K, const float* __restrict__ S, int64_t lds,
+                                       const float* __restrict__ ascale, float* __restrict__ b) {
+  __shared__ float red[4];
+  const int64_t n = blockIdx.x;
+  float mx = 0.0f;
+  for (int64_t k = threadIdx.x; k < K; k += 256) {
+    const float v = S[k + n * lds] * (ascale ? ascale[k >> 7] : 1.0f);
+    mx = fmaxf(mx, fabsf(v));
+  }
+  for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    mx = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+    b[n] = (mx > 0.0f) ? mx / 448.0f : 1.0f;
+  }
+}
+
+// St8[(kb * NTP + nt) * 64 + lane] = e4m3 of S(32 kb + 8 (lane / 16) + e,
+// 16 nt + lane % 16) * a_{k/128} / b_n (a optional), zero outside K x N.
+__global__ void tile_skinny8_kernel(int64_t K, int64_t N, const float* __restrict__ S,
+                                    int64_t lds, int64_t KB, int64_t NTP,
+                                    const float* __restrict__ ascale,
+                                    const float* __restrict__ bscale, fp8x8_t* __restrict__ St) {
+  const int64_t total = KB * NTP * 64;
+  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int lane = (int)(idx & 63);
+    const int64_t t = idx >> 6;
+    const int64_t nt = t % NTP, kb = t / NTP;
+    const int64_t n = nt * 16 + (lane & 15);
+    const int64_t k0 = kb * 32 + 8 * (lane >> 4);
+    float v[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int64_t k = k0 + e;
+      v[e] = (n < N && k < K) ? S[k + n * lds] * (ascale ? ascale[k >> 7] : 1.0f) : 0.0f;
+    }
+    St[t * 64 + lane] = pack_fp8x8(v, (n < N) ? 1.0f / bscale[n] : 0.0f);
+  }
+}
+
+// The 64 x 576 A-streaming tile of apass_kernel with e4m3 fragments: A is
+// quantised (times 1 / a_c) while staged into LDS, three stages; the skinny
+// operand's fragments come from L2, double-buffered in registers.
+template <bool TA, bool VEC>
+__global__ void __launch_bounds__(kThreads, 1)
+apass8_kernel(int64_t M, int64_t N, int64_t K, float alpha, const float* __restrict__ A,
+              int64_t lda, const fp8x8_t* __restrict__ St, int64_t KB, int64_t NTP,
+              const float* __restrict__ ascale, const float* __restrict__ bscale, float beta,
+              float* __restrict__ C, int64_t ldc) {
+  constexpr int MT = kBM / 16;
+  __shared__ fp8x8_t As[3][MT][64];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int64_t m0 = (int64_t)blockIdx.x * kBM;
+  const int64_t ntb = (int64_t)blockIdx.y * kBNT + wave * kNTW;
+  const int r = TA ? (tid >> 2) : (tid & 63);
+  const int g = TA ? (tid & 3) : (tid >> 6);
+  const int64_t gm = (m0 + r < M) ? m0 + r : M - 1;
+  const float* Arow = TA ? A + gm * lda + 8 * g : A + gm + (int64_t)(8 * g) * lda;
+  // op(A) = A^T: this thread's row of op(A) is column gm of A, one scale
+  const float rinv_row = TA ? 1.0f / ascale[gm >> 7] : 1.0f;
+  const int64_t last = KB - 1;
+  float av[8];
+  auto loadA = [&](int64_t kb) {
+    kb = kb < last ? kb : last;
+    if (TA) {
+      const float* p = Arow + kb * 32;
+      if (VEC) {
+        const f4s_t x = *reinterpret_cast<const f4s_t*>(p);
+        const f4s_t y = *reinterpret_cast<const f4s_t*>(p + 4);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          av[e] = x[e];
+          av[4 + e] = y[e];
+        }
+      } else {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) av[e] = p[e];
+      }
+    } else {
+      const float* p = Arow + kb * 32 * lda;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) av[e] = p[e * lda];
+    }
+  };
+  auto storeA = [&](int64_t kb) {
+    // op(A) = A: the k block lies in one 128-column chunk of A
+    const float sc = TA ? rinv_row : 1.0f / ascale[(kb < last ? kb : last) >> 2];
+    As[kb % 3][r >> 4][(r & 15) + 16 * g] = pack_fp8x8(av, sc);
+  };
+  const fp8x8_t* Sp = St + ntb * 64 + lane;
+  auto loadS = [&](int64_t kb, fp8x8_t* h) {
+    kb = kb < last ? kb : last;
+#pragma unroll
+    for (int j = 0; j < kNTW; ++j) h[j] = Sp[(kb * NTP + j) * 64];
+  };
+  f4v_t acc[kNTW][MT];
+#pragma unroll
+  for (int j = 0; j < kNTW; ++j)
+#pragma unroll
+    for (int i = 0; i < MT; ++i) acc[j][i] = f4v_t{0.f, 0.f, 0.f, 0.f};
+  auto step = [&](int64_t kb, fp8x8_t* sc, fp8x8_t* ac, fp8x8_t* sn, fp8x8_t* an) {
+    loadS(kb + 1, sn);
+#pragma unroll
+    for (int i = 0; i < MT; ++i) an[i] = As[(kb + 1) % 3][i][lane];
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int j = 0; j < kNTW; ++j)
+#pragma unroll
+      for (int i = 0; i < MT; ++i)
+        acc[j][i] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(sc[j], ac[i], acc[j][i], 0, 0, 0);
+    __builtin_amdgcn_sched_barrier(0);
+    storeA(kb + 2);
+    loadA(kb + 3);
+    __syncthreads();
+  };
+  fp8x8_t s0[kNTW], s1[kNTW], a0[MT], a1[MT];
+  loadA(0);
+  storeA(0);
+  loadA(1);
+  storeA(1);
+  loadA(2);
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < MT; ++i) a0[i] = As[0][i][lane];
+  loadS(0, s0);
+  int64_t kb = 0;
+  for (; kb + 1 < KB; kb += 2) {
+    step(kb, s0, a0, s1, a1);
+    step(kb + 1, s1, a1, s0, a0);
+  }
+  if (kb < KB) step(kb, s0, a0, s1, a1);
+#pragma unroll
+  for (int j = 0; j < kNTW; ++j) {
+#pragma unroll
+    for (int i = 0; i < MT; ++i) {
+      const int64_t m = m0 + i * 16 + (lane & 15);
+      const float rs = (TA && m < M) ? ascale[m >> 7] : 1.0f;
+#pragma unroll
+      for (int rr = 0; rr < 4; ++rr) {
+        const int64_t n = (ntb + j) * 16 + 4 * (lane >> 4) + rr;
+        if (m < M && n < N) {
+          float* cp = C + m + n * ldc;
+          float v = alpha * acc[j][i][rr] * bscale[n] * rs;
+          if (beta != 0.0f) v += beta * (*cp);
+          *cp = v;
+        }
+      }
+    }
+  }
+}
+
 }  // namespace
 
 bool apass_suitable(Context* c, int64_t M, int64_t N) {
@@ -568,6 +768,84 @@ int apass_bf16(Context* c, const char* tag, bool ta, int64_t M, int64_t N, int64
     const float* Ar = ta ? A + K32 : A + K32 * lda;
     RSVD_TRY(gemm_bf16_general(c, tag, ta ? 'T' : 'N', 'N', M, N, K - K32, alpha, Ar, lda,
                                S + K32, lds, 1.0f, C, ldc, split));
+  }
+  return 0;
+}
+
+int fp8_column_scales(Context* c, int64_t M, int64_t N, const float* A, int64_t lda, float* scale) {
+  const int64_t nc = (N + 127) / 128;
+  if (nc == 0) return 0;
+  DBuf<unsigned> bits;
+  RSVD_TRY(bits.alloc((size_t)nc));
+  RSVD_HIP(hipMemsetAsync(bits.p, 0, sizeof(unsigned) * nc, c->stream));
+  int64_t gy = (M + 8191) / 8192;
+  if (gy > 64) gy = 64;
+  if (gy < 1) gy = 1;
+  hipLaunchKernelGGL(chunk_absmax_kernel, dim3((unsigned)nc, (unsigned)gy), dim3(256), 0,
+                     c->stream, M, N, A, lda, bits.p);
+  RSVD_LAUNCH_CHECK();
+  hipLaunchKernelGGL(chunk_scale_finish_kernel, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0,
+                     c->stream, nc, (const unsigned*)bits.p, scale);
+  RSVD_LAUNCH_CHECK();
+  return 0;
+}
+
+int apass_fp8(Context* c, const char* tag, bool ta, int64_t M, int64_t N, int64_t K, float alpha,
+              const float* A, int64_t lda, const float* ascale, const float* S, int64_t lds,
+              float beta, float* C, int64_t ldc) {
+  if (M < 0 || N < 0 || K < 0) return set_error("apass_fp8: negative dimension");
+  if (!ascale) return set_error("apass_fp8: column scales of A required");
+  if (M == 0 || N == 0) return 0;
+  const int64_t K32 = K - K % 32;
+  if (K32 == 0)
+    return gemm_bf16_general(c, tag, ta ? 'T' : 'N', 'N', M, N, K, alpha, A, lda, S, lds, beta, C,
+                             ldc, 3);
+  const int64_t KB = K32 / 32;
+  const int64_t gx = (M + kBM - 1) / kBM;
+  const int64_t gy = ((N + 15) / 16 + kBNT - 1) / kBNT;
+  if (gy > 65535 || gx > 0x7fffffff) return set_error("apass_fp8: grid too large");
+  const int64_t NTP = gy * kBNT;
+  DBuf<float> b;
+  DBuf<fp8x8_t> St;
+  RSVD_TRY(b.alloc((size_t)N));
+  RSVD_TRY(St.alloc((size_t)(KB * NTP * 64)));
+  const float* afold = ta ? nullptr : ascale;  // op(A) = A: a_c folded into S
+  hipLaunchKernelGGL(skinny_colscale_kernel, dim3((unsigned)N), dim3(256), 0, c->stream, K32, S,
+                     lds, afold, b.p);
+  RSVD_LAUNCH_CHECK();
+  {
+    const int64_t total = KB * NTP * 64;
+    int64_t blocks = (total + 255) / 256;
+    if (blocks > 16384) blocks = 16384;
+    hipLaunchKernelGGL(tile_skinny8_kernel, dim3((unsigned)blocks), dim3(256), 0, c->stream, K32,
+                       N, S, lds, KB, NTP, afold, (const float*)b.p, St.p);
+    RSVD_LAUNCH_CHECK();
+  }
+  const bool vec = ta && (reinterpret_cast<uintptr_t>(A) % 16 == 0) && (lda % 4 == 0);
+  {
+    KTimer kt(c, tag);
+    dim3 grid((unsigned)gx, (unsigned)gy);
+    if (ta) {
+      if (vec)
+        hipLaunchKernelGGL((apass8_kernel<true, true>), grid, dim3(kThreads), 0, c->stream, M, N,
+                           K32, alpha, A, lda, (const fp8x8_t*)St.p, KB, NTP, ascale,
+                           (const float*)b.p, beta, C, ldc);
+      else
+        hipLaunchKernelGGL((apass8_kernel<true, false>), grid, dim3(kThreads), 0, c->stream, M, N,
+                           K32, alpha, A, lda, (const fp8x8_t*)St.p, KB, NTP, ascale,
+                           (const float*)b.p, beta, C, ldc);
+    } else {
+      hipLaunchKernelGGL((apass8_kernel<false, false>), grid, dim3(kThreads), 0, c->stream, M, N,
+                         K32, alpha, A, lda, (const fp8x8_t*)St.p, KB, NTP, ascale,
+                         (const float*)b.p, beta, C, ldc);
+    }
+    RSVD_LAUNCH_CHECK();
+  }
+  if (K32 < K) {
+    // the last K % 32 columns of op(A) in split-bf16: C += alpha op(A)(:, K32:) S(K32:, :)
+    const float* Ar = ta ? A + K32 : A + K32 * lda;
+    RSVD_TRY(gemm_bf16_general(c, tag, ta ? 'T' : 'N', 'N', M, N, K - K32, alpha, Ar, lda,
+                               S + K32, lds, 1.0f, C, ldc, 3));
   }
   return 0;
 }

@@ -807,6 +807,16 @@ int rsvd_dev_gemm(int dtype, char ta, char tb, long long M, long long N, long lo
   if (dtype == 2 || dtype == 3)
     return gemm_bf16(api.c, "gemm_bf16", ta, tb, M, N, K, (float)alpha, (const float*)A, lda,
                      (const float*)B, ldb, (float)beta, (float*)C, ldc, dtype == 2 ? 1 : 3);
+  if (dtype == 4) {
+    if (tb != 'N' && tb != 'n') return set_error("rsvd_dev_gemm: fp8 needs tb = 'N'");
+    const bool t = (ta == 'T' || ta == 't');
+    const int64_t ar = t ? K : M, ac = t ? M : K;  // A is ar x ac
+    DBuf<float> sc;
+    RSVD_TRY(sc.alloc((size_t)((ac + 127) / 128 + 1)));
+    RSVD_TRY(fp8_column_scales(api.c, ar, ac, (const float*)A, lda, sc.p));
+    return apass_fp8(api.c, "gemm_fp8", t, M, N, K, (float)alpha, (const float*)A, lda, sc.p,
+                     (const float*)B, ldb, (float)beta, (float*)C, ldc);
+  }
   return set_error("rsvd_dev_gemm: dtype %d not supported", dtype);
 }
 

@@ -5,8 +5,10 @@
 
 namespace rsvd {
 
-// Precision of the passes over A (the projection B^T = A^T Q is always split-bf16).
-enum MpPrecision { MP_BF16 = 1, MP_BF16X3 = 2 };
+// Precision of the sketch and power-iteration passes over A (the projection
+// B^T = A^T Q is always split-bf16): bf16, split-bf16, or fp8 e4m3 with
+// per-128-column scales.
+enum MpPrecision { MP_BF16 = 1, MP_BF16X3 = 2, MP_FP8 = 3 };
 
 struct MpInfo {
   int qr_fallbacks = 0;

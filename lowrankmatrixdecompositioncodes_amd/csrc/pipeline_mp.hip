@@ -3,8 +3,9 @@
 // Same control flow as low_rank_svd_rand_decomp_fixed_rank (MKL/rra.c:73-234)
 // with the precision of each step chosen by its sensitivity:
 //   * sketch and power-iteration passes over A (Y = A Omega, Z = A^T Y,
-//     Y = A Z): bf16 matrix cores, fp32 accumulation (prec = MP_BF16), or
-//     split-bf16 (prec = MP_BF16X3, fp32-level accuracy);
+//     Y = A Z): bf16 matrix cores, fp32 accumulation (prec = MP_BF16),
+//     split-bf16 (prec = MP_BF16X3, fp32-level accuracy), or fp8 e4m3 with
+//     per-128-column scales of A (prec = MP_FP8);
 //   * orthonormalisation: CholeskyQR with the Gram in fp32 (split-bf16
 //     products) and the Cholesky / triangular inverse in FP64; one pass inside
 //     the power iteration (only the range matters there), two passes (CQR2)
@@ -81,10 +82,24 @@ int fixed_rank_mp(Context* c, const float* A, int64_t m, int64_t n, int64_t lda,
   if (k <= 0 || p < 0) return set_error("fixed_rank_mp: need k > 0 and p >= 0");
   if (l > (m < n ? m : n)) return set_error("fixed_rank_mp: k+p exceeds min(m,n)");
   if (s <= 0) return set_error("fixed_rank_mp: s must be positive");
-  if (prec != MP_BF16 && prec != MP_BF16X3)
+  if (prec != MP_BF16 && prec != MP_BF16X3 && prec != MP_FP8)
     return set_error("fixed_rank_mp: unknown precision %d", prec);
   const int split = (prec == MP_BF16X3) ? 3 : 1;
-  const char* tag = (split == 3) ? "apass_bf16x3" : "apass_bf16";
+  const char* tag = (prec == MP_FP8) ? "apass_fp8" : (split == 3) ? "apass_bf16x3" : "apass_bf16";
+  // fp8: the per-128-column scales of A, once per call
+  DBuf<float> ascale;
+  if (prec == MP_FP8) {
+    RSVD_TRY(ascale.alloc((size_t)((n + 127) / 128)));
+    RSVD_TRY(fp8_column_scales(c, m, n, A, lda, ascale.p));
+  }
+  // one sketch / power pass: C = op(A) S
+  auto pass = [&](bool ta, int64_t M, int64_t K, const float* S, int64_t lds_, float* C,
+                  int64_t ldc) -> int {
+    if (prec == MP_FP8)
+      return apass_fp8(c, tag, ta, M, l, K, 1.0f, A, lda, ascale.p, S, lds_, 0.0f, C, ldc);
+    return gemm_bf16(c, tag, ta ? 'T' : 'N', 'N', M, l, K, 1.0f, A, lda, S, lds_, 0.0f, C, ldc,
+                     split);
+  };
   DBuf<float> Om, Y, Z;
   const float* Omp = dOmega;
   if (!Omp) {
@@ -95,12 +110,12 @@ int fixed_rank_mp(Context* c, const float* A, int64_t m, int64_t n, int64_t lda,
   RSVD_TRY(Y.alloc((size_t)(m * l)));
   RSVD_TRY(Z.alloc((size_t)(n * l)));
   int fb = 0;
-  RSVD_TRY(gemm_bf16(c, tag, 'N', 'N', m, l, n, 1.0f, A, lda, Omp, n, 0.0f, Y.p, m, split));
+  RSVD_TRY(pass(false, m, n, Omp, n, Y.p, m));
   for (int j = 1; j < q; ++j) {                                        // rra.c:101-126
     if ((2 * j - 2) % s == 0) RSVD_TRY(orth_f32(c, m, l, Y.p, m, 1, 0, &fb));
-    RSVD_TRY(gemm_bf16(c, tag, 'T', 'N', n, l, m, 1.0f, A, lda, Y.p, m, 0.0f, Z.p, n, split));
+    RSVD_TRY(pass(true, n, m, Y.p, m, Z.p, n));
     if ((2 * j - 1) % s == 0) RSVD_TRY(orth_f32(c, n, l, Z.p, n, 1, 0, &fb));
-    RSVD_TRY(gemm_bf16(c, tag, 'N', 'N', m, l, n, 1.0f, A, lda, Z.p, n, 0.0f, Y.p, m, split));
+    RSVD_TRY(pass(false, m, n, Z.p, n, Y.p, m));
   }
   RSVD_TRY(orth_f32(c, m, l, Y.p, m, 2, 1, &fb));                      // rra.c:129-130
   // B^T = A^T Q (n x l), split-bf16

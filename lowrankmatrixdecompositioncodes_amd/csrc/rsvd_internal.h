@@ -143,6 +143,16 @@ int gemm_f32(Context* c, char ta, char tb, int64_t M, int64_t N, int64_t K, floa
 int apass_bf16(Context* c, const char* tag, bool ta, int64_t M, int64_t N, int64_t K, float alpha,
                const float* A, int64_t lda, const float* S, int64_t lds, float beta, float* C,
                int64_t ldc, int split);
+// a_c = max |A| over the 128-column chunk c of A (M x N, ld lda) / 448, into
+// scale[0 .. ceil(N / 128)) (1 for an all-zero chunk): the fp8 e4m3 scales.
+int fp8_column_scales(Context* c, int64_t M, int64_t N, const float* A, int64_t lda, float* scale);
+// C = alpha op(A) S + beta C with A quantised to e4m3 per 128-column chunk of A
+// (ascale from fp8_column_scales of A) and S to e4m3 per output column, on
+// the fp8 matrix cores with fp32 accumulation; a K % 32 remainder in
+// split-bf16.
+int apass_fp8(Context* c, const char* tag, bool ta, int64_t M, int64_t N, int64_t K, float alpha,
+              const float* A, int64_t lda, const float* ascale, const float* S, int64_t lds,
+              float beta, float* C, int64_t ldc);
 // true when apass_bf16 has enough row blocks to fill the GPU.
 bool apass_suitable(Context* c, int64_t M, int64_t N);
 // The tiled bf16 GEMM without the A-streaming dispatch (any shape).
