@@ -212,11 +212,13 @@ def bench_batched(args, rp, lib) -> int:
 
 def bench_blocked(args, rp, lib) -> int:
     """low_rank_svd_blockrand_decomp_fixed_rank_or_prec in TOL mode on A =
-    U diag(sigma) V^T, 100000 x 20000 (held as float and its exact double
-    widening), sigma_j = 1 for j < 300, exp(-(j - 300) / 150) after, Haar
-    factors; the 1e-8 floor of the spectrum's tail is dropped (its energy,
-    below 2e-12 of ||A||_F^2, is far under TOL^2); kstep = 256, q = 1, s = 1,
-    TOL = 1e-6 ||A||_F.  FP64 device path (the plain method)."""
+    U diag(sigma) V^T, 100000 x 20000 float32 (and its exact double widening),
+    sigma_j = 1 for j < 300, exp(-(j - 300) / 150) after, Haar factors; the
+    1e-8 floor of the spectrum's tail is dropped (its energy, below 2e-12 of
+    ||A||_F^2, is far under TOL^2); kstep = 256, q = 1, s = 1, TOL = 1e-6
+    ||A||_F.  --precision fp64: the FP64 device path (the plain method) is
+    the measured one; otherwise the mixed path (float32 A and deflation, FP64
+    TOL norm) is measured and the FP64 path runs --fp64-steps times beside it."""
     import ctypes as C
     log = lambda s: print("[bench] " + s, file=sys.stderr, flush=True)  # noqa: E731
     m, n = args.m_blocked, args.n_blocked
@@ -224,44 +226,112 @@ def bench_blocked(args, rp, lib) -> int:
     sigma = np.where(j < 300, 1.0, np.exp(-(j - 300) / 150.0))
     sigma = sigma[sigma >= 1e-8]
     r = len(sigma)
+    prec_id = {"bf16": 1, "bf16x3": 2, "fp8": 3}.get(args.precision)
     log(f"generating {m}x{n} rank-{r} matrix")
     A64 = rp.DeviceMatrix(m, n)
-    if lib.rsvd_dev_lowrank_test_matrix(m, n, r, sigma.ctypes.data, 0.0, 31337, A64.ptr, m, None,
-                                        0):
+    A32 = rp.DeviceMatrix(m, n, np.float32) if prec_id else None
+    if lib.rsvd_dev_lowrank_test_matrix(m, n, r, sigma.ctypes.data, 0.0, 31337, A64.ptr, m,
+                                        A32.ptr if A32 else None, m):
         raise SystemExit("test matrix: " + rp.last_error())
     anorm = float(np.sqrt(np.sum(sigma ** 2)))
     tol = 1e-6 * anorm
     kstep, q, p = 256, 1, 10
-    times, fr_last, sig_last = [], 0, None
-    for it in range(args.warmup + args.steps):
-        fr = C.c_int(0)
-        u, sv, v = C.c_void_p(), C.c_void_p(), C.c_void_p()
-        lib.rsvd_sync()
-        t0 = time.perf_counter()
-        if lib.rsvd_dev_blockrand(A64.ptr, m, n, m, -1, p, tol, 2, kstep, q, 1, 99, None,
-                                  C.byref(fr), C.byref(u), C.byref(sv), C.byref(v)):
-            raise SystemExit("blockrand: " + rp.last_error())
-        dt = time.perf_counter() - t0
-        if it >= args.warmup:
-            times.append(dt)
-        fr_last = fr.value
-        sig_last = np.empty(fr_last)
-        lib.rsvd_memcpy_d2h(sig_last.ctypes.data, sv.value, 8 * fr_last)
-        for ptr in (u, sv, v):
+
+    def run(mp, reps, warm):
+        times, out = [], None
+        for it in range(warm + reps):
+            fr = C.c_int(0)
+            u, sv, v = C.c_void_p(), C.c_void_p(), C.c_void_p()
+            lib.rsvd_sync()
+            t0 = time.perf_counter()
+            if mp:
+                rc = lib.rsvd_dev_blockrand_mp(A32.ptr, m, n, m, -1, p, tol, 2, kstep, q, 1, 99,
+                                               None, prec_id, C.byref(fr), C.byref(u),
+                                               C.byref(sv), C.byref(v))
+            else:
+                rc = lib.rsvd_dev_blockrand(A64.ptr, m, n, m, -1, p, tol, 2, kstep, q, 1, 99, None,
+                                            C.byref(fr), C.byref(u), C.byref(sv), C.byref(v))
+            if rc:
+                raise SystemExit("blockrand: " + rp.last_error())
+            dt = time.perf_counter() - t0
+            if it >= warm:
+                times.append(dt)
+            f = fr.value
+            s_ = np.empty(f)
+            lib.rsvd_memcpy_d2h(s_.ctypes.data, sv.value, 8 * f)
+            if out is not None:
+                for ptr in out[2:]:
+                    lib.rsvd_dev_free(ptr.value)
+            out = (f, s_, u, sv, v)
+            log(f"blockrand {'mixed ' + args.precision if mp else 'FP64'} call {it}: "
+                f"{dt * 1e3:.1f} ms, frank {f}")
+        return 1e3 * float(np.median(times)), out
+
+    def residual(out, f32):
+        f, _, u, sv, v = out
+        res = C.c_double(0)
+        if lib.rsvd_dev_lowrank_residual(A64.ptr, m, n, m, f, u.value, m, sv.value, v.value, n,
+                                         1 if f32 else 0, C.byref(res)):
+            raise SystemExit("residual: " + rp.last_error())
+        for ptr in out[2:]:
             lib.rsvd_dev_free(ptr.value)
-        log(f"blockrand call {it}: {dt * 1e3:.1f} ms, frank {fr_last}")
-    ms = 1e3 * float(np.median(times))
-    resid = float(np.sqrt(max(anorm ** 2 - np.sum(sig_last ** 2), 0.0)))
-    steps = int(np.ceil(fr_last / kstep)) if fr_last else 0
+        return res.value / anorm
+
+    ms64 = out64 = None
+    n64 = args.steps if not prec_id else args.fp64_steps
+    if n64 > 0:
+        ms64, out64 = run(False, n64, args.warmup if not prec_id else 0)
+    breakdown = None
+    if prec_id:
+        ms, out = run(True, args.steps, args.warmup)
+        frank, smp = out[0], out[1]
+        res_mp = residual(out, True)
+        # per-family kernel time of one more call (HIP events around each launch)
+        fams = ["apass_bf16x3", "apass_bf16", "apass_fp8", "proj_bf16x3", "deflate_bf16x6",
+                "reorth_bf16x6", "gram_bf16x3", "orth_bf16x3", "uv_bf16x6", "gemm_f64",
+                "chol_inv", "symeig"]
+        lib.rsvd_kstats_reset()
+        lib.rsvd_kstats_select(None)
+        lib.rsvd_kstats_enable(1)
+        _, extra = run(True, 1, 0)
+        lib.rsvd_kstats_enable(0)
+        for ptr in extra[2:]:
+            lib.rsvd_dev_free(ptr.value)
+        breakdown = {}
+        for t in fams:
+            tot, cnt = rp.kstats(t)
+            if cnt:
+                breakdown[t] = round(tot, 2)
+    else:
+        ms, frank, res_mp = ms64, out64[0], None
+    res64 = residual(out64, False) if out64 is not None else None
+    err = None
+    if prec_id and out64 is not None:
+        s64 = np.sort(out64[1])[::-1]
+        sm = np.sort(smp)[::-1]
+        kk = min(len(s64), len(sm))
+        err = {"max_abs_sigma_diff_over_sigma1": float(np.abs(sm[:kk] - s64[:kk]).max() / sigma[0]),
+               "frank_fp64": out64[0]}
     line = {
         "metric": "blocked fixed-precision QB + rSVD (blockrand TOL mode), seconds per solve",
         "value": round(ms / 1e3, 4), "unit": "s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": False,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": {"bf16": "bf16", "bf16x3": "bf16x3", "fp8": "fp8e4m3",
+                  "fp64": "f64"}[args.precision],
+        "data": "synthetic",
         "config": {"workload": f"blockrand TOL mode {m}x{n}, plateau 300 + exp decay /150, "
-                               f"kstep=256 q=1 s=1 TOL=1e-6 rel", "m": m, "n": n},
-        "frank": fr_last, "residual_rel": resid / anorm, "tol_rel": 1e-6,
-        "note": "frank is the reference's reported rank (shrunk by its rra.c:278 rule)",
+                               f"kstep=256 q=1 s=1 TOL=1e-6 rel", "m": m, "n": n,
+                   "precision": args.precision},
+        "frank": frank,
+        "residual_rel": res_mp if prec_id else res64, "tol_rel": 1e-6,
+        "fp64_plain": None if ms64 is None else {"seconds": round(ms64 / 1e3, 4),
+                                                 "residual_rel": res64},
+        "speedup_vs_fp64_plain": None if (ms64 is None or not prec_id) else round(ms64 / ms, 3),
+        "error_vs_fp64": err,
+        "kernel_ms_per_call": breakdown,
+        "note": "frank is the reference's reported rank (shrunk by its rra.c:278 rule); "
+                "residual_rel = ||A - U S V^T||_F / ||A||_F in FP64 against the double A",
     }
     print(json.dumps(line))
     return 0

@@ -64,7 +64,8 @@ int rsvd_dev_fixed_rank(const double *dA, long long m, long long n, long long ld
 
 /* Mixed-precision fixed-rank rSVD of a float32 device matrix A (m x n, lda).
  * precision 1: sketch and power-iteration passes on the bf16 matrix cores
- * (fp32 accumulation); 2: the same passes in split-bf16 (fp32 accuracy); 3:
+ * (fp32 accumulation); 2: the same passes in split-bf16 (16-bit operands, about
+ * 1e-7 relative error on long sums); 3:
  * the same passes in fp8 e4m3 (A scaled per 128 columns, the skinny operand
  * per column; fp32 accumulation).
  * Always: CholeskyQR with FP64 Cholesky, split-bf16 projection B^T = A^T Q,
@@ -75,6 +76,36 @@ int rsvd_dev_fixed_rank_mp(const float *dA, long long m, long long n, long long 
                            int vnum, int q, int s, unsigned long long seed, const float *dOmega,
                            int precision, float *dU, long long ldu, double *dS, float *dV,
                            long long ldv);
+
+/* randQB_pb_new (is_new = 1, MKL/rra.c:1576-1801) / randQB_pb (is_new = 0,
+ * rra.c:1425-1569) on a float32 device matrix A in mixed precision (the QB of
+ * rsvd_dev_blockrand_mp below).  dQ: m x rsvd_qb_max_cols float (ldq); dBt:
+ * B^T, n x rsvd_qb_max_cols float (ldbt), i.e. B stored by rows.  *frank =
+ * columns produced; *final_norm = the FP64 Frobenius norm of the deflated
+ * float32 copy of A after the last block. */
+int rsvd_dev_randqb_mp(const float *dA, long long m, long long n, long long lda, int kstep,
+                       int nstep, double TOL, int q, int s, int is_new, unsigned long long seed,
+                       const float *dRN, int precision, float *dQ, long long ldq, float *dBt,
+                       long long ldbt, int *frank, double *final_norm);
+
+/* low_rank_svd_blockrand_decomp_fixed_rank_or_prec (MKL/rra.c:239-381, its QB
+ * randQB_pb_new rra.c:1576-1801) on a float32 device matrix A, in mixed
+ * precision: a float32 working copy of A is deflated block by block (R -= Qp
+ * Bp with fp32-exact products, Bp = Qp^T R in split-bf16 with its error
+ * cleared on the next block) with the FP64 sum of squares of R as the TOL
+ * norm; the
+ * sketch and power-iteration passes over R run at `precision` (1 bf16, 2
+ * split-bf16, 3 fp8 e4m3, as rsvd_dev_fixed_rank_mp); each Qp is
+ * orthonormalised in FP64 and re-orthogonalised against every earlier block;
+ * the final SVD as rsvd_dev_fixed_rank_mp.  k, p, TOL, vnum, kstep, q, s as
+ * rsvd_dev_blockrand.  dRN: n x (kstep*nstep) float or NULL (the seeded
+ * stream rounded to float).  Outputs allocated by the library (free with
+ * rsvd_dev_free): *dU m x frank float, *dS frank doubles, *dV n x frank
+ * float. */
+int rsvd_dev_blockrand_mp(const float *dA, long long m, long long n, long long lda, int k, int p,
+                          double TOL, int vnum, int kstep, int q, int s, unsigned long long seed,
+                          const float *dRN, int precision, int *frank, float **dU, double **dS,
+                          float **dV);
 
 /* ||A - U diag(S) V^T||_F for a double device matrix A and device factors U
  * (m x k), S (k), V (n x k) stored as double (uv_f32 = 0) or float (1). */
@@ -103,9 +134,12 @@ int rsvd_dev_blockrand(const double *dA, long long m, long long n, long long lda
 /* C = alpha*op(A)*op(B) + beta*C on device matrices; ta / tb = 'N' or 'T'.
  * dtype 0: double; 1: float (fp32 matrix cores); 2: float storage, products on
  * the bf16 matrix cores with fp32 accumulation; 3: float storage, split-bf16
- * (hi*hi + hi*lo + lo*hi, about fp32 accuracy); 4: float storage, op(A) in fp8
+ * (hi*hi + hi*lo + lo*hi of 16-bit operand splits, fp32 accumulation); 4: float
+ * storage, op(A) in fp8
  * e4m3 with one scale per 128 columns of A and B in e4m3 with one scale per
- * column, fp8 matrix cores, fp32 accumulation (tb = 'N' only). */
+ * column, fp8 matrix cores, fp32 accumulation (tb = 'N' only); 5: float
+ * storage, fp32-exact products from three bf16 planes per operand (six bf16
+ * products), fp32 accumulation. */
 int rsvd_dev_gemm(int dtype, char ta, char tb, long long M, long long N, long long K, double alpha,
                   const void *A, long long lda, const void *B, long long ldb, double beta, void *C,
                   long long ldc);

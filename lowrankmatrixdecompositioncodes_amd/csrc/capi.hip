@@ -817,6 +817,9 @@ int rsvd_dev_gemm(int dtype, char ta, char tb, long long M, long long N, long lo
     return apass_fp8(api.c, "gemm_fp8", t, M, N, K, (float)alpha, (const float*)A, lda, sc.p,
                      (const float*)B, ldb, (float)beta, (float*)C, ldc);
   }
+  if (dtype == 5)
+    return gemm_bf16_general(api.c, "gemm_bf16x6", ta, tb, M, N, K, (float)alpha, (const float*)A,
+                             lda, (const float*)B, ldb, (float)beta, (float*)C, ldc, 6);
   return set_error("rsvd_dev_gemm: dtype %d not supported", dtype);
 }
 
@@ -1043,6 +1046,51 @@ int rsvd_dev_fixed_rank_mp(const float* dA, long long m, long long n, long long 
   if (api.rc) return api.rc;
   return fixed_rank_mp(api.c, dA, m, n, lda, k, p, vnum, q, s, seed, dOmega, precision, dU, ldu,
                        dS, dV, ldv, nullptr);
+}
+
+int rsvd_dev_randqb_mp(const float* dA, long long m, long long n, long long lda, int kstep,
+                       int nstep, double TOL, int q, int s, int is_new, unsigned long long seed,
+                       const float* dRN, int precision, float* dQ, long long ldq, float* dBt,
+                       long long ldbt, int* frank, double* final_norm) {
+  Api api;
+  if (api.rc) return api.rc;
+  int fr = 0, la = 0;
+  QbInfo qi{};
+  RSVD_TRY(qb_blocked_mp(api.c, dA, m, n, lda, kstep, nstep, TOL, q, s, is_new, seed, dRN,
+                         precision, dQ, ldq, dBt, ldbt, &fr, &la, &qi));
+  if (frank) *frank = fr;
+  if (final_norm) *final_norm = qi.last_norm;
+  return stream_sync(api.c);
+}
+
+int rsvd_dev_blockrand_mp(const float* dA, long long m, long long n, long long lda, int k, int p,
+                          double TOL, int vnum, int kstep, int q, int s, unsigned long long seed,
+                          const float* dRN, int precision, int* frank, float** dU, double** dS,
+                          float** dV) {
+  Api api;
+  if (api.rc) return api.rc;
+  if (!frank || !dU || !dS || !dV) return set_error("rsvd_dev_blockrand_mp: null output pointer");
+  BlockrandMpOut out;
+  RSVD_TRY(blockrand_mp(api.c, dA, m, n, lda, k, p, TOL, vnum, kstep, q, s, seed, dRN, precision,
+                        &out));
+  const int fr = out.frank;
+  float *u = nullptr, *v = nullptr;
+  double* sv = nullptr;
+  RSVD_HIP(hipMalloc(&u, sizeof(float) * (size_t)m * fr));
+  RSVD_HIP(hipMalloc(&sv, sizeof(double) * (size_t)fr));
+  RSVD_HIP(hipMalloc(&v, sizeof(float) * (size_t)n * fr));
+  RSVD_HIP(hipMemcpyAsync(u, out.U.p, sizeof(float) * (size_t)m * fr, hipMemcpyDeviceToDevice,
+                          api.c->stream));
+  RSVD_HIP(hipMemcpyAsync(sv, out.S.p, sizeof(double) * (size_t)fr, hipMemcpyDeviceToDevice,
+                          api.c->stream));
+  RSVD_HIP(hipMemcpyAsync(v, out.V.p, sizeof(float) * (size_t)n * fr, hipMemcpyDeviceToDevice,
+                          api.c->stream));
+  RSVD_TRY(stream_sync(api.c));
+  *frank = fr;
+  *dU = u;
+  *dS = sv;
+  *dV = v;
+  return 0;
 }
 
 int rsvd_dev_lowrank_residual(const double* dA, long long m, long long n, long long lda,

@@ -15,8 +15,14 @@
 // stores of C are coalesced.
 //
 // SPLIT = 3 evaluates each product as hi*hi + hi*lo + lo*hi with x = hi + lo,
-// hi = bf16(x), lo = bf16(x - hi): about 16 significant bits per operand,
-// accumulated in fp32 -- the accuracy of an fp32 GEMM at 3 bf16 MFMAs.
+// hi = bf16(x), lo = bf16(x - hi): 16 significant bits per operand (a
+// relative error up to ~2^-16 per product, which averages out in long sums),
+// accumulated in fp32, at 3 bf16 MFMAs.  SPLIT = 6 splits each operand into
+// three bf16 planes x = hi + mid + lo, which hold all 24 bits of an fp32
+// value, and evaluates hi*hi + hi*mid + mid*hi + mid*mid + hi*lo + lo*hi (the
+// dropped terms are below 2^-24 of the product): fp32-exact products,
+// accumulated in fp32, at 6 bf16 MFMAs, with a K step of 32 so that the three
+// planes of both operands fit LDS twice over.
 //
 // Workgroup: 256 threads, 2x2 waves, tile 128 x 128, K step 64.  The grid is
 // ordered so that the tiles sharing a panel of the large operand are launched
@@ -32,8 +38,12 @@ typedef float f4v_t __attribute__((ext_vector_type(4)));
 
 namespace {
 
-constexpr int LBM = 128, LBN = 128, LBK = 64, LPAD = 8, LTHREADS = 256;
-constexpr int LROW = LBK + LPAD;  // bf16 elements per LDS row
+constexpr int LBM = 128, LBN = 128, LPAD = 8, LTHREADS = 256;
+// K step and bf16 planes per operand of a SPLIT mode
+__host__ __device__ constexpr int lp_bk(int split) { return split == 6 ? 32 : 64; }
+__host__ __device__ constexpr int lp_planes(int split) {
+  return split == 6 ? 3 : (split == 3 ? 2 : 1);
+}
 
 // Staging of a 128 x LBK operand tile into LDS rows [row][k] (bf16, k
 // contiguous), optionally as a (hi, lo) split.  Global view: element
@@ -44,16 +54,26 @@ constexpr int LROW = LBK + LPAD;  // bf16 elements per LDS row
 typedef float f4s_t __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x4_t __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ void put4(__bf16* hi, __bf16* lo, f4s_t x, bool split) {
-  bf16x4_t h, l;
+// Four consecutive k of one row into the NP planes at offset `off` of each
+// (plane stride `ps` elements): plane 0 = bf16(x), plane 1 = bf16(x - p0),
+// plane 2 = bf16(x - p0 - p1).  The differences are exact in fp32.
+template <int NP>
+__device__ __forceinline__ void put4(__bf16* base, int ps, int off, f4s_t x) {
+  bf16x4_t h, m, l;
 #pragma unroll
   for (int e = 0; e < 4; ++e) {
     const __bf16 a = (__bf16)x[e];
     h[e] = a;
-    if (split) l[e] = (__bf16)(x[e] - (float)a);
+    if (NP > 1) {
+      const float r1 = x[e] - (float)a;
+      const __bf16 b = (__bf16)r1;
+      m[e] = b;
+      if (NP > 2) l[e] = (__bf16)(r1 - (float)b);
+    }
   }
-  *reinterpret_cast<bf16x4_t*>(hi) = h;
-  if (split) *reinterpret_cast<bf16x4_t*>(lo) = l;
+  *reinterpret_cast<bf16x4_t*>(base + off) = h;
+  if (NP > 1) *reinterpret_cast<bf16x4_t*>(base + ps + off) = m;
+  if (NP > 2) *reinterpret_cast<bf16x4_t*>(base + 2 * ps + off) = l;
 }
 
 template <bool KCONT, int SPLIT>
@@ -63,6 +83,7 @@ struct Stager;
 // cover the 64 k of a row (256 contiguous bytes).  8 items per thread.
 template <int SPLIT>
 struct Stager<true, SPLIT> {
+  static constexpr int LBK = lp_bk(SPLIT), LROW = LBK + LPAD, NP = lp_planes(SPLIT);
   static constexpr int ITEMS = (LBM * LBK / 4) / LTHREADS;
   f4s_t v[ITEMS];
   __device__ __forceinline__ void load(const float* __restrict__ base, int64_t ld, int64_t row0,
@@ -83,13 +104,13 @@ struct Stager<true, SPLIT> {
       }
     }
   }
-  __device__ __forceinline__ void store(__bf16* __restrict__ hi, __bf16* __restrict__ lo,
-                                        int tid) {
+  // planes of LBM * LROW elements each, contiguous from `base`
+  __device__ __forceinline__ void store(__bf16* __restrict__ base, int tid) {
 #pragma unroll
     for (int it = 0; it < ITEMS; ++it) {
       const int f = tid + it * LTHREADS;
       const int rr = f / (LBK / 4), kq = f % (LBK / 4);
-      put4(hi + rr * LROW + kq * 4, lo + rr * LROW + kq * 4, v[it], SPLIT == 3);
+      put4<NP>(base, LBM * LROW, rr * LROW + kq * 4, v[it]);
     }
   }
 };
@@ -101,6 +122,7 @@ struct Stager<true, SPLIT> {
 // LDS writes spread over 32 bank pairs.
 template <int SPLIT>
 struct Stager<false, SPLIT> {
+  static constexpr int LBK = lp_bk(SPLIT), LROW = LBK + LPAD, NP = lp_planes(SPLIT);
   static constexpr int ITEMS = (LBM * LBK / 16) / LTHREADS;
   f4s_t v[ITEMS][4];  // [item][column c] = rows 4rq..4rq+3 at k = 4kq + c
   __device__ __forceinline__ void coords(int tid, int it, int* rq, int* kq) const {
@@ -129,8 +151,7 @@ struct Stager<false, SPLIT> {
       }
     }
   }
-  __device__ __forceinline__ void store(__bf16* __restrict__ hi, __bf16* __restrict__ lo,
-                                        int tid) {
+  __device__ __forceinline__ void store(__bf16* __restrict__ base, int tid) {
 #pragma unroll
     for (int it = 0; it < ITEMS; ++it) {
       int rq, kq;
@@ -139,7 +160,7 @@ struct Stager<false, SPLIT> {
       for (int e = 0; e < 4; ++e) {
         const f4s_t row = {v[it][0][e], v[it][1][e], v[it][2][e], v[it][3][e]};
         const int r = 4 * rq + e;
-        put4(hi + r * LROW + 4 * kq, lo + r * LROW + 4 * kq, row, SPLIT == 3);
+        put4<NP>(base, LBM * LROW, r * LROW + 4 * kq, row);
       }
     }
   }
@@ -156,8 +177,9 @@ __global__ void __launch_bounds__(LTHREADS, 2)
 gemm_bf16_kernel(int64_t M, int64_t N, int64_t K, int64_t kchunk, float alpha,
                  const float* __restrict__ A, int64_t lda, const float* __restrict__ B, int64_t ldb,
                  float beta, float* __restrict__ C, int64_t ldc, int64_t zstride, int n_fast,
-                 int aligned) {
-  constexpr int NBUF = (SPLIT == 3) ? 2 : 1;  // hi (and lo) planes
+                 int aligned, double* __restrict__ sumsq) {
+  constexpr int NBUF = lp_planes(SPLIT);  // bf16 planes per operand
+  constexpr int LBK = lp_bk(SPLIT), LROW = LBK + LPAD;
   __shared__ __attribute__((aligned(16))) __bf16 As[2][NBUF][LBM * LROW];
   __shared__ __attribute__((aligned(16))) __bf16 Bs[2][NBUF][LBN * LROW];
 
@@ -206,8 +228,8 @@ gemm_bf16_kernel(int64_t M, int64_t N, int64_t K, int64_t kchunk, float alpha,
   };
   if (nsteps > 0) {
     load_step(0);
-    sa.store(As[0][0], As[0][NBUF - 1], tid);
-    sb.store(Bs[0][0], Bs[0][NBUF - 1], tid);
+    sa.store(As[0][0], tid);
+    sb.store(Bs[0][0], tid);
     if (nsteps > 1) load_step(1);
     __syncthreads();
   }
@@ -216,43 +238,46 @@ gemm_bf16_kernel(int64_t M, int64_t N, int64_t K, int64_t kchunk, float alpha,
 #pragma unroll
     for (int ks = 0; ks < LBK / 32; ++ks) {
       const int kof = ks * 32 + lk * 8;
-      bf16x8_t fa[4], fb[4], fa2[4], fb2[4];
+      // fragments of plane P: fb[P][a] from op(B), fa[P][b] from op(A)
+      bf16x8_t fa[NBUF][4], fb[NBUF][4];
 #pragma unroll
-      for (int a = 0; a < 4; ++a)
-        fb[a] = *reinterpret_cast<const bf16x8_t*>(&Bs[cur][0][(wn0 + a * 16 + li) * LROW + kof]);
-#pragma unroll
-      for (int b = 0; b < 4; ++b)
-        fa[b] = *reinterpret_cast<const bf16x8_t*>(&As[cur][0][(wm0 + b * 16 + li) * LROW + kof]);
-      if (SPLIT == 3) {
+      for (int P = 0; P < NBUF; ++P) {
 #pragma unroll
         for (int a = 0; a < 4; ++a)
-          fb2[a] = *reinterpret_cast<const bf16x8_t*>(
-              &Bs[cur][NBUF - 1][(wn0 + a * 16 + li) * LROW + kof]);
+          fb[P][a] =
+              *reinterpret_cast<const bf16x8_t*>(&Bs[cur][P][(wn0 + a * 16 + li) * LROW + kof]);
 #pragma unroll
         for (int b = 0; b < 4; ++b)
-          fa2[b] = *reinterpret_cast<const bf16x8_t*>(
-              &As[cur][NBUF - 1][(wm0 + b * 16 + li) * LROW + kof]);
+          fa[P][b] =
+              *reinterpret_cast<const bf16x8_t*>(&As[cur][P][(wm0 + b * 16 + li) * LROW + kof]);
       }
 #pragma unroll
       for (int a = 0; a < 4; ++a)
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
-          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[a], fa[b], acc[a][b], 0, 0, 0);
-          if (SPLIT == 3) {
-            acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[a], fa2[b], acc[a][b], 0, 0, 0);
-            acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb2[a], fa[b], acc[a][b], 0, 0, 0);
+          f4v_t d = acc[a][b];
+          if (SPLIT == 6) {  // smallest terms first
+            d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[0][a], fa[NBUF - 1][b], d, 0, 0, 0);
+            d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[NBUF - 1][a], fa[0][b], d, 0, 0, 0);
+            d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[1][a], fa[1][b], d, 0, 0, 0);
           }
+          if (SPLIT >= 3) {
+            d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[0][a], fa[1][b], d, 0, 0, 0);
+            d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[1][a], fa[0][b], d, 0, 0, 0);
+          }
+          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[0][a], fa[0][b], d, 0, 0, 0);
         }
     }
     if (t + 1 < nsteps) {
       const int nxt = cur ^ 1;
-      sa.store(As[nxt][0], As[nxt][NBUF - 1], tid);
-      sb.store(Bs[nxt][0], Bs[nxt][NBUF - 1], tid);
+      sa.store(As[nxt][0], tid);
+      sb.store(Bs[nxt][0], tid);
       if (t + 2 < nsteps) load_step(t + 2);
     }
     __syncthreads();
   }
 
+  double ss = 0.0;
 #pragma unroll
   for (int a = 0; a < 4; ++a)
 #pragma unroll
@@ -266,9 +291,22 @@ gemm_bf16_kernel(int64_t M, int64_t N, int64_t K, int64_t kchunk, float alpha,
           float v = alpha * acc[a][b][r];
           if (beta != 0.0f) v += beta * (*cp);
           *cp = v;
+          ss += (double)v * (double)v;
         }
       }
     }
+  if (sumsq) {
+    // FP64 sum of squares of the stored tile, one partial per workgroup
+    __shared__ double red[LTHREADS / 64];
+    for (int off = 32; off > 0; off >>= 1) ss += __shfl_xor(ss, off, 64);
+    if (lane == 0) red[wave] = ss;
+    __syncthreads();
+    if (tid == 0) {
+      double t = 0.0;
+      for (int w = 0; w < LTHREADS / 64; ++w) t += red[w];
+      sumsq[blockIdx.x + (int64_t)gridDim.x * blockIdx.y] = t;
+    }
+  }
 }
 
 __global__ void splitk_reduce_f32_kernel(int64_t M, int64_t N, int splits,
@@ -299,14 +337,15 @@ int gemm_bf16(Context* c, const char* tag, char ta, char tb, int64_t M, int64_t 
   return gemm_bf16_general(c, tag, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, split);
 }
 
-int gemm_bf16_general(Context* c, const char* tag, char ta, char tb, int64_t M, int64_t N,
-                      int64_t K, float alpha, const float* A, int64_t lda, const float* B,
-                      int64_t ldb, float beta, float* C, int64_t ldc, int split) {
-  if (M < 0 || N < 0 || K < 0) return set_error("gemm_bf16: negative dimension");
-  if (M == 0 || N == 0) return 0;
-  if (K == 0) return gemm_f32(c, ta, tb, M, N, 0, alpha, A, lda, B, ldb, beta, C, ldc);
+// The tiled kernel; sumsq_part (one double per tile, optional) receives the
+// FP64 sums of squares of the stored tiles and disables the k split.
+static int gemm_bf16_tiled(Context* c, const char* tag, char ta, char tb, int64_t M, int64_t N,
+                           int64_t K, float alpha, const float* A, int64_t lda, const float* B,
+                           int64_t ldb, float beta, float* C, int64_t ldc, int split,
+                           double* sumsq_part) {
   const bool tA = (ta == 'T' || ta == 't'), tB = (tb == 'T' || tb == 't');
-  if (split != 1 && split != 3) return set_error("gemm_bf16: split must be 1 or 3");
+  if (split != 1 && split != 3 && split != 6)
+    return set_error("gemm_bf16: split must be 1, 3 or 6");
   const int64_t gm = (M + LBM - 1) / LBM, gn = (N + LBN - 1) / LBN;
   const int n_fast = (gn <= gm) ? 1 : 0;
   const int64_t gx = n_fast ? gn : gm, gy = n_fast ? gm : gn;
@@ -314,7 +353,7 @@ int gemm_bf16_general(Context* c, const char* tag, char ta, char tb, int64_t M, 
   int splits = 1;
   const int64_t tiles = gm * gn;
   const int64_t target = 2LL * (c->num_cus > 0 ? c->num_cus : 256);
-  if (tiles < target && K >= 8 * 256) {
+  if (!sumsq_part && tiles < target && K >= 8 * 256) {
     int64_t s = (target + tiles - 1) / tiles, maxs = K / 512;
     if (s > maxs) s = maxs;
     if (s > 32) s = 32;
@@ -323,7 +362,7 @@ int gemm_bf16_general(Context* c, const char* tag, char ta, char tb, int64_t M, 
   int64_t kchunk = K;
   if (splits > 1) {
     kchunk = (K + splits - 1) / splits;
-    kchunk = (kchunk + LBK - 1) / LBK * LBK;
+    kchunk = (kchunk + lp_bk(split) - 1) / lp_bk(split) * lp_bk(split);
     splits = (int)((K + kchunk - 1) / kchunk);
   }
   DBuf<float> work;
@@ -346,17 +385,22 @@ int gemm_bf16_general(Context* c, const char* tag, char ta, char tb, int64_t M, 
 #define RSVD_LP_LAUNCH(TA_, TB_, S_)                                                         \
   hipLaunchKernelGGL((gemm_bf16_kernel<TA_, TB_, S_>), grid, dim3(LTHREADS), 0,        \
                      c->stream, M, N, K, kchunk, alpha, A, lda, B, ldb, b_eff, Cout, ldo,    \
-                     zstride, n_fast, aligned)
+                     zstride, n_fast, aligned, sumsq_part)
     if (split == 1) {
       if (!tA && !tB) RSVD_LP_LAUNCH(false, false, 1);
       else if (tA && !tB) RSVD_LP_LAUNCH(true, false, 1);
       else if (!tA && tB) RSVD_LP_LAUNCH(false, true, 1);
       else RSVD_LP_LAUNCH(true, true, 1);
-    } else {
+    } else if (split == 3) {
       if (!tA && !tB) RSVD_LP_LAUNCH(false, false, 3);
       else if (tA && !tB) RSVD_LP_LAUNCH(true, false, 3);
       else if (!tA && tB) RSVD_LP_LAUNCH(false, true, 3);
       else RSVD_LP_LAUNCH(true, true, 3);
+    } else {
+      if (!tA && !tB) RSVD_LP_LAUNCH(false, false, 6);
+      else if (tA && !tB) RSVD_LP_LAUNCH(true, false, 6);
+      else if (!tA && tB) RSVD_LP_LAUNCH(false, true, 6);
+      else RSVD_LP_LAUNCH(true, true, 6);
     }
 #undef RSVD_LP_LAUNCH
     RSVD_LAUNCH_CHECK();
@@ -369,6 +413,49 @@ int gemm_bf16_general(Context* c, const char* tag, char ta, char tb, int64_t M, 
                        splits, (const float*)work.p, beta, C, ldc);
     RSVD_LAUNCH_CHECK();
   }
+  return 0;
+}
+
+int gemm_bf16_general(Context* c, const char* tag, char ta, char tb, int64_t M, int64_t N,
+                      int64_t K, float alpha, const float* A, int64_t lda, const float* B,
+                      int64_t ldb, float beta, float* C, int64_t ldc, int split) {
+  if (M < 0 || N < 0 || K < 0) return set_error("gemm_bf16: negative dimension");
+  if (M == 0 || N == 0) return 0;
+  if (K == 0) return gemm_f32(c, ta, tb, M, N, 0, alpha, A, lda, B, ldb, beta, C, ldc);
+  return gemm_bf16_tiled(c, tag, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, split,
+                         nullptr);
+}
+
+__global__ void sum_partials_lp_kernel(int64_t n, const double* __restrict__ part,
+                                       double* __restrict__ out) {
+  __shared__ double red[256];
+  double s = 0;
+  for (int64_t i = threadIdx.x; i < n; i += 256) s += part[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = red[0];
+}
+
+int gemm_bf16_deflate(Context* c, int64_t M, int64_t N, int64_t K, const float* Q, int64_t ldq,
+                      const float* Bt, int64_t ldbt, float* C, int64_t ldc, double* sumsq) {
+  if (M < 0 || N < 0 || K < 0) return set_error("deflate: negative dimension");
+  if (M == 0 || N == 0) {
+    RSVD_HIP(hipMemsetAsync(sumsq, 0, sizeof(double), c->stream));
+    return 0;
+  }
+  if (K == 0) return set_error("deflate: empty block");
+  const int64_t tiles = ((M + LBM - 1) / LBM) * ((N + LBN - 1) / LBN);
+  DBuf<double> part;
+  RSVD_TRY(part.alloc((size_t)tiles));
+  RSVD_TRY(gemm_bf16_tiled(c, "deflate_bf16x6", 'N', 'T', M, N, K, -1.0f, Q, ldq, Bt, ldbt, 1.0f,
+                           C, ldc, 6, part.p));
+  hipLaunchKernelGGL(sum_partials_lp_kernel, dim3(1), dim3(256), 0, c->stream, tiles,
+                     (const double*)part.p, sumsq);
+  RSVD_LAUNCH_CHECK();
   return 0;
 }
 

@@ -1,7 +1,7 @@
 // pipeline_mp.h -- mixed-precision fixed-rank rSVD on a float32 matrix.
 #pragma once
 
-#include "rsvd_internal.h"
+#include "pipeline.h"
 
 namespace rsvd {
 
@@ -21,6 +21,46 @@ struct MpInfo {
 int fixed_rank_mp(Context* c, const float* A, int64_t m, int64_t n, int64_t lda, int k, int p,
                   int vnum, int q, int s, uint64_t seed, const float* dOmega, int prec, float* U,
                   int64_t ldu, double* S, float* V, int64_t ldv, MpInfo* info);
+
+// U (m x k), S (k), V (n x k) from an orthonormal fp32 Q (m x l) of the fp32
+// A, the vnum branches of rra.c:133-225: B^T = A^T Q in split-bf16 (or the
+// caller's Bt_in, n x l, ld ldbt), the small problem in FP64, U and V
+// products in split-bf16 (exact = 0) or with fp32-exact products (exact = 1).
+// *sw = Jacobi sweeps, *fb += QR fallbacks.
+int svd_from_Q_mp(Context* c, const float* A, int64_t m, int64_t n, int64_t lda, const float* Q,
+                  int64_t ldq, int64_t l, int64_t k, int vnum, const float* Bt_in, int64_t ldbt,
+                  int exact, float* U, int64_t ldu, double* S, float* V, int64_t ldv, int* sw,
+                  int* fb);
+
+// randQB_pb_new (is_new = 1) / randQB_pb on a float32 A: a float32 working
+// copy R of A is deflated block by block (R -= Qp Bp with fp32-exact products,
+// Bp = Qp^T R in split-bf16 and its error cleared on the next block, FP64 sum
+// of squares of R for the TOL test); the sketch and power passes over R run
+// at precision prec; each Qp is orthonormalised in FP64 and, from the second
+// block on, re-orthogonalised against all earlier blocks.  Q: m x l fp32
+// (l = kstep*nstep after qb_adjust); Bt (optional): n x l fp32, B^T by
+// column blocks.  dRN (optional): n x l fp32 Gaussian columns, else the
+// seeded stream rounded to float.
+int qb_blocked_mp(Context* c, const float* M, int64_t m, int64_t n, int64_t ldm, int kstep,
+                  int nstep, double TOL, int q, int s, int is_new, uint64_t seed,
+                  const float* dRN, int prec, float* Q, int64_t ldq, float* Bt, int64_t ldbt,
+                  int* frank, int* l_alloc, QbInfo* info);
+
+struct BlockrandMpOut {
+  DBuf<float> U, V;   // m x frank, n x frank
+  DBuf<double> S;     // frank
+  int frank = 0;
+  int l = 0;
+  QbInfo qb;
+  SvdInfo svd;
+};
+
+// low_rank_svd_blockrand_decomp_fixed_rank_or_prec (rra.c:239-381) on a
+// float32 A: qb_blocked_mp, then svd_from_Q_mp on the QB's Q and B with
+// fp32-exact U and V products.
+int blockrand_mp(Context* c, const float* M, int64_t m, int64_t n, int64_t ldm, int k, int p,
+                 double TOL, int vnum, int kstep, int q, int s, uint64_t seed, const float* dRN,
+                 int prec, BlockrandMpOut* out);
 
 // ||A - U diag(S) V^T||_F with A double (m x n) and U, V double or float
 // (uv_f32), accumulated in FP64 over column chunks of A.

@@ -75,6 +75,81 @@ static int orth_f32(Context* c, int64_t m, int64_t n, float* Y, int64_t ldy, int
   return 0;
 }
 
+// U (m x k), S (k), V (n x k) from an orthonormal Q (m x l, fp32) of the fp32
+// A: B^T = A^T Q in split-bf16, then the vnum branch of rra.c:133-225 with
+// the small problem in FP64 and U, V products in split-bf16.
+int svd_from_Q_mp(Context* c, const float* A, int64_t m, int64_t n, int64_t lda, const float* Q,
+                  int64_t ldq, int64_t l, int64_t k, int vnum, const float* Bt_in, int64_t ldbt,
+                  int exact, float* U, int64_t ldu, double* S, float* V, int64_t ldv, int* sw,
+                  int* fb) {
+  if (k <= 0 || k > l) return set_error("svd_from_Q_mp: k=%lld outside [1, l=%lld]",
+                                        (long long)k, (long long)l);
+  if ((vnum == 1 || vnum > 2) && n < l)
+    return set_error("vnum=1 needs n >= l (n=%lld, l=%lld)", (long long)n, (long long)l);
+  // B^T = A^T Q (n x l), split-bf16, unless the caller has it
+  const int usplit = exact ? 6 : 3;
+  const char* utag = exact ? "uv_bf16x6" : "uv_bf16x3";
+  DBuf<float> Btb;
+  const float* Btp = Bt_in;
+  int64_t ldb = ldbt;
+  if (!Btp) {
+    RSVD_TRY(Btb.alloc((size_t)(n * l)));
+    RSVD_TRY(gemm_bf16(c, "proj_bf16x3", 'T', 'N', n, l, m, 1.0f, A, lda, Q, ldq, 0.0f, Btb.p, n,
+                       3));
+    Btp = Btb.p;
+    ldb = n;
+  }
+  DBuf<double> Bt64, G, Uh, lam, sig;
+  RSVD_TRY(Bt64.alloc((size_t)(n * l)));
+  RSVD_TRY(G.alloc((size_t)(l * l)));
+  RSVD_TRY(Uh.alloc((size_t)(l * l)));
+  RSVD_TRY(lam.alloc((size_t)l));
+  RSVD_TRY(sig.alloc((size_t)l));
+  RSVD_TRY(f32_to_f64(c, n, l, Btp, ldb, Bt64.p, n));
+  DBuf<float> X32;
+  RSVD_TRY(X32.alloc((size_t)(l * l)));
+  if (vnum == 1 || vnum > 2) {
+    // [Qhat, Rhat] = qr(B^T) in FP64, Rhat = Uhat Sigma Vhat^T;
+    // U = Q Vhat, V = Qhat Uhat; descending, first k (rra.c:133-174)
+    DBuf<double> R, Vh;
+    RSVD_TRY(R.alloc((size_t)(l * l)));
+    RSVD_TRY(Vh.alloc((size_t)(l * l)));
+    int f1 = 0;
+    RSVD_TRY(orth_f64(c, n, l, Bt64.p, n, R.p, &f1));
+    *fb += f1;
+    RSVD_TRY(jacobi_f64(c, l, l, R.p, l, Vh.p, l, 60, sw));
+    RSVD_TRY(colnorms_f64(c, l, l, R.p, l, sig.p, 1));
+    RSVD_TRY(sort_pairs_f64(c, l, sig.p, 1, R.p, l, l, Vh.p, l, l));
+    RSVD_TRY(f64_to_f32(c, l, k, Vh.p, l, X32.p, l));
+    RSVD_TRY(gemm_bf16_general(c, utag, 'N', 'N', m, k, l, 1.0f, Q, ldq, X32.p, l, 0.0f, U, ldu,
+                               usplit));
+    DBuf<double> V64;
+    RSVD_TRY(V64.alloc((size_t)(n * k)));
+    RSVD_TRY(gemm_f64(c, 'N', 'N', n, k, l, 1.0, Bt64.p, n, R.p, l, 0.0, V64.p, n));
+    RSVD_TRY(f64_to_f32(c, n, k, V64.p, n, V, ldv));
+    RSVD_HIP(hipMemcpyAsync(S, sig.p, sizeof(double) * k, hipMemcpyDeviceToDevice, c->stream));
+  } else {
+    // BB^T = B^T' B^T in FP64; eig; sigma = sqrt(lambda); U = Q Uhat;
+    // V = B^T Uhat Sigma^-1; ascending, last k (rra.c:175-225)
+    RSVD_TRY(gemm_f64(c, 'T', 'N', l, l, n, 1.0, Bt64.p, n, Bt64.p, n, 0.0, G.p, l));
+    RSVD_TRY(symmetrize_upper_f64(c, l, G.p));
+    RSVD_TRY(symeig_jacobi_f64(c, l, G.p, l, Uh.p, l, lam.p, 60, sw));
+    RSVD_TRY(sort_pairs_f64(c, l, lam.p, 0, Uh.p, l, l, nullptr, 0, 0));
+    RSVD_TRY(sqrt_f64(c, l, lam.p, sig.p));
+    const int64_t off = l - k;
+    RSVD_TRY(f64_to_f32(c, l, k, Uh.p + off * l, l, X32.p, l));
+    RSVD_TRY(gemm_bf16_general(c, utag, 'N', 'N', m, k, l, 1.0f, Q, ldq, X32.p, l, 0.0f, U, ldu,
+                               usplit));
+    RSVD_TRY(scale_cols_f64(c, l, l, Uh.p, l, sig.p, 1));
+    RSVD_TRY(f64_to_f32(c, l, k, Uh.p + off * l, l, X32.p, l));
+    RSVD_TRY(gemm_bf16_general(c, utag, 'N', 'N', n, k, l, 1.0f, Btp, ldb, X32.p, l, 0.0f, V, ldv,
+                               usplit));
+    RSVD_HIP(hipMemcpyAsync(S, sig.p + off, sizeof(double) * k, hipMemcpyDeviceToDevice,
+                            c->stream));
+  }
+  return 0;
+}
+
 int fixed_rank_mp(Context* c, const float* A, int64_t m, int64_t n, int64_t lda, int k, int p,
                   int vnum, int q, int s, uint64_t seed, const float* dOmega, int prec, float* U,
                   int64_t ldu, double* S, float* V, int64_t ldv, MpInfo* info) {
@@ -118,60 +193,191 @@ int fixed_rank_mp(Context* c, const float* A, int64_t m, int64_t n, int64_t lda,
     RSVD_TRY(pass(false, m, n, Z.p, n, Y.p, m));
   }
   RSVD_TRY(orth_f32(c, m, l, Y.p, m, 2, 1, &fb));                      // rra.c:129-130
-  // B^T = A^T Q (n x l), split-bf16
-  DBuf<float> Bt;
-  RSVD_TRY(Bt.alloc((size_t)(n * l)));
-  RSVD_TRY(gemm_bf16(c, "proj_bf16x3", 'T', 'N', n, l, m, 1.0f, A, lda, Y.p, m, 0.0f, Bt.p, n, 3));
-  DBuf<double> Bt64, G, Uh, lam, sig;
-  RSVD_TRY(Bt64.alloc((size_t)(n * l)));
-  RSVD_TRY(G.alloc((size_t)(l * l)));
-  RSVD_TRY(Uh.alloc((size_t)(l * l)));
-  RSVD_TRY(lam.alloc((size_t)l));
-  RSVD_TRY(sig.alloc((size_t)l));
-  RSVD_TRY(f32_to_f64(c, n, l, Bt.p, n, Bt64.p, n));
   int sw = 0;
-  DBuf<float> X32;
-  RSVD_TRY(X32.alloc((size_t)(l * l)));
-  if (vnum == 1 || vnum > 2) {
-    // [Qhat, Rhat] = qr(B^T) in FP64, Rhat = Uhat Sigma Vhat^T;
-    // U = Q Vhat, V = Qhat Uhat; descending, first k (rra.c:133-174)
-    DBuf<double> R, Vh;
-    RSVD_TRY(R.alloc((size_t)(l * l)));
-    RSVD_TRY(Vh.alloc((size_t)(l * l)));
-    int f1 = 0;
-    RSVD_TRY(orth_f64(c, n, l, Bt64.p, n, R.p, &f1));
-    fb += f1;
-    RSVD_TRY(jacobi_f64(c, l, l, R.p, l, Vh.p, l, 60, &sw));
-    RSVD_TRY(colnorms_f64(c, l, l, R.p, l, sig.p, 1));
-    RSVD_TRY(sort_pairs_f64(c, l, sig.p, 1, R.p, l, l, Vh.p, l, l));
-    RSVD_TRY(f64_to_f32(c, l, k, Vh.p, l, X32.p, l));
-    RSVD_TRY(gemm_bf16(c, "uv_bf16x3", 'N', 'N', m, k, l, 1.0f, Y.p, m, X32.p, l, 0.0f, U, ldu, 3));
-    DBuf<double> V64;
-    RSVD_TRY(V64.alloc((size_t)(n * k)));
-    RSVD_TRY(gemm_f64(c, 'N', 'N', n, k, l, 1.0, Bt64.p, n, R.p, l, 0.0, V64.p, n));
-    RSVD_TRY(f64_to_f32(c, n, k, V64.p, n, V, ldv));
-    RSVD_HIP(hipMemcpyAsync(S, sig.p, sizeof(double) * k, hipMemcpyDeviceToDevice, c->stream));
-  } else {
-    // BB^T = B^T' B^T in FP64; eig; sigma = sqrt(lambda); U = Q Uhat;
-    // V = B^T Uhat Sigma^-1; ascending, last k (rra.c:175-225)
-    RSVD_TRY(gemm_f64(c, 'T', 'N', l, l, n, 1.0, Bt64.p, n, Bt64.p, n, 0.0, G.p, l));
-    RSVD_TRY(symmetrize_upper_f64(c, l, G.p));
-    RSVD_TRY(symeig_jacobi_f64(c, l, G.p, l, Uh.p, l, lam.p, 60, &sw));
-    RSVD_TRY(sort_pairs_f64(c, l, lam.p, 0, Uh.p, l, l, nullptr, 0, 0));
-    RSVD_TRY(sqrt_f64(c, l, lam.p, sig.p));
-    const int64_t off = l - k;
-    RSVD_TRY(f64_to_f32(c, l, k, Uh.p + off * l, l, X32.p, l));
-    RSVD_TRY(gemm_bf16(c, "uv_bf16x3", 'N', 'N', m, k, l, 1.0f, Y.p, m, X32.p, l, 0.0f, U, ldu, 3));
-    RSVD_TRY(scale_cols_f64(c, l, l, Uh.p, l, sig.p, 1));
-    RSVD_TRY(f64_to_f32(c, l, k, Uh.p + off * l, l, X32.p, l));
-    RSVD_TRY(gemm_bf16(c, "uv_bf16x3", 'N', 'N', n, k, l, 1.0f, Bt.p, n, X32.p, l, 0.0f, V, ldv, 3));
-    RSVD_HIP(hipMemcpyAsync(S, sig.p + off, sizeof(double) * k, hipMemcpyDeviceToDevice,
-                            c->stream));
-  }
+  RSVD_TRY(svd_from_Q_mp(c, A, m, n, lda, Y.p, m, l, k, vnum, nullptr, 0, 0, U, ldu, S, V, ldv,
+                         &sw, &fb));
   if (info) {
     info->qr_fallbacks = fb;
     info->jacobi_sweeps = sw;
   }
+  return 0;
+}
+
+__global__ void add2d_f32_kernel(int64_t rows, int64_t cols, const float* __restrict__ src,
+                                 int64_t lds, float* __restrict__ dst, int64_t ldd) {
+  const int64_t total = rows * cols;
+  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = idx % rows, j = idx / rows;
+    dst[i + j * ldd] += src[i + j * lds];
+  }
+}
+
+// dst += src (rows x cols, fp32)
+static int add2d_f32(Context* c, int64_t rows, int64_t cols, const float* src, int64_t lds,
+                     float* dst, int64_t ldd) {
+  if (rows * cols == 0) return 0;
+  int64_t b = (rows * cols + 255) / 256;
+  if (b > 4096) b = 4096;
+  hipLaunchKernelGGL(add2d_f32_kernel, dim3((unsigned)b), dim3(256), 0, c->stream, rows, cols, src,
+                     lds, dst, ldd);
+  RSVD_LAUNCH_CHECK();
+  return 0;
+}
+
+// One pass C = op(A) S (N columns) at the sketch precision; ascale: the fp8
+// chunk scales of A (prec = MP_FP8 only).
+static int mp_pass(Context* c, int prec, bool ta, int64_t M, int64_t N, int64_t K, const float* A,
+                   int64_t lda, const float* ascale, const float* S, int64_t lds, float* C,
+                   int64_t ldc) {
+  if (prec == MP_FP8)
+    return apass_fp8(c, "apass_fp8", ta, M, N, K, 1.0f, A, lda, ascale, S, lds, 0.0f, C, ldc);
+  const int split = (prec == MP_BF16X3) ? 3 : 1;
+  return gemm_bf16(c, split == 3 ? "apass_bf16x3" : "apass_bf16", ta ? 'T' : 'N', 'N', M, N, K,
+                   1.0f, A, lda, S, lds, 0.0f, C, ldc, split);
+}
+
+int qb_blocked_mp(Context* c, const float* M, int64_t m, int64_t n, int64_t ldm, int kstep,
+                  int nstep, double TOL, int q, int s, int is_new, uint64_t seed,
+                  const float* dRN, int prec, float* Q, int64_t ldq, float* Bt, int64_t ldbt,
+                  int* frank, int* l_alloc, QbInfo* info) {
+  int tolMode = 0;
+  RSVD_TRY(qb_adjust(m, n, &kstep, &nstep, is_new, &tolMode));
+  if (s <= 0) return set_error("randQB_mp: s must be positive (s=%d)", s);
+  if (prec != MP_BF16 && prec != MP_BF16X3 && prec != MP_FP8)
+    return set_error("randQB_mp: unknown precision %d", prec);
+  const int64_t l = (int64_t)kstep * nstep;
+  if (l_alloc) *l_alloc = (int)l;
+  if (kstep > m) return set_error("randQB_mp: kstep=%d exceeds m=%lld", kstep, (long long)m);
+  DBuf<float> R, Yp, AtQp, Bp, RN, QtQ, ascale;
+  DBuf<double> Y64, sums;
+  RSVD_TRY(R.alloc((size_t)(m * n)));
+  RSVD_TRY(Yp.alloc((size_t)(m * kstep)));
+  RSVD_TRY(AtQp.alloc((size_t)(n * kstep)));
+  RSVD_TRY(Y64.alloc((size_t)(m * kstep)));
+  RSVD_TRY(sums.alloc(1));
+  RSVD_TRY(Bp.alloc((size_t)(n * 2 * kstep)));
+  if (!dRN) RSVD_TRY(RN.alloc((size_t)(n * kstep)));
+  if (prec == MP_FP8) RSVD_TRY(ascale.alloc((size_t)((n + 127) / 128)));
+  RSVD_TRY(copy2d_f32(c, m, n, M, ldm, R.p, m));                      // rra.c:1630
+  int fr = 0, fb = 0, f1 = 0, steps = 0;
+  double normval = 0;
+  for (int step = 0; step < nstep; ++step) {
+    const int64_t c0 = (int64_t)step * kstep;
+    const float* RNp = dRN ? dRN + c0 * n : RN.p;                      // rra.c:1639-1642
+    if (!dRN) RSVD_TRY(gaussian_f32(c, seed, n, kstep, c0, n, RN.p, n));
+    if (prec == MP_FP8) RSVD_TRY(fp8_column_scales(c, m, n, R.p, m, ascale.p));
+    RSVD_TRY(mp_pass(c, prec, false, m, kstep, n, R.p, m, ascale.p, RNp, n, Yp.p, m));
+    for (int j = 1; j <= q; ++j) {                                      // rra.c:1652-1686
+      if ((2 * j - 2) % s == 0) RSVD_TRY(orth_f32(c, m, kstep, Yp.p, m, 1, 0, &fb));
+      RSVD_TRY(mp_pass(c, prec, true, n, kstep, m, R.p, m, ascale.p, Yp.p, m, AtQp.p, n));
+      if ((2 * j - 1) % s == 0) RSVD_TRY(orth_f32(c, n, kstep, AtQp.p, n, 1, 0, &fb));
+      RSVD_TRY(mp_pass(c, prec, false, m, kstep, n, R.p, m, ascale.p, AtQp.p, n, Yp.p, m));
+    }
+    // Qp in FP64 (CholeskyQR2, Householder fallback), rra.c:1690; its fp32
+    // rounding is orthonormal to ~1e-9, so R - Qp (Qp^T R) removes the range
+    // of Qp from R to the rounding of R itself
+    RSVD_TRY(f32_to_f64(c, m, kstep, Yp.p, m, Y64.p, m));
+    RSVD_TRY(orth_f64(c, m, kstep, Y64.p, m, nullptr, &f1));
+    fb += f1;
+    if (step > 0) {
+      // against every earlier block (rra.c:1703-1722 does so every second
+      // step): R is held in fp32, so its rounding (~1e-7 ||A||) is a growing
+      // share of R as R shrinks towards TOL and leaks into the new block
+      RSVD_TRY(f64_to_f32(c, m, kstep, Y64.p, m, Yp.p, m));
+      RSVD_TRY(QtQ.alloc((size_t)(c0 * kstep)));
+      RSVD_TRY(gemm_bf16_general(c, "reorth_bf16x6", 'T', 'N', c0, kstep, m, 1.0f, Q, ldq, Yp.p,
+                                 m, 0.0f, QtQ.p, c0, 6));
+      RSVD_TRY(gemm_bf16_general(c, "reorth_bf16x6", 'N', 'N', m, kstep, c0, -1.0f, Q, ldq, QtQ.p,
+                                 c0, 1.0f, Yp.p, m, 6));
+      RSVD_TRY(f32_to_f64(c, m, kstep, Yp.p, m, Y64.p, m));
+      RSVD_TRY(orth_f64(c, m, kstep, Y64.p, m, nullptr, &f1));
+      fb += f1;
+    }
+    float* Qp = Q + c0 * ldq;
+    RSVD_TRY(f64_to_f32(c, m, kstep, Y64.p, m, Qp, ldq));               // rra.c:1760
+    // Bp^T = R^T Qp (split-bf16), rra.c:1741, and R -= Qp Bp with fp32-exact
+    // products and the FP64 sum of squares of the new R, rra.c:1750 + the TOL
+    // norm of rra.c:1771-1778.  Bp carries the split-bf16 error of a long sum
+    // (~1e-6 relative); that error lies in range(Qp), where later blocks,
+    // orthogonal to Qp, never reach it.  So from the second block on the same
+    // pass also projects R on the previous block (R^T [Q_{p-1} Qp]) and the
+    // deflation removes both: the previous block's residual share is cleared
+    // one step late, to the accuracy of its (small) correction.
+    const int64_t nb = step > 0 ? 2 * kstep : kstep;   // blocks in this deflation
+    const float* Qd = Q + (c0 + kstep - nb) * ldq;
+    RSVD_TRY(gemm_bf16(c, "proj_bf16x3", 'T', 'N', n, nb, m, 1.0f, R.p, m, Qd, ldq, 0.0f, Bp.p, n,
+                       3));
+    RSVD_TRY(gemm_bf16_deflate(c, m, n, nb, Qd, ldq, Bp.p, n, R.p, m, sums.p));
+    if (Bt) {  // B rows of this block, and the previous block's correction
+      RSVD_TRY(copy2d_f32(c, n, kstep, Bp.p + (nb - kstep) * n, n, Bt + c0 * ldbt, ldbt));
+      if (step > 0) RSVD_TRY(add2d_f32(c, n, kstep, Bp.p, n, Bt + (c0 - kstep) * ldbt, ldbt));
+    }
+    fr = (step + 1) * kstep;                                            // rra.c:1770
+    steps = step + 1;
+    if (tolMode) {
+      double ss = 0;
+      RSVD_HIP(hipMemcpyAsync(&ss, sums.p, sizeof ss, hipMemcpyDeviceToHost, c->stream));
+      RSVD_HIP(hipStreamSynchronize(c->stream));
+      normval = sqrt(ss);
+      if (normval < TOL) break;
+    }
+  }
+  if (!tolMode && steps > 0) {
+    double ss = 0;
+    RSVD_HIP(hipMemcpyAsync(&ss, sums.p, sizeof ss, hipMemcpyDeviceToHost, c->stream));
+    RSVD_HIP(hipStreamSynchronize(c->stream));
+    normval = sqrt(ss);
+  }
+  if (frank) *frank = fr;
+  if (info) {
+    info->steps = steps;
+    info->qr_fallbacks = fb;
+    info->last_norm = normval;
+    info->kstep = kstep;
+    info->nstep = nstep;
+    info->tol_mode = tolMode;
+  }
+  return 0;
+}
+
+int blockrand_mp(Context* c, const float* M, int64_t m, int64_t n, int64_t ldm, int k, int p,
+                 double TOL, int vnum, int kstep, int q, int s, uint64_t seed, const float* dRN,
+                 int prec, BlockrandMpOut* out) {
+  int p_eff = 0, nstep = 0, rankMode = 0;
+  RSVD_TRY(blockrand_plan(m, n, k, p, kstep, &p_eff, &nstep, &rankMode));
+  p = p_eff;
+  int ks = kstep, ns = nstep, tm = 0;
+  RSVD_TRY(qb_adjust(m, n, &ks, &ns, 1, &tm));
+  const int64_t lmax = (int64_t)ks * ns;
+  DBuf<float> Q, Bt;
+  RSVD_TRY(Q.alloc((size_t)(m * lmax)));
+  RSVD_TRY(Bt.alloc((size_t)(n * lmax)));
+  int frank = 0, lalloc = 0;
+  QbInfo qi{};
+  RSVD_TRY(qb_blocked_mp(c, M, m, n, ldm, kstep, nstep, TOL, q, s, 1, seed, dRN, prec, Q.p, m,
+                         Bt.p, n, &frank, &lalloc, &qi));
+  const int64_t l = qi.tol_mode ? frank : lalloc;                       // B->nrows
+  if (rankMode == 1) frank = k;                                         // rra.c:273-279
+  else frank = (int)llround((double)(frank / (frank + p + 1e-6)) * frank);
+  out->frank = frank;
+  out->l = (int)l;
+  out->qb = qi;
+  if (frank <= 0 || frank > l)
+    return set_error("blockrand_mp: rank %d outside [1, l=%lld] (k=%d p=%d kstep=%d)", frank,
+                     (long long)l, k, p, kstep);
+  RSVD_TRY(out->U.alloc((size_t)(m * frank)));
+  RSVD_TRY(out->S.alloc((size_t)frank));
+  RSVD_TRY(out->V.alloc((size_t)(n * frank)));
+  int sw = 0, fb = 0;
+  // rra.c:288-380 forms B = Q^T M afresh; here B comes from the QB, where
+  // every block's share was corrected once (a fresh split-bf16 pass over A
+  // would carry a ~1e-6 relative error into a result held to TOL)
+  RSVD_TRY(svd_from_Q_mp(c, M, m, n, ldm, Q.p, m, l, frank, vnum, Bt.p, n, 1, out->U.p, m,
+                         out->S.p, out->V.p, n, &sw, &fb));
+  out->svd.l = l;
+  out->svd.jacobi_sweeps = sw;
+  out->svd.qr_fallbacks = fb;
   return 0;
 }
 

@@ -160,6 +160,16 @@ _SIGS = {
                                          C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                          C.c_ulonglong, C.c_void_p, C.c_int, C.c_void_p,
                                          C.c_longlong, C.c_void_p, C.c_void_p, C.c_longlong]),
+    "rsvd_dev_randqb_mp": (
+        C.c_int, [C.c_void_p, C.c_longlong, C.c_longlong, C.c_longlong, C.c_int, C.c_int,
+                  C.c_double, C.c_int, C.c_int, C.c_int, C.c_ulonglong, C.c_void_p, C.c_int,
+                  C.c_void_p, C.c_longlong, C.c_void_p, C.c_longlong, C.POINTER(C.c_int),
+                  C.POINTER(C.c_double)]),
+    "rsvd_dev_blockrand_mp": (
+        C.c_int, [C.c_void_p, C.c_longlong, C.c_longlong, C.c_longlong, C.c_int, C.c_int,
+                  C.c_double, C.c_int, C.c_int, C.c_int, C.c_int, C.c_ulonglong, C.c_void_p,
+                  C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
+                  C.POINTER(C.c_void_p)]),
     "rsvd_dev_lowrank_residual": (C.c_int, [C.c_void_p, C.c_longlong, C.c_longlong,
                                             C.c_longlong, C.c_longlong, C.c_void_p,
                                             C.c_longlong, C.c_void_p, C.c_void_p, C.c_longlong,

@@ -155,3 +155,162 @@ def test_fixed_rank_mp_against_fp64(gpu_lib, prec, bound):
     # U and V leave the path as float32 after FP64 rotations
     assert ortho <= 5e-5, (prec, ortho)
     assert abs(res.value - res64.value) <= max(bound, 1e-5) * res64.value
+
+
+def _blockrand_case(rp, m, n, seed):
+    """A = U diag(sigma) V^T with a 50-value plateau at 1, then exp(-(j - 50)/40)
+    down to 1e-8 (float32 and its exact double widening)."""
+    lib = rp.load()
+    j = np.arange(min(n, 800), dtype=np.float64)
+    sigma = np.where(j < 50, 1.0, np.exp(-(j - 50) / 40.0))
+    sigma = sigma[sigma >= 1e-8]
+    A64 = rp.DeviceMatrix(m, n, np.float64)
+    A32 = rp.DeviceMatrix(m, n, np.float32)
+    assert lib.rsvd_dev_lowrank_test_matrix(m, n, len(sigma), sigma.ctypes.data, 0.0, seed,
+                                            A64.ptr, m, A32.ptr, m) == 0, rp.last_error()
+    return A64, A32, sigma
+
+
+def _blockrand(rp, mp, A, m, n, k, p, tol, kstep, q, prec=2):
+    lib = rp.load()
+    fr = C.c_int(0)
+    u, sv, v = C.c_void_p(), C.c_void_p(), C.c_void_p()
+    if mp:
+        rc = lib.rsvd_dev_blockrand_mp(A.ptr, m, n, m, k, p, tol, 2, kstep, q, 1, 77, None, prec,
+                                       C.byref(fr), C.byref(u), C.byref(sv), C.byref(v))
+    else:
+        rc = lib.rsvd_dev_blockrand(A.ptr, m, n, m, k, p, tol, 2, kstep, q, 1, 77, None,
+                                    C.byref(fr), C.byref(u), C.byref(sv), C.byref(v))
+    assert rc == 0, rp.last_error()
+    f = fr.value
+    s = np.empty(f)
+    lib.rsvd_memcpy_d2h(s.ctypes.data, sv.value, 8 * f)
+    return f, s, u, sv, v
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", [2, 1, 3])
+def test_blockrand_mp_tol_mode(gpu_lib, prec):
+    """TOL mode (k < 0), TOL = 1e-6 ||A||_F, kstep = 64, q = 1 on 6000 x 1500:
+    the mixed path's explicit residual ||A - U S V^T||_F (FP64, against the
+    double A) meets TOL, its rank is the FP64 path's, and its singular values
+    agree with the FP64 path's to fp32 level relative to sigma_1."""
+    rp = gpu_lib
+    lib = rp.load()
+    m, n, kstep, q = 6000, 1500, 64, 1
+    A64, A32, sigma = _blockrand_case(rp, m, n, 4321)
+    anorm = float(np.sqrt(np.sum(sigma ** 2)))
+    tol = 1e-6 * anorm
+    f64_, s64, *bufs64 = _blockrand(rp, False, A64, m, n, -1, 10, tol, kstep, q)
+    fmp, smp, u, sv, v = _blockrand(rp, True, A32, m, n, -1, 10, tol, kstep, q, prec)
+    res = C.c_double(0)
+    assert lib.rsvd_dev_lowrank_residual(A64.ptr, m, n, m, fmp, u.value, m, sv.value, v.value, n,
+                                         1, C.byref(res)) == 0, rp.last_error()
+    res64 = C.c_double(0)
+    assert lib.rsvd_dev_lowrank_residual(A64.ptr, m, n, m, f64_, bufs64[0].value, m,
+                                         bufs64[1].value, bufs64[2].value, n, 0,
+                                         C.byref(res64)) == 0, rp.last_error()
+    for b in (u, sv, v, *bufs64):
+        lib.rsvd_dev_free(b.value)
+    ds = np.abs(np.sort(smp)[::-1][:min(fmp, f64_)] - np.sort(s64)[::-1][:min(fmp, f64_)]).max()
+    print(f"precision {prec}: frank {fmp} (FP64 {f64_}), residual {res.value / anorm:.3e} "
+          f"(FP64 {res64.value / anorm:.3e}) x ||A||, max |dsigma| {ds:.2e}")
+    # split-bf16 sketches find the FP64 path's blocks; bf16 / fp8 sketches
+    # capture less of R per block and may need more blocks
+    if prec == 2:
+        assert fmp == f64_
+    else:
+        assert fmp >= f64_
+    # the TOL test is the QB's (test_randqb_mp_tol_mode); the shrunk rank of
+    # rra.c:278 then drops the last ~p columns, so the returned factors sit a
+    # little above TOL in either path (FP64 here: 0.90 TOL)
+    assert res.value <= 1.5 * tol and res64.value <= 1.5 * tol
+    assert ds <= 1e-6 * sigma[0]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", [2, 1])
+def test_randqb_mp_tol_mode(gpu_lib, prec):
+    """randQB_pb_new in TOL mode on the float32 A (6000 x 1500, kstep = 64,
+    q = 1, TOL = 1e-6 ||A||_F): the reported norm is below TOL and equals the
+    explicit FP64 ||A - Q B||_F of the returned factors (1%), Q is orthonormal
+    to fp32 level, and split-bf16 sketches stop at the FP64 path's block."""
+    rp = gpu_lib
+    lib = rp.load()
+    m, n, kstep, q = 6000, 1500, 64, 1
+    A64, A32, sigma = _blockrand_case(rp, m, n, 4321)
+    anorm = float(np.sqrt(np.sum(sigma ** 2)))
+    tol = 1e-6 * anorm
+    lmax = lib.rsvd_qb_max_cols(m, n, kstep, 0, 1)
+    Q = rp.DeviceMatrix(m, lmax, np.float32)
+    Bt = rp.DeviceMatrix(n, lmax, np.float32)
+    fr, fn = C.c_int(0), C.c_double(0)
+    assert lib.rsvd_dev_randqb_mp(A32.ptr, m, n, m, kstep, 0, tol, q, 1, 1, 77, None, prec, Q.ptr,
+                                  m, Bt.ptr, n, C.byref(fr), C.byref(fn)) == 0, rp.last_error()
+    Q64, B64 = rp.DeviceMatrix(m, lmax), rp.DeviceMatrix(lmax, n)
+    fr64, fn64 = C.c_int(0), C.c_double(0)
+    assert lib.rsvd_dev_randqb(A64.ptr, m, n, m, kstep, 0, tol, q, 1, 1, 77, None, Q64.ptr, m,
+                               B64.ptr, lmax, C.byref(fr64), C.byref(fn64)) == 0, rp.last_error()
+    f = fr.value
+    A = A32.to_host().astype(np.float64)
+    Qh = Q.to_host()[:, :f].astype(np.float64)
+    Bh = Bt.to_host()[:, :f].astype(np.float64)
+    explicit = np.linalg.norm(A - Qh @ Bh.T)
+    orth = np.abs(Qh.T @ Qh - np.eye(f)).max()
+    print(f"precision {prec}: columns {f} (FP64 {fr64.value}), norm {fn.value / anorm:.4e} "
+          f"(FP64 {fn64.value / anorm:.4e}), explicit {explicit / anorm:.4e}, orth {orth:.1e}")
+    assert fn.value < tol
+    assert abs(explicit - fn.value) <= 0.01 * fn.value
+    assert orth <= 1e-6
+    if prec == 2:
+        assert f == fr64.value
+
+
+@pytest.mark.gpu
+def test_blockrand_mp_rank_mode(gpu_lib):
+    """Rank mode (k = 120, p = 10, kstep = 64 -> the p -> kstep rule of
+    rra.c:260-262): singular values against the FP64 path on the same seeds."""
+    rp = gpu_lib
+    lib = rp.load()
+    m, n = 4000, 1200
+    A64, A32, sigma = _blockrand_case(rp, m, n, 99)
+    f64_, s64, *b64 = _blockrand(rp, False, A64, m, n, 120, 10, 0.0, 64, 1)
+    fmp, smp, *bmp = _blockrand(rp, True, A32, m, n, 120, 10, 0.0, 64, 1)
+    for b in (*b64, *bmp):
+        lib.rsvd_dev_free(b.value)
+    assert fmp == f64_ == 120
+    rel = np.abs(smp - s64) / sigma[0]
+    print(f"rank mode: max |dsigma| / sigma_1 {rel.max():.2e}")
+    assert rel.max() <= 1e-6
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("ta,tb", [("N", "T"), ("T", "N"), ("N", "N"), ("T", "T")])
+@pytest.mark.parametrize("M,N,K", [(1000, 700, 64), (517, 300, 512), (4096, 256, 256)])
+def test_gemm_bf16x6_is_fp32_exact(gpu_lib, ta, tb, M, N, K):
+    """rsvd_dev_gemm dtype 5 (three bf16 planes per operand, six products):
+    C = C0 - op(A) op(B) with C0 the float rounding of op(A) op(B) cancels to
+    the fp32 accumulation error, where the 16-bit split (dtype 3) leaves
+    ~2^-16."""
+    rng = np.random.default_rng(M + K)
+    opA = rng.standard_normal((M, K)).astype(np.float32)
+    opB = rng.standard_normal((K, N)).astype(np.float32)
+    A = np.ascontiguousarray(opA.T) if ta == "T" else opA
+    B = np.ascontiguousarray(opB.T) if tb == "T" else opB
+    exact = opA.astype(np.float64) @ opB.astype(np.float64)
+    C0 = exact.astype(np.float32)
+    mag = np.abs(opA).astype(np.float64) @ np.abs(opB).astype(np.float64)
+    lib = gpu_lib.load()
+    errs = {}
+    for dtype in (5, 3):
+        dA, dB = gpu_lib.DeviceMatrix.from_host(A), gpu_lib.DeviceMatrix.from_host(B)
+        dC = gpu_lib.DeviceMatrix.from_host(C0)
+        assert lib.rsvd_dev_gemm(dtype, ta.encode(), tb.encode(), M, N, K, -1.0, dA.ptr, dA.rows,
+                                 dB.ptr, dB.rows, 1.0, dC.ptr, M) == 0, gpu_lib.last_error()
+        got = dC.to_host().astype(np.float64)
+        want = C0.astype(np.float64) - exact
+        errs[dtype] = float((np.abs(got - want) / mag).max())
+    print(f"max |err| / (|A||B|^T): x6 {errs[5]:.2e}, x3 {errs[3]:.2e}")
+    # fp32 accumulation over K / 32 MFMA steps plus the final subtraction
+    assert errs[5] <= (K / 32 + 8) * 2.0**-24
+    assert errs[3] > 4 * errs[5]
