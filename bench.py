@@ -226,6 +226,8 @@ def bench_blocked(args, rp, lib) -> int:
     sigma = np.where(j < 300, 1.0, np.exp(-(j - 300) / 150.0))
     sigma = sigma[sigma >= 1e-8]
     r = len(sigma)
+    if args.precision in ("bf16+x3", "fp8+x3"):
+        raise SystemExit(f"blocked workload: precision {args.precision} is for the fixed-rank path")
     prec_id = {"bf16": 1, "bf16x3": 2, "fp8": 3}.get(args.precision)
     log(f"generating {m}x{n} rank-{r} matrix")
     A64 = rp.DeviceMatrix(m, n)
@@ -386,7 +388,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--precision", default="bf16x3", choices=["bf16", "bf16x3", "fp8", "fp64"],
+    ap.add_argument("--precision", default="bf16x3", choices=["bf16", "bf16x3", "fp8", "bf16+x3", "fp8+x3", "fp64"],
                     help="bf16x3 (default) meets the 1e-5 error target; bf16 (~4e-5) and fp8 "
                          "e4m3 (per-128-column scales) are faster and coarser")
     ap.add_argument("--m", type=int, default=65536)
@@ -468,7 +470,7 @@ def main() -> int:
         s64 = S64.to_host()[:, 0]
 
     # ---- mixed precision ----
-    prec_id = {"bf16": 1, "bf16x3": 2, "fp8": 3}.get(args.precision)
+    prec_id = {"bf16": 1, "bf16x3": 2, "fp8": 3, "bf16+x3": 4, "fp8+x3": 5}.get(args.precision)
     if prec_id:
         Um, Sm, Vm = (rp.DeviceMatrix(m, k, np.float32), rp.DeviceMatrix(k, 1),
                       rp.DeviceMatrix(n, k, np.float32))
@@ -478,10 +480,13 @@ def main() -> int:
                                           Um.ptr, m, Sm.ptr, Vm.ptr, n):
                 raise SystemExit("rsvd_dev_fixed_rank_mp: " + rp.last_error())
 
-        tag = {1: "apass_bf16", 2: "apass_bf16x3", 3: "apass_fp8"}[prec_id]
+        # roofline kernel: the family with most of the pass time
+        tag = {1: "apass_bf16", 2: "apass_bf16x3", 3: "apass_fp8", 4: "apass_bf16",
+               5: "apass_fp8"}[prec_id]
         ms, ks = _timed(lib, rp, stepmp, args.steps, args.warmup,
                         [tag], ["proj_bf16x3", "symeig", "chol_inv", "gram_bf16x3",
-                                "orth_bf16x3", "uv_bf16x3"])
+                                "orth_bf16x3", "uv_bf16x3", "gram_bf16x6", "orth_bf16x6"]
+                        + (["apass_bf16x3"] if prec_id in (4, 5) else []))
         log(f"{args.precision}: {ms:.2f} ms/step")
         smp = Sm.to_host()[:, 0]
         res = C.c_double(0)
@@ -567,8 +572,8 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"bf16": "bf16", "bf16x3": "bf16x3", "fp8": "fp8e4m3",
-                  "fp64": "f64"}[args.precision],
+        "dtype": {"bf16": "bf16", "bf16x3": "bf16x3", "fp8": "fp8e4m3", "bf16+x3": "bf16+bf16x3",
+                  "fp8+x3": "fp8e4m3+bf16x3", "fp64": "f64"}[args.precision],
         "data": "synthetic",
         "config": {"workload": f"fixed-rank rSVD {m}x{n} fp32, rank-{r} + N(0,1e-4^2) noise, "
                                f"k={k} p={p} q={q} s=1 vnum=2",

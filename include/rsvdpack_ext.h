@@ -67,7 +67,8 @@ int rsvd_dev_fixed_rank(const double *dA, long long m, long long n, long long ld
  * (fp32 accumulation); 2: the same passes in split-bf16 (16-bit operands, about
  * 1e-7 relative error on long sums); 3:
  * the same passes in fp8 e4m3 (A scaled per 128 columns, the skinny operand
- * per column; fp32 accumulation).
+ * per column; fp32 accumulation); 4 / 5: every pass but the last in bf16 /
+ * fp8, the last power-iteration pass (whose range becomes Q) in split-bf16.
  * Always: CholeskyQR with FP64 Cholesky, split-bf16 projection B^T = A^T Q,
  * FP64 BB^T and eigen/SVD solve.  Omega: dOmega (n x (k+p) float, ld n) or the
  * seeded stream rounded to float.  Outputs: dU m x k float (ldu), dS k doubles

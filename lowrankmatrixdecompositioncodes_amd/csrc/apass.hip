@@ -226,12 +226,14 @@ apass_kernel(int64_t M, int64_t N, int64_t K, float alpha, const float* __restri
 }
 
 // ---------------------------------------------------------------- apass2
-// Eight waves (two per SIMD) and a 192 x 288 tile per workgroup: waves
-// (wm, wn) = (wave % 4, wave / 4) own 48 rows x 144 columns (3 x 9 tiles of
-// 16 x 16, 108 accumulators; two waves per SIMD leave 256 registers each).
+// Eight waves (two per SIMD) and a 256 x 288 tile per workgroup: waves
+// (wm, wn) = (wave % 4, wave / 4) own 64 rows x 144 columns (4 x 9 tiles of
+// 16 x 16, 144 accumulators; two waves per SIMD leave 256 registers each).
 // Per k block of 32 the workgroup needs 288 columns of the skinny operand (hi
-// and lo planes, 36 KiB) and 192 rows of A (24 KiB): 2x fewer bytes per
-// output from L2 than the 64 x 576 tile.
+// and lo planes, 36 KiB) and 256 rows of A (32 KiB): 2.5x fewer bytes per
+// output from L2 than the 64 x 576 tile.  At m = 65536, l <= 576 the grid is
+// 512 workgroups, two full waves of one workgroup per CU (a 192-row tile gave
+// 683: 2.67 waves, the last one a third idle, and measured 8% slower).
 //   * The skinny operand goes HBM/L2 -> LDS with direct loads
 //     (global_load_lds_dwordx4, no registers) into a ring of three stages,
 //     issued two k blocks ahead; each wave reads its fragments from LDS one
@@ -251,7 +253,7 @@ typedef __attribute__((address_space(1))) void* gbl_ptr_t;
 
 constexpr int kW2 = 8;
 constexpr int kT2 = 64 * kW2;
-constexpr int kMT2 = 3;               // row tiles per wave
+constexpr int kMT2 = 4;               // row tiles per wave
 constexpr int kBM2 = 4 * 16 * kMT2;   // rows per workgroup
 constexpr int kNTW2 = 9;
 constexpr int kBNT2 = 2 * kNTW2;
@@ -682,9 +684,9 @@ int apass_bf16(Context* c, const char* tag, bool ta, int64_t M, int64_t N, int64
   // the CU count (keeping at least 8 k blocks per slice)
   const int64_t gx2 = (M + kBM2 - 1) / kBM2;
   const int64_t gy2 = ((N + 15) / 16 + kBNT2 - 1) / kBNT2;
-  // (op(A) = A only: with op(A) = A^T the 192 rows of a workgroup are 192
+  // (op(A) = A only: with op(A) = A^T the 256 rows of a workgroup are 256
   // separate 128-byte streams per k block, and the 64-row tile keeps the
-  // number of open streams -- and so DRAM pages and TLB entries -- 3x lower)
+  // number of open streams -- and so DRAM pages and TLB entries -- 4x lower)
   int ks = 1;
   while (gx2 * gy2 * ks < cus && KB / (2 * ks) >= 8) ks *= 2;
   const bool use2 = !ta && gx2 * gy2 * ks >= cus && gx2 * gy2 * ks <= 0x7fffffff;

@@ -7,8 +7,9 @@ namespace rsvd {
 
 // Precision of the sketch and power-iteration passes over A (the projection
 // B^T = A^T Q is always split-bf16): bf16, split-bf16, or fp8 e4m3 with
-// per-128-column scales.
-enum MpPrecision { MP_BF16 = 1, MP_BF16X3 = 2, MP_FP8 = 3 };
+// per-128-column scales, for every pass; or bf16 (MP_BF16_SPLIT) / fp8
+// (MP_FP8_SPLIT) for every pass but the last, which is split-bf16.
+enum MpPrecision { MP_BF16 = 1, MP_BF16X3 = 2, MP_FP8 = 3, MP_BF16_SPLIT = 4, MP_FP8_SPLIT = 5 };
 
 struct MpInfo {
   int qr_fallbacks = 0;

@@ -24,9 +24,9 @@
 namespace rsvd {
 
 // Y (m x n, fp32) <- orthonormal basis of its range.  passes = 1 (CholeskyQR)
-// or 2 (CholeskyQR2).  precise = 1 uses the fp32 matrix cores for the Gram
-// and Y R^-1 products, else split-bf16.  Falls back to FP64 Householder on a
-// Cholesky breakdown.
+// or 2 (CholeskyQR2).  precise = 1 forms the Gram on the fp32 matrix cores
+// and Y R^-1 with fp32-exact products (three bf16 planes per operand), else
+// both in split-bf16.  Falls back to FP64 Householder on a Cholesky breakdown.
 static int orth_f32(Context* c, int64_t m, int64_t n, float* Y, int64_t ldy, int passes,
                     int precise, int* fallback) {
   DBuf<float> G32, W, Rinv32;
@@ -64,7 +64,8 @@ static int orth_f32(Context* c, int64_t m, int64_t n, float* Y, int64_t ldy, int
     float* dst = (src == Y) ? W.p : Y;
     const int64_t ldd = (src == Y) ? m : ldy;
     if (precise)
-      RSVD_TRY(gemm_f32(c, 'N', 'N', m, n, n, 1.0f, src, lds, Rinv32.p, n, 0.0f, dst, ldd));
+      RSVD_TRY(gemm_bf16_general(c, "orth_bf16x6", 'N', 'N', m, n, n, 1.0f, src, lds, Rinv32.p, n,
+                                 0.0f, dst, ldd, 6));
     else
       RSVD_TRY(gemm_bf16(c, "orth_bf16x3", 'N', 'N', m, n, n, 1.0f, src, lds, Rinv32.p, n, 0.0f,
                          dst, ldd, 3));
@@ -150,6 +151,18 @@ int svd_from_Q_mp(Context* c, const float* A, int64_t m, int64_t n, int64_t lda,
   return 0;
 }
 
+// One pass C = op(A) S (N columns) at the sketch precision; ascale: the fp8
+// chunk scales of A (prec = MP_FP8 only).
+static int mp_pass(Context* c, int prec, bool ta, int64_t M, int64_t N, int64_t K, const float* A,
+                   int64_t lda, const float* ascale, const float* S, int64_t lds, float* C,
+                   int64_t ldc) {
+  if (prec == MP_FP8)
+    return apass_fp8(c, "apass_fp8", ta, M, N, K, 1.0f, A, lda, ascale, S, lds, 0.0f, C, ldc);
+  const int split = (prec == MP_BF16X3) ? 3 : 1;
+  return gemm_bf16(c, split == 3 ? "apass_bf16x3" : "apass_bf16", ta ? 'T' : 'N', 'N', M, N, K,
+                   1.0f, A, lda, S, lds, 0.0f, C, ldc, split);
+}
+
 int fixed_rank_mp(Context* c, const float* A, int64_t m, int64_t n, int64_t lda, int k, int p,
                   int vnum, int q, int s, uint64_t seed, const float* dOmega, int prec, float* U,
                   int64_t ldu, double* S, float* V, int64_t ldv, MpInfo* info) {
@@ -157,23 +170,30 @@ int fixed_rank_mp(Context* c, const float* A, int64_t m, int64_t n, int64_t lda,
   if (k <= 0 || p < 0) return set_error("fixed_rank_mp: need k > 0 and p >= 0");
   if (l > (m < n ? m : n)) return set_error("fixed_rank_mp: k+p exceeds min(m,n)");
   if (s <= 0) return set_error("fixed_rank_mp: s must be positive");
-  if (prec != MP_BF16 && prec != MP_BF16X3 && prec != MP_FP8)
+  if (prec < MP_BF16 || prec > MP_FP8_SPLIT)
     return set_error("fixed_rank_mp: unknown precision %d", prec);
-  const int split = (prec == MP_BF16X3) ? 3 : 1;
-  const char* tag = (prec == MP_FP8) ? "apass_fp8" : (split == 3) ? "apass_bf16x3" : "apass_bf16";
+  // precision of pass i of npass (the sketch, then 2(q-1) power passes): the
+  // two-level modes run every pass but the last in bf16 / fp8 and the last,
+  // whose range becomes Q, in split-bf16
+  const int npass = 1 + 2 * (q > 1 ? q - 1 : 0);
+  auto prec_of = [&](int i) -> int {
+    if (prec <= MP_FP8) return prec;
+    if (i + 1 == npass) return MP_BF16X3;
+    return prec == MP_BF16_SPLIT ? MP_BF16 : MP_FP8;
+  };
   // fp8: the per-128-column scales of A, once per call
   DBuf<float> ascale;
-  if (prec == MP_FP8) {
+  bool any_fp8 = false;
+  for (int i = 0; i < npass; ++i) any_fp8 = any_fp8 || prec_of(i) == MP_FP8;
+  if (any_fp8) {
     RSVD_TRY(ascale.alloc((size_t)((n + 127) / 128)));
     RSVD_TRY(fp8_column_scales(c, m, n, A, lda, ascale.p));
   }
   // one sketch / power pass: C = op(A) S
+  int ip = 0;
   auto pass = [&](bool ta, int64_t M, int64_t K, const float* S, int64_t lds_, float* C,
                   int64_t ldc) -> int {
-    if (prec == MP_FP8)
-      return apass_fp8(c, tag, ta, M, l, K, 1.0f, A, lda, ascale.p, S, lds_, 0.0f, C, ldc);
-    return gemm_bf16(c, tag, ta ? 'T' : 'N', 'N', M, l, K, 1.0f, A, lda, S, lds_, 0.0f, C, ldc,
-                     split);
+    return mp_pass(c, prec_of(ip++), ta, M, l, K, A, lda, ascale.p, S, lds_, C, ldc);
   };
   DBuf<float> Om, Y, Z;
   const float* Omp = dOmega;
@@ -225,18 +245,6 @@ static int add2d_f32(Context* c, int64_t rows, int64_t cols, const float* src, i
   return 0;
 }
 
-// One pass C = op(A) S (N columns) at the sketch precision; ascale: the fp8
-// chunk scales of A (prec = MP_FP8 only).
-static int mp_pass(Context* c, int prec, bool ta, int64_t M, int64_t N, int64_t K, const float* A,
-                   int64_t lda, const float* ascale, const float* S, int64_t lds, float* C,
-                   int64_t ldc) {
-  if (prec == MP_FP8)
-    return apass_fp8(c, "apass_fp8", ta, M, N, K, 1.0f, A, lda, ascale, S, lds, 0.0f, C, ldc);
-  const int split = (prec == MP_BF16X3) ? 3 : 1;
-  return gemm_bf16(c, split == 3 ? "apass_bf16x3" : "apass_bf16", ta ? 'T' : 'N', 'N', M, N, K,
-                   1.0f, A, lda, S, lds, 0.0f, C, ldc, split);
-}
-
 int qb_blocked_mp(Context* c, const float* M, int64_t m, int64_t n, int64_t ldm, int kstep,
                   int nstep, double TOL, int q, int s, int is_new, uint64_t seed,
                   const float* dRN, int prec, float* Q, int64_t ldq, float* Bt, int64_t ldbt,
@@ -245,7 +253,7 @@ int qb_blocked_mp(Context* c, const float* M, int64_t m, int64_t n, int64_t ldm,
   RSVD_TRY(qb_adjust(m, n, &kstep, &nstep, is_new, &tolMode));
   if (s <= 0) return set_error("randQB_mp: s must be positive (s=%d)", s);
   if (prec != MP_BF16 && prec != MP_BF16X3 && prec != MP_FP8)
-    return set_error("randQB_mp: unknown precision %d", prec);
+    return set_error("randQB_mp: precision %d not available for the blocked QB", prec);
   const int64_t l = (int64_t)kstep * nstep;
   if (l_alloc) *l_alloc = (int)l;
   if (kstep > m) return set_error("randQB_mp: kstep=%d exceeds m=%lld", kstep, (long long)m);
