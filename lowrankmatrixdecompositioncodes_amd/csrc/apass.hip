@@ -233,7 +233,8 @@ apass_kernel(int64_t M, int64_t N, int64_t K, float alpha, const float* __restri
 // and lo planes, 36 KiB) and 256 rows of A (32 KiB): 2.5x fewer bytes per
 // output from L2 than the 64 x 576 tile.  At m = 65536, l <= 576 the grid is
 // 512 workgroups, two full waves of one workgroup per CU (a 192-row tile gave
-// 683: 2.67 waves, the last one a third idle, and measured 8% slower).
+// 683: 2.67 waves, the last one a third idle, and measured 8% slower).  For
+// op(A) = A^T the tile is 128 x 288 (two row tiles per wave).
 //   * The skinny operand goes HBM/L2 -> LDS with direct loads
 //     (global_load_lds_dwordx4, no registers) into a ring of three stages,
 //     issued two k blocks ahead; each wave reads its fragments from LDS one
@@ -253,13 +254,14 @@ typedef __attribute__((address_space(1))) void* gbl_ptr_t;
 
 constexpr int kW2 = 8;
 constexpr int kT2 = 64 * kW2;
-constexpr int kMT2 = 4;               // row tiles per wave
+constexpr int kMT2 = 4;               // row tiles per wave (op(A) = A)
 constexpr int kBM2 = 4 * 16 * kMT2;   // rows per workgroup
+constexpr int kMT2T = 2;              // row tiles per wave (op(A) = A^T)
 constexpr int kNTW2 = 9;
 constexpr int kBNT2 = 2 * kNTW2;
 constexpr int kST2 = 3;
 
-template <bool TA, int SPLIT, bool VEC>
+template <bool TA, int SPLIT, bool VEC, int MT>
 __global__ void __launch_bounds__(kT2)
 apass2_kernel(int64_t M, int64_t N, int64_t KBtot, int64_t KBper, float alpha,
               const float* __restrict__ A, int64_t lda, const bf16x8_t* __restrict__ St,
@@ -292,12 +294,12 @@ apass2_kernel(int64_t M, int64_t N, int64_t KBtot, int64_t KBper, float alpha,
   bx = __builtin_amdgcn_readfirstlane(bx);
   by = __builtin_amdgcn_readfirstlane(by);
   kz = __builtin_amdgcn_readfirstlane(kz);
-  if ((int64_t)bx * kBM2 >= M) return;  // padding of the row-block count to 8
+  if ((int64_t)bx * (64 * MT) >= M) return;  // padding of the row-block count to 8
   const int64_t kb0 = (int64_t)kz * KBper;
   const int64_t kbe = (kb0 + KBper < KBtot) ? kb0 + KBper : KBtot;
   const int64_t nk = kbe - kb0;
   const int64_t klast = kbe - 1;
-  const int64_t mw = (int64_t)bx * kBM2 + wm * 16 * kMT2;  // this wave's first row
+  const int64_t mw = (int64_t)bx * (64 * MT) + wm * 16 * MT;  // this wave's first row
   const int64_t ntw = (int64_t)by * kBNT2;          // workgroup's first n tile
 
   // ---- S stage loads: chunk c = (plane c / 18, tile c % 18), wave-strided;
@@ -316,18 +318,18 @@ apass2_kernel(int64_t M, int64_t N, int64_t KBtot, int64_t KBper, float alpha,
   };
 
   // ---- A: lane -> row mw + 16 i + lane % 16, k 8 (lane / 16) .. + 7
-  const float* Ar[kMT2];
+  const float* Ar[MT];
 #pragma unroll
-  for (int i = 0; i < kMT2; ++i) {
+  for (int i = 0; i < MT; ++i) {
     int64_t r = mw + 16 * i + (lane & 15);
     r = r < M ? r : M - 1;
     Ar[i] = TA ? A + r * lda + 8 * (lane >> 4) : A + r + (int64_t)(8 * (lane >> 4)) * lda;
   }
-  float raw[kMT2][8];
+  float raw[MT][8];
   auto loadA = [&](int64_t kb) {
     kb = kb < klast ? kb : klast;
 #pragma unroll
-    for (int i = 0; i < kMT2; ++i) {
+    for (int i = 0; i < MT; ++i) {
       if (TA) {
         const float* p = Ar[i] + kb * 32;
         if (VEC) {
@@ -349,10 +351,10 @@ apass2_kernel(int64_t M, int64_t N, int64_t KBtot, int64_t KBper, float alpha,
       }
     }
   };
-  bf16x8_t ah[kMT2], al[kMT2];
+  bf16x8_t ah[MT], al[MT];
   auto split = [&]() {
 #pragma unroll
-    for (int i = 0; i < kMT2; ++i)
+    for (int i = 0; i < MT; ++i)
 #pragma unroll
       for (int e = 0; e < 8; ++e) {
         const __bf16 hb = (__bf16)raw[i][e];
@@ -361,11 +363,11 @@ apass2_kernel(int64_t M, int64_t N, int64_t KBtot, int64_t KBper, float alpha,
       }
   };
 
-  f4v_t acc[kNTW2][kMT2];
+  f4v_t acc[kNTW2][MT];
 #pragma unroll
   for (int j = 0; j < kNTW2; ++j)
 #pragma unroll
-    for (int i = 0; i < kMT2; ++i) acc[j][i] = f4v_t{0.f, 0.f, 0.f, 0.f};
+    for (int i = 0; i < MT; ++i) acc[j][i] = f4v_t{0.f, 0.f, 0.f, 0.f};
 
   // one k block on stage `cur`: A(kb + 1) and S(kb + 2) loads first, then the
   // MFMAs with the S fragments read one tile ahead, then the A split for kb + 1
@@ -384,14 +386,14 @@ apass2_kernel(int64_t M, int64_t N, int64_t KBtot, int64_t KBper, float alpha,
         if (SPLIT == 3) sl[b ^ 1] = cur[NP - 1][wn * kNTW2 + j + 1][lane];
       }
 #pragma unroll
-      for (int i = 0; i < kMT2; ++i)
+      for (int i = 0; i < MT; ++i)
         acc[j][i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(sh[b], ah[i], acc[j][i], 0, 0, 0);
       if (SPLIT == 3) {
 #pragma unroll
-        for (int i = 0; i < kMT2; ++i)
+        for (int i = 0; i < MT; ++i)
           acc[j][i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(sh[b], al[i], acc[j][i], 0, 0, 0);
 #pragma unroll
-        for (int i = 0; i < kMT2; ++i)
+        for (int i = 0; i < MT; ++i)
           acc[j][i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(sl[b], ah[i], acc[j][i], 0, 0, 0);
       }
       // keep the fragment reads one tile ahead (not all hoisted: registers)
@@ -422,7 +424,7 @@ apass2_kernel(int64_t M, int64_t N, int64_t KBtot, int64_t KBper, float alpha,
 #pragma unroll
   for (int j = 0; j < kNTW2; ++j) {
 #pragma unroll
-    for (int i = 0; i < kMT2; ++i) {
+    for (int i = 0; i < MT; ++i) {
       const int64_t m = mw + i * 16 + (lane & 15);
 #pragma unroll
       for (int rr = 0; rr < 4; ++rr) {
@@ -680,16 +682,18 @@ int apass_bf16(Context* c, const char* tag, bool ta, int64_t M, int64_t N, int64
                              ldc, split);
   const int64_t KB = K32 / 32;
   const int64_t cus = c->num_cus > 0 ? c->num_cus : 256;
-  // the 256 x 288 kernel, with the k range split while the grid is short of
-  // the CU count (keeping at least 8 k blocks per slice)
-  const int64_t gx2 = (M + kBM2 - 1) / kBM2;
+  // the LDS-ring kernel, 256 x 288 tiles for op(A) = A and 128 x 288 for
+  // op(A) = A^T, whose rows are separate 128-byte streams per k block (columns
+  // of A): 128 of them measured 6% faster than the 64 x 576 tile at m = 65536,
+  // n = 16384, l = 544, and 192 or 256 of them slower (open DRAM pages and
+  // TLB entries).  The k range is split while the grid is short of the CU
+  // count (keeping at least 8 k blocks per slice).
+  const int64_t bm2 = ta ? 64 * kMT2T : kBM2;
+  const int64_t gx2 = (M + bm2 - 1) / bm2;
   const int64_t gy2 = ((N + 15) / 16 + kBNT2 - 1) / kBNT2;
-  // (op(A) = A only: with op(A) = A^T the 256 rows of a workgroup are 256
-  // separate 128-byte streams per k block, and the 64-row tile keeps the
-  // number of open streams -- and so DRAM pages and TLB entries -- 4x lower)
   int ks = 1;
   while (gx2 * gy2 * ks < cus && KB / (2 * ks) >= 8) ks *= 2;
-  const bool use2 = !ta && gx2 * gy2 * ks >= cus && gx2 * gy2 * ks <= 0x7fffffff;
+  const bool use2 = gx2 * gy2 * ks >= cus && gx2 * gy2 * ks <= 0x7fffffff;
   const int64_t gx = use2 ? gx2 : (M + kBM - 1) / kBM;
   const int64_t gy = use2 ? gy2 : ((N + 15) / 16 + kBNT - 1) / kBNT;
   if (gy > 65535 || gx > 0x7fffffff) return set_error("apass: grid too large");
@@ -717,7 +721,8 @@ int apass_bf16(Context* c, const char* tag, bool ta, int64_t M, int64_t N, int64
     {
       KTimer kt(c, tag);
 #define RSVD_APASS2(TA_, S_, V_)                                                               \
-  hipLaunchKernelGGL((apass2_kernel<TA_, S_, V_>), dim3(blocks), dim3(kT2), 0, c->stream, M, N, KB, \
+  hipLaunchKernelGGL((apass2_kernel<TA_, S_, V_, TA_ ? kMT2T : kMT2>), dim3(blocks), dim3(kT2), 0, \
+                     c->stream, M, N, KB,                                                      \
                      KBper, alpha, A, lda, (const bf16x8_t*)St.p, NTP, beta, C, ldc, Wk.p,        \
                      (int)gxp, (int)gy, ks)
       if (ta) {

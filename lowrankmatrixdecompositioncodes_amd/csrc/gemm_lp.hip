@@ -177,7 +177,7 @@ __global__ void __launch_bounds__(LTHREADS, 2)
 gemm_bf16_kernel(int64_t M, int64_t N, int64_t K, int64_t kchunk, float alpha,
                  const float* __restrict__ A, int64_t lda, const float* __restrict__ B, int64_t ldb,
                  float beta, float* __restrict__ C, int64_t ldc, int64_t zstride, int n_fast,
-                 int aligned, double* __restrict__ sumsq) {
+                 int aligned, double* __restrict__ sumsq, int64_t klo) {
   constexpr int NBUF = lp_planes(SPLIT);  // bf16 planes per operand
   constexpr int LBK = lp_bk(SPLIT), LROW = LBK + LPAD;
   __shared__ __attribute__((aligned(16))) __bf16 As[2][NBUF][LBM * LROW];
@@ -235,6 +235,8 @@ gemm_bf16_kernel(int64_t M, int64_t N, int64_t K, int64_t kchunk, float alpha,
   }
   for (int64_t t = 0; t < nsteps; ++t) {
     const int cur = (int)(t & 1);
+    // SPLIT = 6: k below klo takes the hi * hi product only
+    const bool lo_only = (SPLIT == 6) && (kbeg + t * LBK < klo);
 #pragma unroll
     for (int ks = 0; ks < LBK / 32; ++ks) {
       const int kof = ks * 32 + lk * 8;
@@ -256,12 +258,12 @@ gemm_bf16_kernel(int64_t M, int64_t N, int64_t K, int64_t kchunk, float alpha,
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
           f4v_t d = acc[a][b];
-          if (SPLIT == 6) {  // smallest terms first
+          if (SPLIT == 6 && !lo_only) {  // smallest terms first
             d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[0][a], fa[NBUF - 1][b], d, 0, 0, 0);
             d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[NBUF - 1][a], fa[0][b], d, 0, 0, 0);
             d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[1][a], fa[1][b], d, 0, 0, 0);
           }
-          if (SPLIT >= 3) {
+          if (SPLIT == 3 || (SPLIT == 6 && !lo_only)) {
             d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[0][a], fa[1][b], d, 0, 0, 0);
             d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[1][a], fa[0][b], d, 0, 0, 0);
           }
@@ -342,7 +344,7 @@ int gemm_bf16(Context* c, const char* tag, char ta, char tb, int64_t M, int64_t 
 static int gemm_bf16_tiled(Context* c, const char* tag, char ta, char tb, int64_t M, int64_t N,
                            int64_t K, float alpha, const float* A, int64_t lda, const float* B,
                            int64_t ldb, float beta, float* C, int64_t ldc, int split,
-                           double* sumsq_part) {
+                           double* sumsq_part, int64_t klo = 0) {
   const bool tA = (ta == 'T' || ta == 't'), tB = (tb == 'T' || tb == 't');
   if (split != 1 && split != 3 && split != 6)
     return set_error("gemm_bf16: split must be 1, 3 or 6");
@@ -385,7 +387,7 @@ static int gemm_bf16_tiled(Context* c, const char* tag, char ta, char tb, int64_
 #define RSVD_LP_LAUNCH(TA_, TB_, S_)                                                         \
   hipLaunchKernelGGL((gemm_bf16_kernel<TA_, TB_, S_>), grid, dim3(LTHREADS), 0,        \
                      c->stream, M, N, K, kchunk, alpha, A, lda, B, ldb, b_eff, Cout, ldo,    \
-                     zstride, n_fast, aligned, sumsq_part)
+                     zstride, n_fast, aligned, sumsq_part, klo)
     if (split == 1) {
       if (!tA && !tB) RSVD_LP_LAUNCH(false, false, 1);
       else if (tA && !tB) RSVD_LP_LAUNCH(true, false, 1);
@@ -441,7 +443,8 @@ __global__ void sum_partials_lp_kernel(int64_t n, const double* __restrict__ par
 }
 
 int gemm_bf16_deflate(Context* c, int64_t M, int64_t N, int64_t K, const float* Q, int64_t ldq,
-                      const float* Bt, int64_t ldbt, float* C, int64_t ldc, double* sumsq) {
+                      const float* Bt, int64_t ldbt, float* C, int64_t ldc, double* sumsq,
+                      int64_t klo) {
   if (M < 0 || N < 0 || K < 0) return set_error("deflate: negative dimension");
   if (M == 0 || N == 0) {
     RSVD_HIP(hipMemsetAsync(sumsq, 0, sizeof(double), c->stream));
@@ -451,8 +454,9 @@ int gemm_bf16_deflate(Context* c, int64_t M, int64_t N, int64_t K, const float* 
   const int64_t tiles = ((M + LBM - 1) / LBM) * ((N + LBN - 1) / LBN);
   DBuf<double> part;
   RSVD_TRY(part.alloc((size_t)tiles));
+  if (klo % 32 != 0) return set_error("deflate: klo must be a multiple of 32");
   RSVD_TRY(gemm_bf16_tiled(c, "deflate_bf16x6", 'N', 'T', M, N, K, -1.0f, Q, ldq, Bt, ldbt, 1.0f,
-                           C, ldc, 6, part.p));
+                           C, ldc, 6, part.p, klo));
   hipLaunchKernelGGL(sum_partials_lp_kernel, dim3(1), dim3(256), 0, c->stream, tiles,
                      (const double*)part.p, sumsq);
   RSVD_LAUNCH_CHECK();

@@ -316,7 +316,10 @@ int qb_blocked_mp(Context* c, const float* M, int64_t m, int64_t n, int64_t ldm,
     const float* Qd = Q + (c0 + kstep - nb) * ldq;
     RSVD_TRY(gemm_bf16(c, "proj_bf16x3", 'T', 'N', n, nb, m, 1.0f, R.p, m, Qd, ldq, 0.0f, Bp.p, n,
                        3));
-    RSVD_TRY(gemm_bf16_deflate(c, m, n, nb, Qd, ldq, Bp.p, n, R.p, m, sums.p));
+    // the previous block's correction is ~1e-6 of Bp: its bf16 product is
+    // exact enough, the new block's needs fp32-exact products
+    const int64_t klo = (nb > kstep && kstep % 32 == 0) ? kstep : 0;
+    RSVD_TRY(gemm_bf16_deflate(c, m, n, nb, Qd, ldq, Bp.p, n, R.p, m, sums.p, klo));
     if (Bt) {  // B rows of this block, and the previous block's correction
       RSVD_TRY(copy2d_f32(c, n, kstep, Bp.p + (nb - kstep) * n, n, Bt + c0 * ldbt, ldbt));
       if (step > 0) RSVD_TRY(add2d_f32(c, n, kstep, Bp.p, n, Bt + (c0 - kstep) * ldbt, ldbt));

@@ -163,10 +163,12 @@ int gemm_bf16(Context* c, const char* tag, char ta, char tb, int64_t M, int64_t 
               float alpha, const float* A, int64_t lda, const float* B, int64_t ldb, float beta,
               float* C, int64_t ldc, int split);
 // fp32 C = C - Q Bt^T (Q M x K, Bt N x K) with fp32-exact products (three
-// bf16 planes per operand, six products, fp32 accumulation) and the FP64 sum
-// of squares of the updated C into *sumsq (device scalar, overwritten).
+// bf16 planes per operand, six products, fp32 accumulation) for k >= klo and
+// the bf16 product alone for k < klo (a multiple of 32), and the FP64 sum of
+// squares of the updated C into *sumsq (device scalar, overwritten).
 int gemm_bf16_deflate(Context* c, int64_t M, int64_t N, int64_t K, const float* Q, int64_t ldq,
-                      const float* Bt, int64_t ldbt, float* C, int64_t ldc, double* sumsq);
+                      const float* Bt, int64_t ldbt, float* C, int64_t ldc, double* sumsq,
+                      int64_t klo);
 // C = C - Qp*Bp with fused FP64 sum of squares of the updated C into *sumsq
 // (device scalar, overwritten).
 int gemm_f64_deflate(Context* c, int64_t M, int64_t N, int64_t K, const double* Q, int64_t ldq,
