@@ -49,8 +49,8 @@ TRAFFIC_PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "prof
 
 def measured_traffic(tag, m, n, l):
     """HBM bytes per launch of the split-bf16 A passes timed under `tag` (two
-    NN passes and one TN pass per call), from the committed FETCH_SIZE +
-    WRITE_SIZE counter runs (scaled by the copy-kernel calibration); only for
+    NN passes and one TN pass per call), from the committed rocprofv3 counter
+    runs (DRAM read + write bytes, checked against a 1 GiB copy); only for
     the shape those runs measured, else None."""
     if tag != "apass_bf16x3" or (m, n, l) != (65536, 16384, 544):
         return None
