@@ -11,6 +11,10 @@ namespace rsvd {
 constexpr double kRotTol = 1e-15;
 // a sweep that met no off-diagonal above this relative size ends the iteration
 constexpr double kStopTol = 1e-10;
+// the same for callers that need eigenvectors to fp32 level only: Jacobi
+// converges quadratically, so a sweep whose rotations were all below 1e-8
+// leaves off-diagonals near 1e-16
+constexpr double kLooseStopTol = 1e-8;
 
 // 1/x and 1/sqrt(x) from the hardware estimates plus two Newton steps
 // (error a few ulp; a rotation only needs c^2 + s^2 = 1 to rounding).
@@ -39,7 +43,8 @@ static __device__ __forceinline__ double rsq_nr(double x) {
 // serial path and c^2 + s^2 = (den^2 + 4g^2) / (2 r den) = 1 identically.
 // Evaluated on the entries scaled by a power of two so that the squares stay
 // in range.  Branch-free.  No rotation when |g| <= flo or |g| <= kRotTol
-// sqrt|ab|; *sig reports |g| > kStopTol sqrt|ab|.
+// sqrt|ab|; *sig bit 0 reports |g| > kStopTol sqrt|ab|, bit 1 |g| >
+// kLooseStopTol sqrt|ab| (so *sig != 0 exactly when bit 0 is set).
 static __device__ __forceinline__ void rotation(double a, double b, double g, double flo, double* cs,
                                          double* sn, int* sig) {
   const double ag = fabs(g);
@@ -48,7 +53,8 @@ static __device__ __forceinline__ void rotation(double a, double b, double g, do
   const double sa = ldexp(a, -e), sb = ldexp(b, -e), sg = ldexp(g, -e);
   const double ab = fabs(sa * sb), g2 = sg * sg;
   const bool big = ag > flo;
-  *sig = (big && g2 > kStopTol * kStopTol * ab) ? 1 : 0;
+  *sig = ((big && g2 > kStopTol * kStopTol * ab) ? 1 : 0) |
+         ((big && g2 > kLooseStopTol * kLooseStopTol * ab) ? 2 : 0);
   const bool rot = big && g2 > kRotTol * kRotTol * ab;
   const double d = sb - sa;
   const double r2 = rot ? fma(d, d, 4.0 * g2) : 1.0;

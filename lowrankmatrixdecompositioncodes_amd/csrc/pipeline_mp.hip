@@ -134,7 +134,8 @@ int svd_from_Q_mp(Context* c, const float* A, int64_t m, int64_t n, int64_t lda,
     // V = B^T Uhat Sigma^-1; ascending, last k (rra.c:175-225)
     RSVD_TRY(gemm_f64(c, 'T', 'N', l, l, n, 1.0, Bt64.p, n, Bt64.p, n, 0.0, G.p, l));
     RSVD_TRY(symmetrize_upper_f64(c, l, G.p));
-    RSVD_TRY(symeig_jacobi_f64(c, l, G.p, l, Uh.p, l, lam.p, 60, sw));
+    // U and V leave as fp32: the looser sweep test is enough
+    RSVD_TRY(symeig_jacobi_f64(c, l, G.p, l, Uh.p, l, lam.p, 60, sw, 1));
     RSVD_TRY(sort_pairs_f64(c, l, lam.p, 0, Uh.p, l, l, nullptr, 0, 0));
     RSVD_TRY(sqrt_f64(c, l, lam.p, sig.p));
     const int64_t off = l - k;

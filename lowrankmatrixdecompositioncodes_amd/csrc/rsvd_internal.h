@@ -254,9 +254,11 @@ int sketch_features(Context* c, const float* A, int64_t m, int64_t n, int64_t ld
 // Symmetric eigensolver (two-sided block Jacobi).  G (n x n, ld ldg, both
 // triangles, symmetric) is diagonalised in place: on return G ~ diag(lambda)
 // and V (optional, n x n) holds the eigenvectors (V^T G_in V = diag);
-// evals (optional) = diag(G), unsorted.  *sweeps_out = sweeps used.
+// evals (optional) = diag(G), unsorted.  *sweeps_out = sweeps used.  The
+// iteration ends after the first sweep whose rotations were all below 1e-10
+// of sqrt(g_pp g_qq), or 1e-8 with loose = 1 (eigenvectors to fp32 level).
 int symeig_jacobi_f64(Context* c, int64_t n, double* G, int64_t ldg, double* V, int64_t ldv,
-                      double* evals, int max_sweeps, int* sweeps_out);
+                      double* evals, int max_sweeps, int* sweeps_out, int loose = 0);
 // Column norms of W into nrm[n]; optional normalisation of the columns.
 int colnorms_f64(Context* c, int64_t L, int64_t n, double* W, int64_t ldw, double* nrm,
                  int normalise);

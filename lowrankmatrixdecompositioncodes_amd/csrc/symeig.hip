@@ -133,7 +133,8 @@ __device__ double* jac_inner(double* Ha, double* Hb, double* Jl, double* rc, dou
 }
 
 // J and the final tile H of pair w out (column-major S x S each); the flag
-// records whether any rotation was above kStopTol.
+// collects the rotation-size bits of rotation() (bit 0: above kStopTol, bit
+// 1: above kLooseStopTol).
 template <int B>
 __device__ void jac_out(const double* Hf, const double* Jl, int w, double* __restrict__ Jout,
                         double* __restrict__ Hout, int any_sig,
@@ -147,7 +148,9 @@ __device__ void jac_out(const double* Hf, const double* Jl, int w, double* __res
     Jw[i + j * S] = Jl[i * HP + j];
     Hw[i + j * S] = Hf[i * HP + j];
   }
-  if (__any(any_sig) && (tid & 63) == 0) atomicOr(flag, 1ULL);
+  const unsigned long long m =
+      (__any(any_sig & 1) ? 1ULL : 0ULL) | (__any(any_sig & 2) ? 2ULL : 0ULL);
+  if (m && (tid & 63) == 0) atomicOr(flag, m);
 }
 
 // First round of the iteration: the pair tiles are read from G.
@@ -443,7 +446,7 @@ __global__ void symj_diag_kernel(int64_t n, const double* __restrict__ G, int64_
 
 template <int B>
 int run_sweeps(Context* c, int64_t n, double* G, int64_t ldg, double* V, int64_t ldv,
-               int max_sweeps, int* sweep_out) {
+               int max_sweeps, int loose, int* sweep_out) {
   constexpr int S = 2 * B;
   const int nblocks = (int)((n + B - 1) / B);
   const int N = nblocks + (nblocks & 1);
@@ -491,7 +494,8 @@ int run_sweeps(Context* c, int64_t n, double* G, int64_t ldg, double* V, int64_t
       unsigned long long h = 0;
       RSVD_HIP(hipMemcpyAsync(&h, flags.p + sweep, sizeof(h), hipMemcpyDeviceToHost, c->stream));
       RSVD_HIP(hipStreamSynchronize(c->stream));
-      if (h == 0) {  // no pair met in this sweep was above kStopTol
+      // no pair met in this sweep was above kStopTol (kLooseStopTol)
+      if ((h & (loose ? 2ULL : 1ULL)) == 0) {
         ++sweep;
         break;
       }
@@ -507,7 +511,7 @@ int run_sweeps(Context* c, int64_t n, double* G, int64_t ldg, double* V, int64_t
 }  // namespace
 
 int symeig_jacobi_f64(Context* c, int64_t n, double* G, int64_t ldg, double* V, int64_t ldv,
-                      double* evals, int max_sweeps, int* sweeps_out) {
+                      double* evals, int max_sweeps, int* sweeps_out, int loose) {
   if (sweeps_out) *sweeps_out = 0;
   if (n <= 0) return 0;
   if (ldg < n || (V && ldv < n)) return set_error("symeig: leading dimension < n");
@@ -520,7 +524,7 @@ int symeig_jacobi_f64(Context* c, int64_t n, double* G, int64_t ldg, double* V, 
   int sweep = 0;
   // B = 16: each thread owns one 2x2 block and computes two rotations per
   // step; a larger B trades launches for a slower, LDS-bound inner sweep
-  const int rc = run_sweeps<16>(c, n, G, ldg, V, ldv, max_sweeps, &sweep);
+  const int rc = run_sweeps<16>(c, n, G, ldg, V, ldv, max_sweeps, loose, &sweep);
   if (rc) return rc;
   if (evals) {
     hipLaunchKernelGGL(symj_diag_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
