@@ -24,9 +24,10 @@
 namespace rsvd {
 
 // Y (m x n, fp32) <- orthonormal basis of its range.  passes = 1 (CholeskyQR)
-// or 2 (CholeskyQR2).  precise = 1 forms the Gram on the fp32 matrix cores
-// and Y R^-1 with fp32-exact products (three bf16 planes per operand), else
-// both in split-bf16.  Falls back to FP64 Householder on a Cholesky breakdown.
+// or 2 (CholeskyQR2).  precise = 1 forms the last pass's Gram on the fp32
+// matrix cores and its Y R^-1 with fp32-exact products (three bf16 planes per
+// operand); every other product is split-bf16 (the last pass corrects the
+// earlier ones).  Falls back to FP64 Householder on a Cholesky breakdown.
 static int orth_f32(Context* c, int64_t m, int64_t n, float* Y, int64_t ldy, int passes,
                     int precise, int* fallback) {
   DBuf<float> G32, W, Rinv32;
@@ -42,7 +43,10 @@ static int orth_f32(Context* c, int64_t m, int64_t n, float* Y, int64_t ldy, int
   float* src = Y;
   int64_t lds = ldy;
   for (int pass = 0; pass < passes; ++pass) {
-    if (precise)
+    // the last pass fixes what the ones before it leave: only its products
+    // need to be exact
+    const bool exact = precise && pass + 1 == passes;
+    if (exact)
       RSVD_TRY(gemm_f32(c, 'T', 'N', n, n, m, 1.0f, src, lds, src, lds, 0.0f, G32.p, n));
     else
       RSVD_TRY(gemm_bf16(c, "gram_bf16x3", 'T', 'N', n, n, m, 1.0f, src, lds, src, lds, 0.0f,
@@ -63,7 +67,7 @@ static int orth_f32(Context* c, int64_t m, int64_t n, float* Y, int64_t ldy, int
     RSVD_TRY(f64_to_f32(c, n, n, Rinv64.p, n, Rinv32.p, n));
     float* dst = (src == Y) ? W.p : Y;
     const int64_t ldd = (src == Y) ? m : ldy;
-    if (precise)
+    if (exact)
       RSVD_TRY(gemm_bf16_general(c, "orth_bf16x6", 'N', 'N', m, n, n, 1.0f, src, lds, Rinv32.p, n,
                                  0.0f, dst, ldd, 6));
     else
