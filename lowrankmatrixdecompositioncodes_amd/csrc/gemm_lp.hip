@@ -21,8 +21,8 @@
 // three bf16 planes x = hi + mid + lo, which hold all 24 bits of an fp32
 // value, and evaluates hi*hi + hi*mid + mid*hi + mid*mid + hi*lo + lo*hi (the
 // dropped terms are below 2^-24 of the product): fp32-exact products,
-// accumulated in fp32, at 6 bf16 MFMAs, with a K step of 32 so that the three
-// planes of both operands fit LDS twice over.
+// accumulated in fp32, at 6 bf16 MFMAs, with a K step of 32 and one LDS stage
+// (61 KB: two workgroups per CU).
 //
 // Workgroup: 256 threads, 2x2 waves, tile 128 x 128, K step 64.  The grid is
 // ordered so that the tiles sharing a panel of the large operand are launched
@@ -180,8 +180,11 @@ gemm_bf16_kernel(int64_t M, int64_t N, int64_t K, int64_t kchunk, float alpha,
                  int aligned, double* __restrict__ sumsq, int64_t klo) {
   constexpr int NBUF = lp_planes(SPLIT);  // bf16 planes per operand
   constexpr int LBK = lp_bk(SPLIT), LROW = LBK + LPAD;
-  __shared__ __attribute__((aligned(16))) __bf16 As[2][NBUF][LBM * LROW];
-  __shared__ __attribute__((aligned(16))) __bf16 Bs[2][NBUF][LBN * LROW];
+  // LDS stages: two, or one for the three-plane split (61 KB, so that two
+  // workgroups share a CU and hide each other's staging and epilogue)
+  constexpr int NST = (SPLIT == 6) ? 1 : 2;
+  __shared__ __attribute__((aligned(16))) __bf16 As[NST][NBUF][LBM * LROW];
+  __shared__ __attribute__((aligned(16))) __bf16 Bs[NST][NBUF][LBN * LROW];
 
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
@@ -234,7 +237,7 @@ gemm_bf16_kernel(int64_t M, int64_t N, int64_t K, int64_t kchunk, float alpha,
     __syncthreads();
   }
   for (int64_t t = 0; t < nsteps; ++t) {
-    const int cur = (int)(t & 1);
+    const int cur = (NST == 2) ? (int)(t & 1) : 0;
     // SPLIT = 6: k below klo takes the hi * hi product only
     const bool lo_only = (SPLIT == 6) && (kbeg + t * LBK < klo);
 #pragma unroll
@@ -270,13 +273,23 @@ gemm_bf16_kernel(int64_t M, int64_t N, int64_t K, int64_t kchunk, float alpha,
           acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[0][a], fa[0][b], d, 0, 0, 0);
         }
     }
-    if (t + 1 < nsteps) {
-      const int nxt = cur ^ 1;
-      sa.store(As[nxt][0], tid);
-      sb.store(Bs[nxt][0], tid);
-      if (t + 2 < nsteps) load_step(t + 2);
+    if constexpr (NST == 2) {
+      if (t + 1 < nsteps) {
+        const int nxt = cur ^ 1;
+        sa.store(As[nxt][0], tid);
+        sb.store(Bs[nxt][0], tid);
+        if (t + 2 < nsteps) load_step(t + 2);
+      }
+      __syncthreads();
+    } else {
+      __syncthreads();  // every wave is done with the stage
+      if (t + 1 < nsteps) {
+        sa.store(As[0][0], tid);
+        sb.store(Bs[0][0], tid);
+        if (t + 2 < nsteps) load_step(t + 2);
+      }
+      __syncthreads();
     }
-    __syncthreads();
   }
 
   double ss = 0.0;
