@@ -21,8 +21,7 @@
 // three bf16 planes x = hi + mid + lo, which hold all 24 bits of an fp32
 // value, and evaluates hi*hi + hi*mid + mid*hi + mid*mid + hi*lo + lo*hi (the
 // dropped terms are below 2^-24 of the product): fp32-exact products,
-// accumulated in fp32, at 6 bf16 MFMAs, with a K step of 32 and one LDS stage
-// (61 KB: two workgroups per CU).
+// accumulated in fp32, at 6 bf16 MFMAs, with a K step of 32.
 //
 // Workgroup: 256 threads, 2x2 waves, tile 128 x 128, K step 64.  The grid is
 // ordered so that the tiles sharing a panel of the large operand are launched
@@ -180,9 +179,11 @@ gemm_bf16_kernel(int64_t M, int64_t N, int64_t K, int64_t kchunk, float alpha,
                  int aligned, double* __restrict__ sumsq, int64_t klo) {
   constexpr int NBUF = lp_planes(SPLIT);  // bf16 planes per operand
   constexpr int LBK = lp_bk(SPLIT), LROW = LBK + LPAD;
-  // LDS stages: two, or one for the three-plane split (61 KB, so that two
-  // workgroups share a CU and hide each other's staging and epilogue)
-  constexpr int NST = (SPLIT == 6) ? 1 : 2;
+  // LDS stages: one for the split modes (74 KB with two planes, 61 KB with
+  // three), so that two workgroups share a CU and hide each other's staging
+  // and epilogue (measured faster than two stages at one workgroup per CU);
+  // two for plain bf16, which fits twice either way
+  constexpr int NST = (SPLIT == 1) ? 2 : 1;
   __shared__ __attribute__((aligned(16))) __bf16 As[NST][NBUF][LBM * LROW];
   __shared__ __attribute__((aligned(16))) __bf16 Bs[NST][NBUF][LBN * LROW];
 
