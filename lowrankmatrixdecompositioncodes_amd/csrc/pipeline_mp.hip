@@ -23,15 +23,36 @@
 
 namespace rsvd {
 
+// *out = max_ij |G_ij - delta_ij| of an n x n FP64 matrix (one workgroup).
+__global__ void max_dev_identity_kernel(int64_t n, const double* __restrict__ G,
+                                        double* __restrict__ out) {
+  __shared__ double red[256];
+  double mx = 0.0;
+  for (int64_t e = threadIdx.x; e < n * n; e += 256) {
+    const int64_t i = e % n, j = e / n;
+    mx = fmax(mx, fabs(G[e] - (i == j ? 1.0 : 0.0)));
+  }
+  red[threadIdx.x] = mx;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = red[0];
+}
+
 // Y (m x n, fp32) <- orthonormal basis of its range.  passes = 1 (CholeskyQR)
 // or 2 (CholeskyQR2).  precise = 1 forms the last pass's Gram on the fp32
 // matrix cores and its Y R^-1 with fp32-exact products (three bf16 planes per
 // operand); every other product is split-bf16 (the last pass corrects the
-// earlier ones).  Falls back to FP64 Householder on a Cholesky breakdown.
+// earlier ones).  The last pass's Gram Q1^T Q1 is checked: if it is more than
+// 0.25 from the identity (the split-bf16 first pass met a basis too
+// ill-conditioned for its 16-bit products), one more exact pass follows, at
+// most two.  Falls back to FP64 Householder on a Cholesky breakdown.
 static int orth_f32(Context* c, int64_t m, int64_t n, float* Y, int64_t ldy, int passes,
                     int precise, int* fallback) {
   DBuf<float> G32, W, Rinv32;
-  DBuf<double> G64, R64, Rinv64, diag0;
+  DBuf<double> G64, R64, Rinv64, diag0, dev;
   DBuf<int> fail;
   RSVD_TRY(G32.alloc((size_t)(n * n)));
   RSVD_TRY(W.alloc((size_t)(m * n)));
@@ -42,16 +63,27 @@ static int orth_f32(Context* c, int64_t m, int64_t n, float* Y, int64_t ldy, int
   RSVD_TRY(fail.alloc(1));
   float* src = Y;
   int64_t lds = ldy;
-  for (int pass = 0; pass < passes; ++pass) {
+  int total = passes;
+  for (int pass = 0; pass < total; ++pass) {
     // the last pass fixes what the ones before it leave: only its products
     // need to be exact
-    const bool exact = precise && pass + 1 == passes;
+    const bool exact = precise && pass + 1 >= passes;
     if (exact)
       RSVD_TRY(gemm_f32(c, 'T', 'N', n, n, m, 1.0f, src, lds, src, lds, 0.0f, G32.p, n));
     else
       RSVD_TRY(gemm_bf16(c, "gram_bf16x3", 'T', 'N', n, n, m, 1.0f, src, lds, src, lds, 0.0f,
                          G32.p, n, 3));
     RSVD_TRY(f32_to_f64(c, n, n, G32.p, n, G64.p, n));
+    if (exact && pass > 0 && pass + 1 == total && total < passes + 2) {
+      if (!dev.p) RSVD_TRY(dev.alloc(1));
+      hipLaunchKernelGGL(max_dev_identity_kernel, dim3(1), dim3(256), 0, c->stream, n,
+                         (const double*)G64.p, dev.p);
+      RSVD_LAUNCH_CHECK();
+      double h = 0.0;
+      RSVD_HIP(hipMemcpyAsync(&h, dev.p, sizeof h, hipMemcpyDeviceToHost, c->stream));
+      RSVD_HIP(hipStreamSynchronize(c->stream));
+      if (!(h <= 0.25)) ++total;  // (NaN included)
+    }
     int ok = 1;
     RSVD_TRY(chol_inv_upper_f64(c, n, G64.p, R64.p, Rinv64.p, &ok));
     if (!ok) {

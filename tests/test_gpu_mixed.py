@@ -314,3 +314,36 @@ def test_gemm_bf16x6_is_fp32_exact(gpu_lib, ta, tb, M, N, K):
     # fp32 accumulation over K / 32 MFMA steps plus the final subtraction
     assert errs[5] <= (K / 32 + 8) * 2.0**-24
     assert errs[3] > 4 * errs[5]
+
+
+@pytest.mark.gpu
+def test_fixed_rank_mp_ill_conditioned_basis(gpu_lib):
+    """A steep spectrum, sigma_j = 10^(-3 j / 600) (the sketch basis Y = A Z
+    has condition ~1e3, past what a split-bf16 CholeskyQR pass resolves): the
+    final CholeskyQR2 falls back to an extra exact pass or to Householder, and
+    the result still matches the FP64 path to fp32 level (relative to sigma_1)
+    with orthonormal factors."""
+    rp = gpu_lib
+    lib = rp.load()
+    m, n, r, k, p, q, seed = 8192, 2048, 600, 128, 16, 2, 777
+    sigma = 10.0 ** (-3.0 * np.arange(r) / r)
+    A64 = rp.DeviceMatrix(m, n, np.float64)
+    A32 = rp.DeviceMatrix(m, n, np.float32)
+    assert lib.rsvd_dev_lowrank_test_matrix(m, n, r, sigma.ctypes.data, 0.0, 5, A64.ptr, m,
+                                            A32.ptr, m) == 0, rp.last_error()
+    om = rp.gaussian(seed, 0, n * (k + p)).astype(np.float32).astype(np.float64)
+    dOm = rp.DeviceMatrix.from_host(om.reshape(k + p, n).T)
+    U64, S64, V64 = rp.DeviceMatrix(m, k), rp.DeviceMatrix(k, 1), rp.DeviceMatrix(n, k)
+    assert lib.rsvd_dev_fixed_rank(A64.ptr, m, n, m, k, p, 2, q, 1, seed, dOm.ptr, U64.ptr, m,
+                                   S64.ptr, V64.ptr, n) == 0, rp.last_error()
+    Um, Sm, Vm = (rp.DeviceMatrix(m, k, np.float32), rp.DeviceMatrix(k, 1),
+                  rp.DeviceMatrix(n, k, np.float32))
+    assert lib.rsvd_dev_fixed_rank_mp(A32.ptr, m, n, m, k, p, 2, q, 1, seed, None, 2, Um.ptr, m,
+                                      Sm.ptr, Vm.ptr, n) == 0, rp.last_error()
+    s64, sm = S64.to_host()[:, 0], Sm.to_host()[:, 0]
+    U, V = Um.to_host().astype(np.float64), Vm.to_host().astype(np.float64)
+    ortho = max(np.abs(U.T @ U - np.eye(k)).max(), np.abs(V.T @ V - np.eye(k)).max())
+    err = np.abs(sm - s64).max() / s64.max()
+    print(f"steep spectrum: max |dsigma| / sigma_1 {err:.2e}, ortho {ortho:.1e}")
+    assert err <= 1e-6
+    assert ortho <= 5e-5
