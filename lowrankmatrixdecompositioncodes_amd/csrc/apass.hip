@@ -304,16 +304,43 @@ apass2_kernel(int64_t M, int64_t N, int64_t KBtot, int64_t KBper, float alpha,
 
   // ---- S stage loads: chunk c = (plane c / 18, tile c % 18), wave-strided;
   // the padding slots reload chunk 0 into Sdummy so every wave issues NLD
+  // per-wave chunk offsets, fixed over the k loop (wave-uniform): source
+  // element offset at k block 0 and the chunk's slot within an LDS stage
+  // (padding slots go to Sdummy).  Used for op(A) = A^T, where it removes
+  // most of the scalar work per k block; the op(A) = A variant, at the
+  // register limit, spills with it and recomputes them instead.
+  int64_t sbase[NLD];
+  int soff[NLD];
+  bool sreal[NLD];
+#pragma unroll
+  for (int q = 0; q < NLD; ++q) {
+    const int c = wave + kW2 * q;
+    sreal[q] = c < NCH;
+    const int pl = sreal[q] ? c / kBNT2 : 0, t = sreal[q] ? c % kBNT2 : 0;
+    sbase[q] = ((int64_t)pl * KBtot * NTP + ntw + t) * 64;
+    soff[q] = pl * kBNT2 + t;
+  }
+  const int64_t kstride = NTP * 64;  // St elements per k block
   auto loadS = [&](int64_t kb, bf16x8_t (*dst)[kBNT2][64]) {
     kb = kb < klast ? kb : klast;
+    if constexpr (TA) {
+      const int64_t koff = kb * kstride;
 #pragma unroll
-    for (int q = 0; q < NLD; ++q) {
-      const int c = wave + kW2 * q;
-      const bool real = c < NCH;
-      const int pl = real ? c / kBNT2 : 0, t = real ? c % kBNT2 : 0;
-      const bf16x8_t* g = St + ((pl * KBtot + kb) * NTP + ntw + t) * 64 + lane;
-      lds_ptr_t d = real ? (lds_ptr_t)&dst[pl][t][0] : (lds_ptr_t)&Sdummy[0];
-      __builtin_amdgcn_global_load_lds((gbl_ptr_t)g, d, 16, 0, 0);
+      for (int q = 0; q < NLD; ++q) {
+        lds_ptr_t d = sreal[q] ? (lds_ptr_t)&dst[0][soff[q]][0] : (lds_ptr_t)&Sdummy[0];
+        __builtin_amdgcn_global_load_lds((gbl_ptr_t)(St + (sbase[q] + koff) + lane), d, 16, 0,
+                                         0);
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < NLD; ++q) {
+        const int c = wave + kW2 * q;
+        const bool real = c < NCH;
+        const int pl = real ? c / kBNT2 : 0, t = real ? c % kBNT2 : 0;
+        const bf16x8_t* g = St + ((pl * KBtot + kb) * NTP + ntw + t) * 64 + lane;
+        lds_ptr_t d = real ? (lds_ptr_t)&dst[pl][t][0] : (lds_ptr_t)&Sdummy[0];
+        __builtin_amdgcn_global_load_lds((gbl_ptr_t)g, d, 16, 0, 0);
+      }
     }
   };
 
