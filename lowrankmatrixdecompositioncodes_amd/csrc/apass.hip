@@ -26,6 +26,8 @@
 // fragments (the k loop is unrolled by two so the buffers swap by name).
 #include <hip/hip_bf16.h>
 
+#include <cmath>
+
 #include "rsvd_internal.h"
 
 namespace rsvd {
@@ -772,8 +774,22 @@ int apass_bf16(Context* c, const char* tag, bool ta, int64_t M, int64_t N, int64
   const int64_t bm2 = ta ? 64 * kMT2T : kBM2;
   const int64_t gx2 = (M + bm2 - 1) / bm2;
   const int64_t gy2 = ((N + 15) / 16 + kBNT2 - 1) / kBNT2;
+  // k slices: one workgroup per CU runs at a time, so the grid runs in
+  // ceil(gx gy ks / cus) waves; take the smallest ks (at least 8 k blocks a
+  // slice) whose waves are within 3% of the best fill, e.g. 314 workgroups
+  // (61% fill of the last of two waves) -> ks = 3 (942: 92%)
   int ks = 1;
-  while (gx2 * gy2 * ks < cus && KB / (2 * ks) >= 8) ks *= 2;
+  {
+    double best = 0.0;
+    double fill[9] = {0.0};
+    for (int t = 1; t <= 8 && KB / t >= 8; ++t) {
+      const double w = (double)(gx2 * gy2 * t) / (double)cus;
+      fill[t] = w / std::ceil(w);
+      if (fill[t] > best) best = fill[t];
+    }
+    for (int t = 1; t <= 8 && KB / t >= 8; ++t)
+      if (fill[t] >= best - 0.03) { ks = t; break; }
+  }
   const bool use2 = gx2 * gy2 * ks >= cus && gx2 * gy2 * ks <= 0x7fffffff;
   const int64_t gx = use2 ? gx2 : (M + kBM - 1) / kBM;
   const int64_t gy = use2 ? gy2 : ((N + 15) / 16 + kBNT - 1) / kBNT;
