@@ -224,11 +224,9 @@ static int launch_gemm(Context* c, char ta, char tb, int64_t M, int64_t N, int64
   const int64_t tiles = gx * gy;
   const int64_t target = 2LL * (c->num_cus > 0 ? c->num_cus : 256);
   if (!sumsq_partials && tiles < target && K >= 4 * 256) {
-    int64_t s = (target + tiles - 1) / tiles;
     int64_t maxs = K / 256;
-    if (s > maxs) s = maxs;
-    if (s > 64) s = 64;
-    if (s > 1) splits = (int)s;
+    if (maxs > 64) maxs = 64;
+    splits = pick_k_slices(tiles, target, maxs);
   }
   int64_t kchunk = K;
   if (splits > 1) {

@@ -370,10 +370,9 @@ static int gemm_bf16_tiled(Context* c, const char* tag, char ta, char tb, int64_
   const int64_t tiles = gm * gn;
   const int64_t target = 2LL * (c->num_cus > 0 ? c->num_cus : 256);
   if (!sumsq_part && tiles < target && K >= 8 * 256) {
-    int64_t s = (target + tiles - 1) / tiles, maxs = K / 512;
-    if (s > maxs) s = maxs;
-    if (s > 32) s = 32;
-    if (s > 1) splits = (int)s;
+    int64_t maxs = K / 512;
+    if (maxs > 32) maxs = 32;
+    splits = pick_k_slices(tiles, target, maxs);  // two workgroups per CU
   }
   int64_t kchunk = K;
   if (splits > 1) {

@@ -119,6 +119,22 @@ struct DMat {
   __host__ __device__ T* col(int64_t j) const { return p + j * ld; }
 };
 
+// k slices for a grid of `tiles` workgroups when `slots` of them run at once:
+// the s in [1, smax] that minimises ceil(tiles s / slots) / s (waves per unit
+// of work), the fewest slices among equals.
+static inline int pick_k_slices(int64_t tiles, int64_t slots, int64_t smax) {
+  int best = 1;
+  double best_cost = 1e300;
+  for (int64_t t = 1; t <= smax; ++t) {
+    const double cost = (double)((tiles * t + slots - 1) / slots) / (double)t;
+    if (cost < best_cost * (1.0 - 1e-9)) {
+      best_cost = cost;
+      best = (int)t;
+    }
+  }
+  return best;
+}
+
 // ---------------------------------------------------------------- kernels
 // GEMM: C = alpha * op(A) * op(B) + beta * C, column-major, double or float.
 // ta/tb: 'N' or 'T'.  sumsq (optional, double[grid]) receives per-workgroup
