@@ -55,10 +55,17 @@ typedef __bf16 bf16x4_t __attribute__((ext_vector_type(4)));
 
 // Four consecutive k of one row into the NP planes at offset `off` of each
 // (plane stride `ps` elements): plane 0 = bf16(x), plane 1 = bf16(x - p0),
-// plane 2 = bf16(x - p0 - p1).  The differences are exact in fp32.
+// plane 2 = bf16(x - p0 - p1).  The differences are exact in fp32.  With
+// hi_only (uniform) only plane 0 is formed and written.
 template <int NP>
-__device__ __forceinline__ void put4(__bf16* base, int ps, int off, f4s_t x) {
+__device__ __forceinline__ void put4(__bf16* base, int ps, int off, f4s_t x, bool hi_only) {
   bf16x4_t h, m, l;
+  if (NP > 1 && hi_only) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) h[e] = (__bf16)x[e];
+    *reinterpret_cast<bf16x4_t*>(base + off) = h;
+    return;
+  }
 #pragma unroll
   for (int e = 0; e < 4; ++e) {
     const __bf16 a = (__bf16)x[e];
@@ -74,6 +81,13 @@ __device__ __forceinline__ void put4(__bf16* base, int ps, int off, f4s_t x) {
   if (NP > 1) *reinterpret_cast<bf16x4_t*>(base + ps + off) = m;
   if (NP > 2) *reinterpret_cast<bf16x4_t*>(base + 2 * ps + off) = l;
 }
+
+// Row-contiguous operands are staged with the 16-byte k chunk c of LDS row r
+// stored at chunk c ^ lds_swz(r).  The stager writes 4-row x 4-k micro-tiles
+// (rows four apart across lanes), which at the padded row pitch alone put
+// four lanes on one bank; the XOR spreads them over all banks and keeps the
+// 16-byte fragment reads conflict-free.
+__device__ __forceinline__ int lds_swz(int r) { return (r >> 3) & 3; }
 
 template <bool KCONT, int SPLIT>
 struct Stager;
@@ -104,12 +118,12 @@ struct Stager<true, SPLIT> {
     }
   }
   // planes of LBM * LROW elements each, contiguous from `base`
-  __device__ __forceinline__ void store(__bf16* __restrict__ base, int tid) {
+  __device__ __forceinline__ void store(__bf16* __restrict__ base, int tid, bool hi_only) {
 #pragma unroll
     for (int it = 0; it < ITEMS; ++it) {
       const int f = tid + it * LTHREADS;
       const int rr = f / (LBK / 4), kq = f % (LBK / 4);
-      put4<NP>(base, LBM * LROW, rr * LROW + kq * 4, v[it]);
+      put4<NP>(base, LBM * LROW, rr * LROW + kq * 4, v[it], hi_only);
     }
   }
 };
@@ -150,7 +164,7 @@ struct Stager<false, SPLIT> {
       }
     }
   }
-  __device__ __forceinline__ void store(__bf16* __restrict__ base, int tid) {
+  __device__ __forceinline__ void store(__bf16* __restrict__ base, int tid, bool hi_only) {
 #pragma unroll
     for (int it = 0; it < ITEMS; ++it) {
       int rq, kq;
@@ -159,7 +173,8 @@ struct Stager<false, SPLIT> {
       for (int e = 0; e < 4; ++e) {
         const f4s_t row = {v[it][0][e], v[it][1][e], v[it][2][e], v[it][3][e]};
         const int r = 4 * rq + e;
-        put4<NP>(base, LBM * LROW, r * LROW + 4 * kq, row);
+        const int off = r * LROW + ((((kq >> 1) ^ lds_swz(r)) << 3) | ((kq & 1) << 2));
+        put4<NP>(base, LBM * LROW, off, row, hi_only);
       }
     }
   }
@@ -216,6 +231,14 @@ gemm_bf16_kernel(int64_t M, int64_t N, int64_t K, int64_t kchunk, float alpha,
   // op(B)(k, n): TB -> B[n + k*ldb] (n-contiguous) ; !TB -> B[k + n*ldb] (k-contiguous)
   Stager<TA, SPLIT> sa;
   Stager<!TB, SPLIT> sb;
+  // LDS offset of fragment (row, k chunk at kof) of op(A) / op(B): the
+  // operands staged row-contiguously (!TA, TB) are chunk-swizzled
+  auto offA = [&](int row, int kof) {
+    return row * LROW + (TA ? kof : (((kof >> 3) ^ lds_swz(row)) << 3));
+  };
+  auto offB = [&](int row, int kof) {
+    return row * LROW + (!TB ? kof : (((kof >> 3) ^ lds_swz(row)) << 3));
+  };
 
   f4v_t acc[4][4];
 #pragma unroll
@@ -230,63 +253,83 @@ gemm_bf16_kernel(int64_t M, int64_t N, int64_t K, int64_t kchunk, float alpha,
     sa.load(A, lda, m0, M, k0, kend, tid, full);
     sb.load(B, ldb, n0, N, k0, kend, tid, full);
   };
+  // SPLIT = 6: a K step below klo takes the hi * hi product only, so only
+  // plane 0 is staged and read for it
+  auto hi_step = [&](int64_t t) { return SPLIT == 6 && kbeg + t * LBK < klo; };
   if (nsteps > 0) {
     load_step(0);
-    sa.store(As[0][0], tid);
-    sb.store(Bs[0][0], tid);
+    sa.store(As[0][0], tid, hi_step(0));
+    sb.store(Bs[0][0], tid, hi_step(0));
     if (nsteps > 1) load_step(1);
     __syncthreads();
   }
   for (int64_t t = 0; t < nsteps; ++t) {
     const int cur = (NST == 2) ? (int)(t & 1) : 0;
-    // SPLIT = 6: k below klo takes the hi * hi product only
-    const bool lo_only = (SPLIT == 6) && (kbeg + t * LBK < klo);
+    if (hi_step(t)) {
 #pragma unroll
-    for (int ks = 0; ks < LBK / 32; ++ks) {
-      const int kof = ks * 32 + lk * 8;
-      // fragments of plane P: fb[P][a] from op(B), fa[P][b] from op(A)
-      bf16x8_t fa[NBUF][4], fb[NBUF][4];
-#pragma unroll
-      for (int P = 0; P < NBUF; ++P) {
+      for (int ks = 0; ks < LBK / 32; ++ks) {
+        const int kof = ks * 32 + lk * 8;
+        bf16x8_t fa[4], fb[4];
 #pragma unroll
         for (int a = 0; a < 4; ++a)
-          fb[P][a] =
-              *reinterpret_cast<const bf16x8_t*>(&Bs[cur][P][(wn0 + a * 16 + li) * LROW + kof]);
+          fb[a] = *reinterpret_cast<const bf16x8_t*>(&Bs[cur][0][offB(wn0 + a * 16 + li, kof)]);
 #pragma unroll
         for (int b = 0; b < 4; ++b)
-          fa[P][b] =
-              *reinterpret_cast<const bf16x8_t*>(&As[cur][P][(wm0 + b * 16 + li) * LROW + kof]);
+          fa[b] = *reinterpret_cast<const bf16x8_t*>(&As[cur][0][offA(wm0 + b * 16 + li, kof)]);
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+          for (int b = 0; b < 4; ++b)
+            acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[a], fa[b], acc[a][b], 0, 0, 0);
       }
+    } else {
 #pragma unroll
-      for (int a = 0; a < 4; ++a)
+      for (int ks = 0; ks < LBK / 32; ++ks) {
+        const int kof = ks * 32 + lk * 8;
+        // fragments of plane P: fb[P][a] from op(B), fa[P][b] from op(A)
+        bf16x8_t fa[NBUF][4], fb[NBUF][4];
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
-          f4v_t d = acc[a][b];
-          if (SPLIT == 6 && !lo_only) {  // smallest terms first
-            d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[0][a], fa[NBUF - 1][b], d, 0, 0, 0);
-            d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[NBUF - 1][a], fa[0][b], d, 0, 0, 0);
-            d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[1][a], fa[1][b], d, 0, 0, 0);
-          }
-          if (SPLIT == 3 || (SPLIT == 6 && !lo_only)) {
-            d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[0][a], fa[1][b], d, 0, 0, 0);
-            d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[1][a], fa[0][b], d, 0, 0, 0);
-          }
-          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[0][a], fa[0][b], d, 0, 0, 0);
+        for (int P = 0; P < NBUF; ++P) {
+#pragma unroll
+          for (int a = 0; a < 4; ++a)
+            fb[P][a] =
+                *reinterpret_cast<const bf16x8_t*>(&Bs[cur][P][offB(wn0 + a * 16 + li, kof)]);
+#pragma unroll
+          for (int b = 0; b < 4; ++b)
+            fa[P][b] =
+                *reinterpret_cast<const bf16x8_t*>(&As[cur][P][offA(wm0 + b * 16 + li, kof)]);
         }
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+          for (int b = 0; b < 4; ++b) {
+            f4v_t d = acc[a][b];
+            if (SPLIT == 6) {  // smallest terms first
+              d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[0][a], fa[NBUF - 1][b], d, 0, 0, 0);
+              d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[NBUF - 1][a], fa[0][b], d, 0, 0, 0);
+              d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[1][a], fa[1][b], d, 0, 0, 0);
+            }
+            if (SPLIT == 3 || SPLIT == 6) {
+              d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[0][a], fa[1][b], d, 0, 0, 0);
+              d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[1][a], fa[0][b], d, 0, 0, 0);
+            }
+            acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[0][a], fa[0][b], d, 0, 0, 0);
+          }
+      }
     }
     if constexpr (NST == 2) {
       if (t + 1 < nsteps) {
         const int nxt = cur ^ 1;
-        sa.store(As[nxt][0], tid);
-        sb.store(Bs[nxt][0], tid);
+        sa.store(As[nxt][0], tid, hi_step(t + 1));
+        sb.store(Bs[nxt][0], tid, hi_step(t + 1));
         if (t + 2 < nsteps) load_step(t + 2);
       }
       __syncthreads();
     } else {
       __syncthreads();  // every wave is done with the stage
       if (t + 1 < nsteps) {
-        sa.store(As[0][0], tid);
-        sb.store(Bs[0][0], tid);
+        sa.store(As[0][0], tid, hi_step(t + 1));
+        sb.store(Bs[0][0], tid, hi_step(t + 1));
         if (t + 2 < nsteps) load_step(t + 2);
       }
       __syncthreads();
