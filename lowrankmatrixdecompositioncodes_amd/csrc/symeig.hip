@@ -338,9 +338,13 @@ symj_step_kernel(int64_t n, const double* __restrict__ Gin, double* __restrict__
                  double* __restrict__ Jout, double* __restrict__ Hout, int ntg,
                  unsigned long long* __restrict__ flag) {
   constexpr int S = 2 * B, HP = S + 1, BP = B + 1;
-  // solve role: Ha, Hb, Jl, Tab (S x HP), JAh, JBh, X3 (S x BP); apply role:
-  // T, Ja, Jb, X (S x HP)
-  __shared__ double sm[4 * S * HP + 3 * S * BP];
+  // four S x HP regions.  Apply role: T, Ja, Jb, X.  Solve role: Ha; the
+  // column halves JAh, JBh (stored transposed, B x HP each), then Jl; Tab,
+  // then Hb; X3 (S x BP).  Each later tenant is first written after the last
+  // read of the earlier one, and the footprint of the apply role lets four
+  // workgroups share a CU.
+  static_assert(2 * B * HP <= S * HP && S * BP <= S * HP, "solve role fits the apply regions");
+  __shared__ double sm[4 * S * HP];
   __shared__ double rc[B], rs[B];
   __shared__ int64_t c0[S], c1[S];
   const int tid = threadIdx.x;
@@ -350,12 +354,12 @@ symj_step_kernel(int64_t n, const double* __restrict__ Gin, double* __restrict__
     return;
   }
   double* Ha = sm;
-  double* Hb = sm + S * HP;
-  double* Jl = sm + 2 * S * HP;
-  double* Tab = sm + 3 * S * HP;
-  double* JAh = sm + 4 * S * HP;
-  double* JBh = JAh + S * BP;
-  double* X3 = JBh + S * BP;
+  double* Jl = sm + S * HP;
+  double* JAh = Jl;            // JAh[c * HP + i], until H_ab is formed
+  double* JBh = Jl + B * HP;   // JBh[c * HP + i]
+  double* Tab = sm + 2 * S * HP;
+  double* Hb = Tab;            // from the first inner step on
+  double* X3 = sm + 3 * S * HP;
   const int w = blockIdx.x;
   // the blocks of pair w in round1 and where they sat in round r
   const int xa = player(w, round1, N), xb = player(N - 1 - w, round1, N);
@@ -385,12 +389,11 @@ symj_step_kernel(int64_t n, const double* __restrict__ Gin, double* __restrict__
     const int i = e % S, j = e / S;
     const int64_t gi = c0[i], gj = c1[j];
     Tab[i * HP + j] = (gi >= 0 && gj >= 0) ? Gin[gi + gj * ldg] : 0.0;
-    Jl[i * HP + j] = (i == j) ? 1.0 : 0.0;
   }
   for (int e = tid; e < S * B; e += 256) {
     const int i = e % S, c = e / S;
-    JAh[i * BP + c] = JA[i + (ha * B + c) * S];
-    JBh[i * BP + c] = JB[i + (hb * B + c) * S];
+    JAh[c * HP + i] = JA[i + (ha * B + c) * S];
+    JBh[c * HP + i] = JB[i + (hb * B + c) * S];
   }
   {
     // diagonal blocks of the new tile from round r's final tiles
@@ -407,8 +410,8 @@ symj_step_kernel(int64_t n, const double* __restrict__ Gin, double* __restrict__
 #pragma unroll 8
     for (int j = 0; j < S; ++j) {
       const double t = Tab[i * HP + j];
-      s0 = fma(t, JBh[j * BP + cc], s0);
-      s1 = fma(t, JBh[j * BP + cc + 1], s1);
+      s0 = fma(t, JBh[cc * HP + j], s0);
+      s1 = fma(t, JBh[(cc + 1) * HP + j], s1);
     }
     X3[i * BP + cc] = s0;
     X3[i * BP + cc + 1] = s1;
@@ -419,11 +422,17 @@ symj_step_kernel(int64_t n, const double* __restrict__ Gin, double* __restrict__
     const int c = tid % B, d = tid / B;
     double acc = 0.0;
 #pragma unroll 8
-    for (int i = 0; i < S; ++i) acc = fma(JAh[i * BP + c], X3[i * BP + d], acc);
+    for (int i = 0; i < S; ++i) acc = fma(JAh[c * HP + i], X3[i * BP + d], acc);
     Ha[c * HP + B + d] = acc;
     Ha[(B + d) * HP + c] = acc;
   }
   __syncthreads();
+  // JAh / JBh are dead: J starts as the identity (first read after the
+  // barrier inside the inner sweep's first step)
+  for (int e = tid; e < S * S; e += 256) {
+    const int i = e % S, j = e / S;
+    Jl[i * HP + j] = (i == j) ? 1.0 : 0.0;
+  }
   int sig = 0;
   const double* Hf = jac_inner<B>(Ha, Hb, Jl, rc, rs, round1 > 0, flo, &sig);
   jac_out<B>(Hf, Jl, w, Jout, Hout, sig, flag);
