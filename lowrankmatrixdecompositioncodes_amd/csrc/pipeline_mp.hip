@@ -24,21 +24,37 @@
 namespace rsvd {
 
 // *out = max_ij |G_ij - delta_ij| of an n x n FP64 matrix (one workgroup).
-__global__ void max_dev_identity_kernel(int64_t n, const double* __restrict__ G,
-                                        double* __restrict__ out) {
-  __shared__ double red[256];
-  double mx = 0.0;
-  for (int64_t e = threadIdx.x; e < n * n; e += 256) {
-    const int64_t i = e % n, j = e / n;
-    mx = fmax(mx, fabs(G[e] - (i == j ? 1.0 : 0.0)));
-  }
+// max |G - I| of an n x n FP64 matrix (leading dimension n): one workgroup
+// per column writes that column's maximum, then one workgroup takes the
+// maximum of those (fmax is exact, so the split does not change the value).
+__device__ __forceinline__ double block_max_256(double mx, double* red) {
   red[threadIdx.x] = mx;
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
     __syncthreads();
   }
-  if (threadIdx.x == 0) *out = red[0];
+  return red[0];
+}
+
+__global__ void col_dev_identity_kernel(int64_t n, const double* __restrict__ G,
+                                        double* __restrict__ part) {
+  __shared__ double red[256];
+  const int64_t j = blockIdx.x;
+  double mx = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += 256)
+    mx = fmax(mx, fabs(G[i + j * n] - (i == j ? 1.0 : 0.0)));
+  mx = block_max_256(mx, red);
+  if (threadIdx.x == 0) part[j] = mx;
+}
+
+__global__ void max_of_kernel(int64_t n, const double* __restrict__ part,
+                              double* __restrict__ out) {
+  __shared__ double red[256];
+  double mx = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += 256) mx = fmax(mx, part[i]);
+  mx = block_max_256(mx, red);
+  if (threadIdx.x == 0) *out = mx;
 }
 
 // Y (m x n, fp32) <- orthonormal basis of its range.  passes = 1 (CholeskyQR)
@@ -75,9 +91,12 @@ static int orth_f32(Context* c, int64_t m, int64_t n, float* Y, int64_t ldy, int
                          G32.p, n, 3));
     RSVD_TRY(f32_to_f64(c, n, n, G32.p, n, G64.p, n));
     if (exact && pass > 0 && pass + 1 == total && total < passes + 2) {
-      if (!dev.p) RSVD_TRY(dev.alloc(1));
-      hipLaunchKernelGGL(max_dev_identity_kernel, dim3(1), dim3(256), 0, c->stream, n,
-                         (const double*)G64.p, dev.p);
+      if (!dev.p) RSVD_TRY(dev.alloc((size_t)n + 1));  // result, then n column maxima
+      hipLaunchKernelGGL(col_dev_identity_kernel, dim3((unsigned)n), dim3(256), 0, c->stream, n,
+                         (const double*)G64.p, dev.p + 1);
+      RSVD_LAUNCH_CHECK();
+      hipLaunchKernelGGL(max_of_kernel, dim3(1), dim3(256), 0, c->stream, n,
+                         (const double*)(dev.p + 1), dev.p);
       RSVD_LAUNCH_CHECK();
       double h = 0.0;
       RSVD_HIP(hipMemcpyAsync(&h, dev.p, sizeof h, hipMemcpyDeviceToHost, c->stream));
