@@ -27,6 +27,7 @@
 // ordered so that the tiles sharing a panel of the large operand are launched
 // together (their reads of that panel then hit the shared last-level cache).
 #include <hip/hip_bf16.h>
+#include <type_traits>
 
 #include "rsvd_internal.h"
 
@@ -498,6 +499,202 @@ __global__ void sum_partials_lp_kernel(int64_t n, const double* __restrict__ par
   if (threadIdx.x == 0) *out = red[0];
 }
 
+// ---- Deflation C -= Q B^T on an LDS ring -------------------------------------
+// The deflation's operands are small (Q: M x K, B^T: N x K, K <= 2048) and its
+// output is the whole residual.  The operands are split into the three bf16
+// planes once per call, in MFMA fragment order (split3_frag_kernel), and each
+// 128 x 64 tile streams them into two LDS stages by direct loads
+// (global_load_lds_dwordx4, no registers) one k block ahead; 73 KiB of LDS,
+// so that two workgroups share a CU and overlap each other's prologue and
+// read-modify-write of C.  Every accumulator takes the products of the tiled
+// kernel in the same order, so C is the same to the bit.
+typedef __attribute__((address_space(3))) void* lds_ptr_t;
+typedef __attribute__((address_space(1))) void* gbl_ptr_t;
+
+// X (rows x K, column-major, ld) -> out[((P * KB + kb) * RT + t) * 64 + lane]
+// = plane P of rows 16 t + lane % 16, k = 32 kb + 8 (lane / 16) .. + 7
+// (planes as put4; rows past `rows` are zero).  Planes 1 and 2 are written
+// for k blocks at or above kblo only.
+__global__ void split3_frag_kernel(int64_t rows, int64_t KB, int64_t RT, int64_t kblo,
+                                   const float* __restrict__ X, int64_t ldx,
+                                   bf16x8_t* __restrict__ out) {
+  const int64_t total = KB * RT * 64;
+  const int64_t plane = total;
+  for (int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * 256) {
+    const int lane = (int)(idx & 63);
+    const int64_t t = (idx >> 6) % RT, kb = (idx >> 6) / RT;
+    const int64_t r = 16 * t + (lane & 15);
+    const int64_t k0 = 32 * kb + 8 * (lane >> 4);
+    bf16x8_t h, m, l;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const float x = (r < rows) ? X[r + (k0 + e) * ldx] : 0.0f;
+      const __bf16 a = (__bf16)x;
+      const float r1 = x - (float)a;
+      const __bf16 b = (__bf16)r1;
+      h[e] = a;
+      m[e] = b;
+      l[e] = (__bf16)(r1 - (float)b);
+    }
+    out[idx] = h;
+    if (kb >= kblo) {
+      out[plane + idx] = m;
+      out[2 * plane + idx] = l;
+    }
+  }
+}
+
+constexpr int kDBN = 64;  // ring tile columns (4 row tiles of B)
+
+// C -= Q B^T over k blocks [0, KB) (hi * hi only below kblo), C's FP64 sum of
+// squares per tile into sumsq[tile].  Qs / Bs: split3_frag_kernel layouts
+// with RTQ / RTB row tiles (multiples of 8 and 4).
+__global__ void __launch_bounds__(LTHREADS, 2)
+deflate_ring_kernel(int64_t M, int64_t N, int KB, int kblo, const bf16x8_t* __restrict__ Qs,
+                    int64_t RTQ, const bf16x8_t* __restrict__ Bs, int64_t RTB,
+                    float* __restrict__ C, int64_t ldc, int n_fast,
+                    double* __restrict__ sumsq) {
+  // [stage][chunk][lane], chunk = plane * 12 + row tile (Q: 0..7, B: 8..11)
+  __shared__ bf16x8_t Ls[2][36][64];
+  // A direct load's LDS target is picked between its ring slot and Ldummy on
+  // a condition that always holds but that the compiler cannot prove; the
+  // target then carries no LDS memory operand and the compiler adds no wait
+  // for the loads in flight before each LDS read (it cannot tell the stages
+  // apart).  Ordering comes from the explicit waits and barrier.
+  __shared__ bf16x8_t Ldummy[64];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int li = lane & 15, lk = lane >> 4;
+  int64_t tm, tn;
+  {
+    const int64_t nx = (int64_t)gridDim.x * gridDim.y;
+    const int64_t b = (int64_t)blockIdx.y * gridDim.x + blockIdx.x;
+    const int64_t per = nx / 8;
+    const int64_t logical = (b < 8 * per) ? (b % 8) * per + b / 8 : b;
+    const int64_t fx = logical % gridDim.x, fy = logical / gridDim.x;
+    if (n_fast) { tn = fx; tm = fy; }
+    else { tm = fx; tn = fy; }
+  }
+  const int64_t m0 = tm * LBM, n0 = tn * kDBN;
+  const int wa = (wave & 1) * 4, wb = (wave >> 1) * 2;  // the wave's first Q / B row tile
+  const bf16x8_t* Qb = Qs + tm * 8 * 64 + lane;
+  const bf16x8_t* Bb = Bs + tn * 4 * 64 + lane;
+  const int64_t qkb = RTQ * 64, qpl = (int64_t)KB * qkb;
+  const int64_t bkb = RTB * 64, bpl = (int64_t)KB * bkb;
+  // direct loads of k block kb into stage st: chunks taken by the four waves
+  // in turn -- 3 per wave (plane 0 only) below kblo, 9 above
+  auto issue = [&](int kb, int st) {
+    const int np = (kb < kblo) ? 1 : 3;
+    if (np == 1) {
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const int ch = wave + 4 * q, t = ch;  // 0..11
+        const bf16x8_t* src = (t < 8) ? Qb + kb * qkb + t * 64 : Bb + kb * bkb + (t - 8) * 64;
+        lds_ptr_t d = (ch < 36) ? (lds_ptr_t)&Ls[st][ch][0] : (lds_ptr_t)&Ldummy[0];
+        __builtin_amdgcn_global_load_lds((gbl_ptr_t)src, d, 16, 0, 0);
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < 9; ++q) {
+        const int ch = wave + 4 * q, P = ch / 12, t = ch % 12;  // 0..35
+        const bf16x8_t* src = (t < 8) ? Qb + P * qpl + kb * qkb + t * 64
+                                      : Bb + P * bpl + kb * bkb + (t - 8) * 64;
+        lds_ptr_t d = (ch < 36) ? (lds_ptr_t)&Ls[st][ch][0] : (lds_ptr_t)&Ldummy[0];
+        __builtin_amdgcn_global_load_lds((gbl_ptr_t)src, d, 16, 0, 0);
+      }
+    }
+  };
+
+  f4v_t acc[2][4];  // [B row tile a][Q row tile b]
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = f4v_t{0.f, 0.f, 0.f, 0.f};
+
+  // one k block on stage st (a compile-time index, so that the fragment
+  // reads have constant offsets)
+  auto step = [&](int kb, auto stc) {
+    constexpr int st = decltype(stc)::value;
+    // this wave's loads of block kb have landed and its reads of the other
+    // stage (block kb - 1) have returned; after the barrier block kb is
+    // complete in LDS and the other stage is free for block kb + 1
+    __builtin_amdgcn_s_waitcnt(0 | (0x7 << 4));
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+    if (kb + 1 < KB) issue(kb + 1, st ^ 1);
+    if (kb < kblo) {
+      bf16x8_t fa[4], fb[2];
+#pragma unroll
+      for (int b = 0; b < 4; ++b) fa[b] = Ls[st][wa + b][lane];
+#pragma unroll
+      for (int a = 0; a < 2; ++a) fb[a] = Ls[st][8 + wb + a][lane];
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[a], fa[b], acc[a][b], 0, 0, 0);
+    } else {
+      bf16x8_t fa[3][4], fb[3][2];
+#pragma unroll
+      for (int P = 0; P < 3; ++P) {
+#pragma unroll
+        for (int b = 0; b < 4; ++b) fa[P][b] = Ls[st][12 * P + wa + b][lane];
+#pragma unroll
+        for (int a = 0; a < 2; ++a) fb[P][a] = Ls[st][12 * P + 8 + wb + a][lane];
+      }
+      // per accumulator the tiled kernel's order (smallest terms first);
+      // consecutive MFMAs go to different accumulators
+      constexpr int PB[6] = {0, 2, 1, 0, 1, 0}, PA[6] = {2, 0, 1, 1, 0, 0};
+#pragma unroll
+      for (int term = 0; term < 6; ++term)
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+          for (int b = 0; b < 4; ++b)
+            acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fb[PB[term]][a], fa[PA[term]][b],
+                                                                acc[a][b], 0, 0, 0);
+    }
+  };
+  issue(0, 0);
+  for (int kb = 0; kb < KB; kb += 2) {
+    step(kb, std::integral_constant<int, 0>{});
+    if (kb + 1 < KB) step(kb + 1, std::integral_constant<int, 1>{});
+  }
+
+  // C <- C - acc and the tile's FP64 sum of squares
+  double ss = 0.0;
+  const int wm0 = (wave & 1) * 64, wn0 = (wave >> 1) * 32;
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int64_t gm = m0 + wm0 + b * 16 + li;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t gn = n0 + wn0 + a * 16 + 4 * lk + r;
+        if (gm < M && gn < N) {
+          float* cp = C + gm + gn * ldc;
+          float v = -1.0f * acc[a][b][r];
+          v += 1.0f * (*cp);
+          *cp = v;
+          ss += (double)v * (double)v;
+        }
+      }
+    }
+  __shared__ double red[LTHREADS / 64];
+  for (int off = 32; off > 0; off >>= 1) ss += __shfl_xor(ss, off, 64);
+  if (lane == 0) red[wave] = ss;
+  __syncthreads();
+  if (tid == 0) {
+    double t = 0.0;
+    for (int w = 0; w < LTHREADS / 64; ++w) t += red[w];
+    sumsq[blockIdx.x + (int64_t)gridDim.x * blockIdx.y] = t;
+  }
+}
+
 int gemm_bf16_deflate(Context* c, int64_t M, int64_t N, int64_t K, const float* Q, int64_t ldq,
                       const float* Bt, int64_t ldbt, float* C, int64_t ldc, double* sumsq,
                       int64_t klo) {
@@ -507,12 +704,42 @@ int gemm_bf16_deflate(Context* c, int64_t M, int64_t N, int64_t K, const float* 
     return 0;
   }
   if (K == 0) return set_error("deflate: empty block");
-  const int64_t tiles = ((M + LBM - 1) / LBM) * ((N + LBN - 1) / LBN);
-  DBuf<double> part;
-  RSVD_TRY(part.alloc((size_t)tiles));
   if (klo % 32 != 0) return set_error("deflate: klo must be a multiple of 32");
-  RSVD_TRY(gemm_bf16_tiled(c, "deflate_bf16x6", 'N', 'T', M, N, K, -1.0f, Q, ldq, Bt, ldbt, 1.0f,
-                           C, ldc, 6, part.p, klo));
+  const int64_t gm = (M + LBM - 1) / LBM, gnr = (N + kDBN - 1) / kDBN;
+  const int n_fast = (gnr <= gm) ? 1 : 0;
+  const int64_t gx = n_fast ? gnr : gm, gy = n_fast ? gm : gnr;
+  DBuf<double> part;
+  int64_t tiles;
+  if (K % 32 == 0 && gy <= 65535) {
+    // the LDS ring on operands split once
+    const int64_t KB = K / 32, kblo = klo / 32;
+    const int64_t RTQ = gm * 8, RTB = gnr * 4;
+    tiles = gm * gnr;
+    RSVD_TRY(part.alloc((size_t)tiles));
+    DBuf<bf16x8_t> qs, bs;
+    RSVD_TRY(qs.alloc((size_t)(3 * KB * RTQ * 64)));
+    RSVD_TRY(bs.alloc((size_t)(3 * KB * RTB * 64)));
+    KTimer kt(c, "deflate_bf16x6");
+    auto blocks_for = [](int64_t items) {
+      const int64_t b = (items + 255) / 256;
+      return (unsigned)(b < 8192 ? b : 8192);
+    };
+    hipLaunchKernelGGL(split3_frag_kernel, dim3(blocks_for(KB * RTQ * 64)), dim3(256), 0,
+                       c->stream, M, KB, RTQ, kblo, Q, ldq, qs.p);
+    RSVD_LAUNCH_CHECK();
+    hipLaunchKernelGGL(split3_frag_kernel, dim3(blocks_for(KB * RTB * 64)), dim3(256), 0,
+                       c->stream, N, KB, RTB, kblo, Bt, ldbt, bs.p);
+    RSVD_LAUNCH_CHECK();
+    hipLaunchKernelGGL(deflate_ring_kernel, dim3((unsigned)gx, (unsigned)gy), dim3(LTHREADS), 0,
+                       c->stream, M, N, (int)KB, (int)kblo, (const bf16x8_t*)qs.p, RTQ,
+                       (const bf16x8_t*)bs.p, RTB, C, ldc, n_fast, part.p);
+    RSVD_LAUNCH_CHECK();
+  } else {
+    tiles = gm * ((N + LBN - 1) / LBN);
+    RSVD_TRY(part.alloc((size_t)tiles));
+    RSVD_TRY(gemm_bf16_tiled(c, "deflate_bf16x6", 'N', 'T', M, N, K, -1.0f, Q, ldq, Bt, ldbt,
+                             1.0f, C, ldc, 6, part.p, klo));
+  }
   hipLaunchKernelGGL(sum_partials_lp_kernel, dim3(1), dim3(256), 0, c->stream, tiles,
                      (const double*)part.p, sumsq);
   RSVD_LAUNCH_CHECK();
