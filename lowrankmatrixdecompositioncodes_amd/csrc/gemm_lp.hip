@@ -183,6 +183,20 @@ struct Stager<false, SPLIT> {
 
 }  // namespace
 
+// XCD-aware tile order: workgroups are dispatched round-robin over the 8 XCDs
+// (block b -> XCD b % 8).  Remap so that consecutive logical tiles -- the
+// tiles sharing one panel of the large operand -- land on one XCD and share
+// its L2.  n_fast: the column-tile index runs fastest (over gridDim.x).
+__device__ __forceinline__ void xcd_tile(int n_fast, int64_t* tm, int64_t* tn) {
+  const int64_t nx = (int64_t)gridDim.x * gridDim.y;
+  const int64_t b = (int64_t)blockIdx.y * gridDim.x + blockIdx.x;
+  const int64_t per = nx / 8;
+  const int64_t logical = (b < 8 * per) ? (b % 8) * per + b / 8 : b;
+  const int64_t fx = logical % gridDim.x, fy = logical / gridDim.x;
+  if (n_fast) { *tn = fx; *tm = fy; }
+  else { *tm = fx; *tn = fy; }
+}
+
 // C[z] = alpha * op(A) op(B) (+ beta C), K range of slice z.  Two LDS stages:
 // while the MFMAs read stage t, the registers loaded during step t-1 are
 // written to stage t+1 and the loads for step t+2 are issued; one barrier
@@ -207,20 +221,8 @@ gemm_bf16_kernel(int64_t M, int64_t N, int64_t K, int64_t kchunk, float alpha,
   const int lane = tid & 63, wave = tid >> 6;
   const int li = lane & 15, lk = lane >> 4;
   const int wm0 = (wave & 1) * 64, wn0 = (wave >> 1) * 64;
-  // XCD-aware tile order: workgroups are dispatched round-robin over the 8
-  // XCDs (block b -> XCD b % 8).  Remap so that consecutive logical tiles --
-  // the tiles sharing one panel of the large operand -- land on one XCD and
-  // share its L2.
   int64_t tm, tn;
-  {
-    const int64_t nx = (int64_t)gridDim.x * gridDim.y;
-    const int64_t b = (int64_t)blockIdx.y * gridDim.x + blockIdx.x;
-    const int64_t per = nx / 8;
-    const int64_t logical = (b < 8 * per) ? (b % 8) * per + b / 8 : b;
-    const int64_t fx = logical % gridDim.x, fy = logical / gridDim.x;
-    if (n_fast) { tn = fx; tm = fy; }
-    else { tm = fx; tn = fy; }
-  }
+  xcd_tile(n_fast, &tm, &tn);
   const int64_t m0 = tm * LBM, n0 = tn * LBN;
   const int64_t kbeg = (int64_t)blockIdx.z * kchunk;
   const int64_t kend = (kbeg + kchunk < K) ? kbeg + kchunk : K;
@@ -568,15 +570,7 @@ deflate_ring_kernel(int64_t M, int64_t N, int KB, int kblo, const bf16x8_t* __re
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int li = lane & 15, lk = lane >> 4;
   int64_t tm, tn;
-  {
-    const int64_t nx = (int64_t)gridDim.x * gridDim.y;
-    const int64_t b = (int64_t)blockIdx.y * gridDim.x + blockIdx.x;
-    const int64_t per = nx / 8;
-    const int64_t logical = (b < 8 * per) ? (b % 8) * per + b / 8 : b;
-    const int64_t fx = logical % gridDim.x, fy = logical / gridDim.x;
-    if (n_fast) { tn = fx; tm = fy; }
-    else { tm = fx; tn = fy; }
-  }
+  xcd_tile(n_fast, &tm, &tn);
   const int64_t m0 = tm * LBM, n0 = tn * kDBN;
   const int wa = (wave & 1) * 4, wb = (wave >> 1) * 2;  // the wave's first Q / B row tile
   const bf16x8_t* Qb = Qs + tm * 8 * 64 + lane;
