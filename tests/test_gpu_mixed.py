@@ -229,6 +229,36 @@ def test_blockrand_mp_tol_mode(gpu_lib, prec):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("kstep", [40, 96])
+def test_blockrand_mp_tol_mode_other_kstep(gpu_lib, kstep):
+    """The deflation's two kernels: kstep = 40 gives k blocks of 40 and 80
+    columns (not multiples of 32: the register-staged tiled kernel, no
+    hi-only range); kstep = 96 gives 96 and 192 (the LDS-ring kernel with an
+    odd k-block count and the hi-only / six-product boundary after block 3).
+    Same checks as the kstep = 64 case, split-bf16 sketches."""
+    rp = gpu_lib
+    lib = rp.load()
+    m, n, q = 6000, 1500, 1
+    A64, A32, sigma = _blockrand_case(rp, m, n, 4321)
+    anorm = float(np.sqrt(np.sum(sigma ** 2)))
+    tol = 1e-6 * anorm
+    f64_, s64, *bufs64 = _blockrand(rp, False, A64, m, n, -1, 10, tol, kstep, q)
+    fmp, smp, u, sv, v = _blockrand(rp, True, A32, m, n, -1, 10, tol, kstep, q, 2)
+    res = C.c_double(0)
+    assert lib.rsvd_dev_lowrank_residual(A64.ptr, m, n, m, fmp, u.value, m, sv.value, v.value, n,
+                                         1, C.byref(res)) == 0, rp.last_error()
+    for b in (u, sv, v, *bufs64):
+        lib.rsvd_dev_free(b.value)
+    kk = min(fmp, f64_)
+    ds = np.abs(np.sort(smp)[::-1][:kk] - np.sort(s64)[::-1][:kk]).max()
+    print(f"kstep {kstep}: frank {fmp} (FP64 {f64_}), residual {res.value / anorm:.3e} x ||A||, "
+          f"max |dsigma| {ds:.2e}")
+    assert fmp >= f64_
+    assert res.value <= 1.5 * tol
+    assert ds <= 1e-6 * sigma[0]
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("prec", [2, 1])
 def test_randqb_mp_tol_mode(gpu_lib, prec):
     """randQB_pb_new in TOL mode on the float32 A (6000 x 1500, kstep = 64,
