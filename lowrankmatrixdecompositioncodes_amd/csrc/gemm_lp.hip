@@ -652,6 +652,22 @@ deflate_ring_kernel(int64_t M, int64_t N, int KB, int kblo, const bf16x8_t* __re
                                                                 acc[a][b], 0, 0, 0);
     }
   };
+  // this lane's 32 C values, loaded before the first direct loads so that
+  // their latency overlaps stage 0's instead of following the k loop (no
+  // other workgroup touches this tile, so they are the values read later)
+  const int wm0 = (wave & 1) * 64, wn0 = (wave >> 1) * 32;
+  float cpre[2][4][4];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int64_t gm = m0 + wm0 + b * 16 + li;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t gn = n0 + wn0 + a * 16 + 4 * lk + r;
+        cpre[a][b][r] = (gm < M && gn < N) ? C[gm + gn * ldc] : 0.0f;
+      }
+    }
   issue(0, 0);
   for (int kb = 0; kb < KB; kb += 2) {
     step(kb, std::integral_constant<int, 0>{});
@@ -660,7 +676,6 @@ deflate_ring_kernel(int64_t M, int64_t N, int KB, int kblo, const bf16x8_t* __re
 
   // C <- C - acc and the tile's FP64 sum of squares
   double ss = 0.0;
-  const int wm0 = (wave & 1) * 64, wn0 = (wave >> 1) * 32;
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -670,10 +685,9 @@ deflate_ring_kernel(int64_t M, int64_t N, int KB, int kblo, const bf16x8_t* __re
       for (int r = 0; r < 4; ++r) {
         const int64_t gn = n0 + wn0 + a * 16 + 4 * lk + r;
         if (gm < M && gn < N) {
-          float* cp = C + gm + gn * ldc;
           float v = -1.0f * acc[a][b][r];
-          v += 1.0f * (*cp);
-          *cp = v;
+          v += 1.0f * cpre[a][b][r];
+          C[gm + gn * ldc] = v;
           ss += (double)v * (double)v;
         }
       }
