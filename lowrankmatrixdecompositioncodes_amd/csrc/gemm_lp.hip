@@ -487,18 +487,22 @@ int gemm_bf16_general(Context* c, const char* tag, char ta, char tb, int64_t M, 
                          nullptr);
 }
 
-__global__ void sum_partials_lp_kernel(int64_t n, const double* __restrict__ part,
+// out[b] = sum of part[b * chunk .. (b + 1) * chunk) (clipped to n), one
+// workgroup per b: a fixed order, so the FP64 total is the same on every run.
+__global__ void sum_partials_lp_kernel(int64_t n, int64_t chunk, const double* __restrict__ part,
                                        double* __restrict__ out) {
   __shared__ double red[256];
+  const int64_t lo = (int64_t)blockIdx.x * chunk;
+  const int64_t hi = (lo + chunk < n) ? lo + chunk : n;
   double s = 0;
-  for (int64_t i = threadIdx.x; i < n; i += 256) s += part[i];
+  for (int64_t i = lo + threadIdx.x; i < hi; i += 256) s += part[i];
   red[threadIdx.x] = s;
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x == 0) *out = red[0];
+  if (threadIdx.x == 0) out[blockIdx.x] = red[0];
 }
 
 // ---- Deflation C -= Q B^T on an LDS ring -------------------------------------
@@ -748,8 +752,16 @@ int gemm_bf16_deflate(Context* c, int64_t M, int64_t N, int64_t K, const float* 
     RSVD_TRY(gemm_bf16_tiled(c, "deflate_bf16x6", 'N', 'T', M, N, K, -1.0f, Q, ldq, Bt, ldbt,
                              1.0f, C, ldc, 6, part.p, klo));
   }
-  hipLaunchKernelGGL(sum_partials_lp_kernel, dim3(1), dim3(256), 0, c->stream, tiles,
-                     (const double*)part.p, sumsq);
+  // the per-tile partials: 256-way in slices of `chunk`, then the slice sums
+  const int64_t chunk = (tiles + 255) / 256;
+  const int64_t nsl = (tiles + chunk - 1) / chunk;
+  DBuf<double> slice;
+  RSVD_TRY(slice.alloc((size_t)nsl));
+  hipLaunchKernelGGL(sum_partials_lp_kernel, dim3((unsigned)nsl), dim3(256), 0, c->stream, tiles,
+                     chunk, (const double*)part.p, slice.p);
+  RSVD_LAUNCH_CHECK();
+  hipLaunchKernelGGL(sum_partials_lp_kernel, dim3(1), dim3(256), 0, c->stream, nsl, nsl,
+                     (const double*)slice.p, sumsq);
   RSVD_LAUNCH_CHECK();
   return 0;
 }
