@@ -506,14 +506,14 @@ __global__ void sum_partials_lp_kernel(int64_t n, int64_t chunk, const double* _
 }
 
 // ---- Deflation C -= Q B^T on an LDS ring -------------------------------------
-// The deflation's operands are small (Q: M x K, B^T: N x K, K <= 2048) and its
-// output is the whole residual.  The operands are split into the three bf16
-// planes once per call, in MFMA fragment order (split3_frag_kernel), and each
-// 128 x 64 tile streams them into two LDS stages by direct loads
-// (global_load_lds_dwordx4, no registers) one k block ahead; 73 KiB of LDS,
-// so that two workgroups share a CU and overlap each other's prologue and
-// read-modify-write of C.  Every accumulator takes the products of the tiled
-// kernel in the same order, so C is the same to the bit.
+// The deflation's operands are small (Q: M x K, B^T: N x K, K = one or two QB
+// blocks) and its output is the whole residual.  The operands are split into
+// the three bf16 planes once per call, in MFMA fragment order
+// (split3_frag_kernel), and each 128 x 64 tile streams them into two LDS
+// stages by direct loads (global_load_lds_dwordx4, no registers) one k block
+// ahead; 73 KiB of LDS, so that two workgroups share a CU and overlap each
+// other's prologue and stores.  Every accumulator takes the products of the
+// tiled kernel in the same order, so C is the same to the bit.
 typedef __attribute__((address_space(3))) void* lds_ptr_t;
 typedef __attribute__((address_space(1))) void* gbl_ptr_t;
 
