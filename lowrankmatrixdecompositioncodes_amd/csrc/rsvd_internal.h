@@ -272,7 +272,7 @@ int sketch_features(Context* c, const float* A, int64_t m, int64_t n, int64_t ld
 // and V (optional, n x n) holds the eigenvectors (V^T G_in V = diag);
 // evals (optional) = diag(G), unsorted.  *sweeps_out = sweeps used.  The
 // iteration ends after the first sweep whose rotations were all below 1e-10
-// of sqrt(g_pp g_qq), or 1e-8 with loose = 1 (eigenvectors to fp32 level).
+// of sqrt(g_pp g_qq), or 1e-4 with loose = 1 (eigenvectors to fp32 level).
 int symeig_jacobi_f64(Context* c, int64_t n, double* G, int64_t ldg, double* V, int64_t ldv,
                       double* evals, int max_sweeps, int* sweeps_out, int loose = 0);
 // Column norms of W into nrm[n]; optional normalisation of the columns.
