@@ -206,7 +206,7 @@ __global__ void __launch_bounds__(LTHREADS, 2)
 gemm_bf16_kernel(int64_t M, int64_t N, int64_t K, int64_t kchunk, float alpha,
                  const float* __restrict__ A, int64_t lda, const float* __restrict__ B, int64_t ldb,
                  float beta, float* __restrict__ C, int64_t ldc, int64_t zstride, int n_fast,
-                 int aligned, double* __restrict__ sumsq, int64_t klo) {
+                 int aligned, double* __restrict__ sumsq, int64_t klo, int sym) {
   constexpr int NBUF = lp_planes(SPLIT);  // bf16 planes per operand
   constexpr int LBK = lp_bk(SPLIT), LROW = LBK + LPAD;
   // LDS stages: one for the split modes (74 KB with two planes, 61 KB with
@@ -223,6 +223,8 @@ gemm_bf16_kernel(int64_t M, int64_t N, int64_t K, int64_t kchunk, float alpha,
   const int wm0 = (wave & 1) * 64, wn0 = (wave >> 1) * 64;
   int64_t tm, tn;
   xcd_tile(n_fast, &tm, &tn);
+  // a Gram (sym): the tiles below the diagonal are the mirror of those above
+  if (sym && tm > tn) return;
   const int64_t m0 = tm * LBM, n0 = tn * LBN;
   const int64_t kbeg = (int64_t)blockIdx.z * kchunk;
   const int64_t kend = (kbeg + kchunk < K) ? kbeg + kchunk : K;
@@ -399,6 +401,16 @@ int gemm_bf16(Context* c, const char* tag, char ta, char tb, int64_t M, int64_t 
   return gemm_bf16_general(c, tag, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, split);
 }
 
+// C(i, j) <- C(j, i) for i > j: the strictly lower triangle from the upper one
+__global__ void mirror_lower_f32_kernel(int64_t n, float* __restrict__ C, int64_t ldc) {
+  const int64_t total = n * n;
+  for (int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * 256) {
+    const int64_t i = idx % n, j = idx / n;
+    if (i > j) C[i + j * ldc] = C[j + i * ldc];
+  }
+}
+
 // The tiled kernel; sumsq_part (one double per tile, optional) receives the
 // FP64 sums of squares of the stored tiles and disables the k split.
 static int gemm_bf16_tiled(Context* c, const char* tag, char ta, char tb, int64_t M, int64_t N,
@@ -412,8 +424,11 @@ static int gemm_bf16_tiled(Context* c, const char* tag, char ta, char tb, int64_
   const int n_fast = (gn <= gm) ? 1 : 0;
   const int64_t gx = n_fast ? gn : gm, gy = n_fast ? gm : gn;
   if (gy > 65535) return set_error("gemm_bf16: grid too large");
+  // A^T A with beta = 0: only the tiles on and above the diagonal are formed,
+  // the lower triangle is copied from the upper one afterwards
+  const bool sym = A == B && lda == ldb && tA && !tB && M == N && beta == 0.0f && !sumsq_part;
   int splits = 1;
-  const int64_t tiles = gm * gn;
+  const int64_t tiles = sym ? gm * (gm + 1) / 2 : gm * gn;
   const int64_t target = 2LL * (c->num_cus > 0 ? c->num_cus : 256);
   if (!sumsq_part && tiles < target && K >= 8 * 256) {
     int64_t maxs = K / 512;
@@ -446,7 +461,7 @@ static int gemm_bf16_tiled(Context* c, const char* tag, char ta, char tb, int64_
 #define RSVD_LP_LAUNCH(TA_, TB_, S_)                                                         \
   hipLaunchKernelGGL((gemm_bf16_kernel<TA_, TB_, S_>), grid, dim3(LTHREADS), 0,        \
                      c->stream, M, N, K, kchunk, alpha, A, lda, B, ldb, b_eff, Cout, ldo,    \
-                     zstride, n_fast, aligned, sumsq_part, klo)
+                     zstride, n_fast, aligned, sumsq_part, klo, sym ? 1 : 0)
     if (split == 1) {
       if (!tA && !tB) RSVD_LP_LAUNCH(false, false, 1);
       else if (tA && !tB) RSVD_LP_LAUNCH(true, false, 1);
@@ -472,6 +487,14 @@ static int gemm_bf16_tiled(Context* c, const char* tag, char ta, char tb, int64_
     if (blocks > 4096) blocks = 4096;
     hipLaunchKernelGGL(splitk_reduce_f32_kernel, dim3(blocks), dim3(256), 0, c->stream, M, N,
                        splits, (const float*)work.p, beta, C, ldc);
+    RSVD_LAUNCH_CHECK();
+  }
+  if (sym) {
+    const int64_t total = M * M;
+    int blocks = (int)((total + 255) / 256);
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(mirror_lower_f32_kernel, dim3(blocks), dim3(256), 0, c->stream, M, C,
+                       ldc);
     RSVD_LAUNCH_CHECK();
   }
   return 0;
