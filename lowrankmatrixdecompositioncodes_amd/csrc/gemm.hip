@@ -55,7 +55,7 @@ template <typename T, bool TA, bool TB, int BM, int BN, int BK>
 __global__ void __launch_bounds__(kThreads)
 gemm_kernel(int64_t M, int64_t N, int64_t K, int64_t kchunk, T alpha, const T* __restrict__ A,
             int64_t lda, const T* __restrict__ B, int64_t ldb, T beta, T* __restrict__ C,
-            int64_t ldc, int64_t zstride, double* __restrict__ sumsq) {
+            int64_t ldc, int64_t zstride, double* __restrict__ sumsq, int sym) {
   constexpr int PAD = 4;
   constexpr int WM = BM / 2, WN = BN / 2;
   constexpr int TM = WM / 16, TN = WN / 16;
@@ -73,6 +73,8 @@ gemm_kernel(int64_t M, int64_t N, int64_t K, int64_t kchunk, T alpha, const T* _
   const int wn0 = (wave >> 1) * WN;
   const int64_t m0 = (int64_t)blockIdx.x * BM;
   const int64_t n0 = (int64_t)blockIdx.y * BN;
+  // a Gram (sym): tiles wholly below the diagonal are mirrored afterwards
+  if (sym && m0 >= n0 + BN) return;
   const int64_t kbeg = (int64_t)blockIdx.z * kchunk;
   const int64_t kend = (kbeg + kchunk < K) ? kbeg + kchunk : K;
   T* Cz = C + (int64_t)blockIdx.z * zstride;
@@ -213,15 +215,36 @@ __global__ void sum_partials_kernel(int64_t n, const double* __restrict__ part,
   if (threadIdx.x == 0) *out = red[0];
 }
 
+// C(i, j) <- C(j, i) for i > j: the strictly lower triangle from the upper one
+template <typename T>
+__global__ void mirror_lower_kernel(int64_t n, T* __restrict__ C, int64_t ldc) {
+  const int64_t total = n * n;
+  for (int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * 256) {
+    const int64_t i = idx % n, j = idx / n;
+    if (i > j) C[i + j * ldc] = C[j + i * ldc];
+  }
+}
+
 template <typename T, int BM, int BN, int BK>
 static int launch_gemm(Context* c, char ta, char tb, int64_t M, int64_t N, int64_t K, T alpha,
                        const T* A, int64_t lda, const T* B, int64_t ldb, T beta, T* C,
                        int64_t ldc, double* sumsq_partials, const char* tag) {
   const int64_t gx = (M + BM - 1) / BM, gy = (N + BN - 1) / BN;
   if (gy > 65535) return set_error("gemm: N=%lld too large for the grid", (long long)N);
+  // A^T A with beta = 0: only the tiles that reach the diagonal or above are
+  // formed, the strictly lower triangle is copied from the upper one afterwards
+  const bool tA0 = (ta == 'T' || ta == 't'), tB0 = (tb == 'T' || tb == 't');
+  const bool sym = A == B && lda == ldb && tA0 && !tB0 && M == N && beta == T(0) &&
+                   !sumsq_partials;
+  int64_t tiles = gx * gy;
+  if (sym) {
+    tiles = 0;
+    for (int64_t bx = 0; bx < gx; ++bx)
+      for (int64_t by = 0; by < gy; ++by) tiles += (bx * BM < by * BN + BN) ? 1 : 0;
+  }
   // split K when the output has too few tiles to fill the CUs
   int splits = 1;
-  const int64_t tiles = gx * gy;
   const int64_t target = 2LL * (c->num_cus > 0 ? c->num_cus : 256);
   if (!sumsq_partials && tiles < target && K >= 4 * 256) {
     int64_t maxs = K / 256;
@@ -252,7 +275,7 @@ static int launch_gemm(Context* c, char ta, char tb, int64_t M, int64_t N, int64
 #define RSVD_GEMM_LAUNCH(TA_, TB_)                                                          \
   hipLaunchKernelGGL((gemm_kernel<T, TA_, TB_, BM, BN, BK>), grid, dim3(kThreads), 0,         \
                      c->stream, M, N, K, kchunk, a_eff, A, lda, B, ldb, b_eff, Cout, ldo,    \
-                     zstride, sumsq_partials)
+                     zstride, sumsq_partials, sym ? 1 : 0)
     if (!tA && !tB) RSVD_GEMM_LAUNCH(false, false);
     else if (tA && !tB) RSVD_GEMM_LAUNCH(true, false);
     else if (!tA && tB) RSVD_GEMM_LAUNCH(false, true);
@@ -266,6 +289,13 @@ static int launch_gemm(Context* c, char ta, char tb, int64_t M, int64_t N, int64
     if (blocks > 4096) blocks = 4096;
     hipLaunchKernelGGL(splitk_reduce_kernel<T>, dim3(blocks), dim3(256), 0, c->stream, M, N,
                        splits, (const T*)work.p, beta, C, ldc);
+    RSVD_LAUNCH_CHECK();
+  }
+  if (sym) {
+    const int64_t total = M * M;
+    int blocks = (int)((total + 255) / 256);
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(mirror_lower_kernel<T>, dim3(blocks), dim3(256), 0, c->stream, M, C, ldc);
     RSVD_LAUNCH_CHECK();
   }
   return 0;
