@@ -347,6 +347,38 @@ def test_gemm_bf16x6_is_fp32_exact(gpu_lib, ta, tb, M, N, K):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("dtype", [1, 0, 3])
+@pytest.mark.parametrize("n", [544, 300])
+def test_gram_upper_tiles_mirrored(gpu_lib, dtype, n):
+    """A^T A with one pointer for both operands and beta = 0 forms only the
+    tiles on and above the diagonal and mirrors the rest: the result is
+    exactly symmetric and agrees with the same product taken as a general GEMM
+    (B a separate copy of A) to the accumulation-order level of each dtype
+    (0 FP64, 1 fp32, 3 split-bf16)."""
+    rng = np.random.default_rng(n + dtype)
+    m = 65536 if dtype != 0 else 16384
+    np_t = np.float64 if dtype == 0 else np.float32
+    A = rng.standard_normal((m, n)).astype(np_t)
+    lib = gpu_lib.load()
+    dA = gpu_lib.DeviceMatrix.from_host(A)
+    dB = gpu_lib.DeviceMatrix.from_host(A.copy())
+    out = {}
+    for name, bptr in (("gram", dA.ptr), ("general", dB.ptr)):
+        dC = gpu_lib.DeviceMatrix.from_host(np.full((n, n), np.nan, dtype=np_t))
+        assert lib.rsvd_dev_gemm(dtype, b"T", b"N", n, n, m, 1.0, dA.ptr, m, bptr, m, 0.0,
+                                 dC.ptr, n) == 0, gpu_lib.last_error()
+        out[name] = dC.to_host().astype(np.float64)
+    G, H = out["gram"], out["general"]
+    assert np.isfinite(G).all()
+    assert np.array_equal(G, G.T)
+    exact = A.astype(np.float64).T @ A.astype(np.float64)
+    scale = np.sqrt(np.outer(np.diag(exact), np.diag(exact)))
+    tol = {0: 1e-13, 1: 2e-6, 3: 5e-5}[dtype]
+    assert (np.abs(G - H) / scale).max() <= tol
+    assert (np.abs(G - exact) / scale).max() <= tol
+
+
+@pytest.mark.gpu
 def test_fixed_rank_mp_ill_conditioned_basis(gpu_lib):
     """A steep spectrum, sigma_j = 10^(-3 j / 600) (the sketch basis Y = A Z
     has condition ~1e3, past what a split-bf16 CholeskyQR pass resolves): the
