@@ -460,8 +460,7 @@ int jacobi_f64(Context* c, int64_t L, int64_t n, double* W, int64_t ldw, double*
                          off.p);
       RSVD_LAUNCH_CHECK();
     }
-    RSVD_HIP(hipMemcpyAsync(&h, off.p, sizeof(h), hipMemcpyDeviceToHost, c->stream));
-    RSVD_HIP(hipStreamSynchronize(c->stream));
+    RSVD_TRY(read_small(c, &h, off.p, sizeof(h)));
     double mx;
     memcpy(&mx, &h, sizeof mx);
     if (!(mx > tol)) {

@@ -219,8 +219,7 @@ int qb_blocked_f64(Context* c, const double* M, int64_t m, int64_t n, int64_t ld
     steps = step + 1;
     if (tolMode) {                                                      // rra.c:1771-1778
       double ss = 0;
-      RSVD_HIP(hipMemcpyAsync(&ss, sums.p, sizeof ss, hipMemcpyDeviceToHost, c->stream));
-      RSVD_HIP(hipStreamSynchronize(c->stream));
+      RSVD_TRY(read_small(c, &ss, sums.p, sizeof ss));
       normval = sqrt(ss);
       if (normval < TOL) break;
     }

@@ -99,8 +99,7 @@ static int orth_f32(Context* c, int64_t m, int64_t n, float* Y, int64_t ldy, int
                          (const double*)(dev.p + 1), dev.p);
       RSVD_LAUNCH_CHECK();
       double h = 0.0;
-      RSVD_HIP(hipMemcpyAsync(&h, dev.p, sizeof h, hipMemcpyDeviceToHost, c->stream));
-      RSVD_HIP(hipStreamSynchronize(c->stream));
+      RSVD_TRY(read_small(c, &h, dev.p, sizeof h));
       if (!(h <= 0.25)) ++total;  // (NaN included)
     }
     int ok = 1;
@@ -384,16 +383,14 @@ int qb_blocked_mp(Context* c, const float* M, int64_t m, int64_t n, int64_t ldm,
     steps = step + 1;
     if (tolMode) {
       double ss = 0;
-      RSVD_HIP(hipMemcpyAsync(&ss, sums.p, sizeof ss, hipMemcpyDeviceToHost, c->stream));
-      RSVD_HIP(hipStreamSynchronize(c->stream));
+      RSVD_TRY(read_small(c, &ss, sums.p, sizeof ss));
       normval = sqrt(ss);
       if (normval < TOL) break;
     }
   }
   if (!tolMode && steps > 0) {
     double ss = 0;
-    RSVD_HIP(hipMemcpyAsync(&ss, sums.p, sizeof ss, hipMemcpyDeviceToHost, c->stream));
-    RSVD_HIP(hipStreamSynchronize(c->stream));
+    RSVD_TRY(read_small(c, &ss, sums.p, sizeof ss));
     normval = sqrt(ss);
   }
   if (frank) *frank = fr;
@@ -475,8 +472,7 @@ int lowrank_residual(Context* c, const double* A, int64_t m, int64_t n, int64_t 
     RSVD_TRY(gemm_f64(c, 'N', 'T', m, w, k, 1.0, Us.p, m, V64.p + j0, n, 0.0, P.p, m));
     RSVD_TRY(sumsq_diff_f64(c, m, w, A + j0 * lda, lda, P.p, m, part.p));
     double h = 0;
-    RSVD_HIP(hipMemcpyAsync(&h, part.p, sizeof h, hipMemcpyDeviceToHost, c->stream));
-    RSVD_HIP(hipStreamSynchronize(c->stream));
+    RSVD_TRY(read_small(c, &h, part.p, sizeof h));
     total += h;
   }
   *out = sqrt(total);

@@ -428,8 +428,7 @@ int chol_inv_upper_f64(Context* c, int64_t n, double* G, double* R, double* Rinv
   RSVD_HIP(hipMemsetAsync(fail.p, 0, sizeof(int), c->stream));
   RSVD_TRY(chol_inv_launch(c, n, G, R, Rinv, fail.p, diag0.p, acc.p));
   int h = 0;
-  RSVD_HIP(hipMemcpyAsync(&h, fail.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-  RSVD_HIP(hipStreamSynchronize(c->stream));
+  RSVD_TRY(read_small(c, &h, fail.p, sizeof(int)));
   *ok = h ? 0 : 1;
   return 0;
 }

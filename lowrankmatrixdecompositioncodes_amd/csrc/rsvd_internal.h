@@ -59,10 +59,17 @@ struct Context {
   struct Pending { int idx; hipEvent_t a, b; };
   std::vector<Pending> pending;
   std::vector<hipEvent_t> event_pool;
+  // pinned host slot for the small device -> host reads of the control flow
+  // (convergence and breakdown flags, norms); allocated on first use, kept
+  void* pinned = nullptr;
 };
 
 // Lazily initialises the current device and the library stream.
 int ctx_get(Context** out);
+
+// dst <- bytes (<= 64) at device address src, after the work queued on the
+// library stream: one direct copy into pinned host memory, then a wait.
+int read_small(Context* c, void* dst, const void* src, size_t bytes);
 
 // Device buffer from the pool (contents undefined).  free() returns it.
 int dmalloc(void** p, size_t bytes);

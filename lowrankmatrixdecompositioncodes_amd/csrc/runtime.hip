@@ -50,6 +50,15 @@ int ctx_get(Context** out) {
   return 0;
 }
 
+int read_small(Context* c, void* dst, const void* src, size_t bytes) {
+  if (bytes > 64) return set_error("read_small: %zu bytes", bytes);
+  if (!c->pinned) RSVD_HIP(hipHostMalloc(&c->pinned, 64, hipHostMallocDefault));
+  RSVD_HIP(hipMemcpyAsync(c->pinned, src, bytes, hipMemcpyDeviceToHost, c->stream));
+  RSVD_HIP(hipStreamSynchronize(c->stream));
+  memcpy(dst, c->pinned, bytes);
+  return 0;
+}
+
 // ---------------------------------------------------------------- pool
 // Buffers are rounded up to 2 MiB granules above 2 MiB and to powers of two
 // below, and recycled best-fit.  All frees are deferred to the stream order

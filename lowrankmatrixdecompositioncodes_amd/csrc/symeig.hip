@@ -501,8 +501,7 @@ int run_sweeps(Context* c, int64_t n, double* G, int64_t ldg, double* V, int64_t
         jc = 1 - jc;
       }
       unsigned long long h = 0;
-      RSVD_HIP(hipMemcpyAsync(&h, flags.p + sweep, sizeof(h), hipMemcpyDeviceToHost, c->stream));
-      RSVD_HIP(hipStreamSynchronize(c->stream));
+      RSVD_TRY(read_small(c, &h, flags.p + sweep, sizeof(h)));
       // no pair met in this sweep was above kStopTol (kLooseStopTol)
       if ((h & (loose ? 2ULL : 1ULL)) == 0) {
         ++sweep;
