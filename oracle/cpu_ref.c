@@ -66,6 +66,19 @@ typedef void (*dgesvd_t)(const char *, const char *, const int *, const int *, d
                          const int *, double *, double *, const int *, double *, const int *,
                          double *, const int *, int *, size_t, size_t);
 typedef void (*setthreads_t)(int);
+/* single precision (the oneAPI variant, see the FP32 section below) */
+typedef void (*sgemm_t)(const char *, const char *, const int *, const int *, const int *,
+                        const float *, const float *, const int *, const float *, const int *,
+                        const float *, float *, const int *, size_t, size_t);
+typedef void (*sgeqrf_t)(const int *, const int *, float *, const int *, float *, float *,
+                         const int *, int *);
+typedef void (*sorgqr_t)(const int *, const int *, const int *, float *, const int *,
+                         const float *, float *, const int *, int *);
+typedef void (*ssyev_t)(const char *, const char *, const int *, float *, const int *, float *,
+                        float *, const int *, int *, size_t, size_t);
+typedef void (*sgesvd_t)(const char *, const char *, const int *, const int *, float *,
+                         const int *, float *, float *, const int *, float *, const int *,
+                         float *, const int *, int *, size_t, size_t);
 
 static dgemm_t p_dgemm;
 static dgeqrf_t p_dgeqrf;
@@ -73,6 +86,11 @@ static dorgqr_t p_dorgqr;
 static dsyev_t p_dsyev;
 static dgesvd_t p_dgesvd;
 static setthreads_t p_setthreads;
+static sgemm_t p_sgemm;
+static sgeqrf_t p_sgeqrf;
+static sorgqr_t p_sorgqr;
+static sgesvd_t p_sgesvd;
+static ssyev_t p_ssyev;
 static int g_backend = 0; /* 1 = MKL, 2 = OpenBLAS */
 static char g_backend_name[512];
 
@@ -89,6 +107,11 @@ static int bind_prefix(void *h, const char *pre) {
     BIND(p_dorgqr, dorgqr_t, "dorgqr_");
     BIND(p_dsyev, dsyev_t, "dsyev_");
     BIND(p_dgesvd, dgesvd_t, "dgesvd_");
+    BIND(p_sgemm, sgemm_t, "sgemm_");
+    BIND(p_sgeqrf, sgeqrf_t, "sgeqrf_");
+    BIND(p_sorgqr, sorgqr_t, "sorgqr_");
+    BIND(p_sgesvd, sgesvd_t, "sgesvd_");
+    BIND(p_ssyev, ssyev_t, "ssyev_");
 #undef BIND
     return 1;
 }
@@ -659,4 +682,369 @@ double oref_wall_seconds(void) {
     struct timeval tv;
     gettimeofday(&tv, NULL);
     return (double)tv.tv_sec + 1e-6 * (double)tv.tv_usec;
+}
+
+/* ------------------------------------------------------------------ */
+/* FP32 restatement of the oneAPI variant (second oracle)              */
+/* ------------------------------------------------------------------ */
+/* ONEAPI = oneapi_code/; its mat holds int64 dimensions and float data
+ * (ONEAPI/matrix_vector_functions_one_api.h:24-27) and every product, QR and
+ * SVD is the single-precision LAPACK / BLAS routine (sgemm, sgeqrf+sorgqr,
+ * sgesvd).  Restated here: randQB_pb2 (ONEAPI/rank_revealing_algorithms_one_api.c:605-813)
+ * and low_rank_svd_rand_decomp_fromQB (same file :244-304).  Omega is the
+ * seeded stream above (already float-valued) or an injected buffer, as for the
+ * double oracle.  The Frobenius norm of the TOL test follows
+ * ONEAPI/matrix_vector_functions_one_api.c:356-368: squares summed in float by
+ * an OpenMP reduction, restated as `g_norm_chunks` contiguous chunks (the
+ * static schedule of that many threads) each summed in float in index order,
+ * the chunk sums added in float in chunk order. */
+typedef struct {
+    int64_t nrows, ncols;
+    float *d;
+} mat32;
+
+static int g_norm_chunks = 16;
+void oref32_set_norm_chunks(int c) { g_norm_chunks = c > 0 ? c : 1; }
+
+mat32 *oref32_matrix_new(int64_t nrows, int64_t ncols) {
+    mat32 *M = (mat32 *)malloc(sizeof(mat32));
+    M->d = (float *)calloc((size_t)(nrows * ncols) + 1, sizeof(float));
+    M->nrows = nrows;
+    M->ncols = ncols;
+    return M;
+}
+void oref32_matrix_delete(mat32 *M) {
+    if (!M) return;
+    free(M->d);
+    free(M);
+}
+#define MEL32(M, i, j) ((M)->d[(size_t)(j) * (size_t)(M)->nrows + (size_t)(i)])
+
+static void gemm32(char ta, char tb, int m, int n, int k, mat32 *A, mat32 *B, mat32 *C) {
+    const float one = 1.0f, zero = 0.0f;
+    int lda = (int)OMAX(1, A->nrows), ldb = (int)OMAX(1, B->nrows), ldc = (int)OMAX(1, C->nrows);
+    if (m == 0 || n == 0) return;
+    p_sgemm(&ta, &tb, &m, &n, &k, &one, A->d, &lda, B->d, &ldb, &zero, C->d, &ldc, 1, 1);
+}
+static void mm32(mat32 *A, mat32 *B, mat32 *C) {  /* ONEAPI mvf matrix_matrix_mult */
+    gemm32('N', 'N', (int)A->nrows, (int)B->ncols, (int)A->ncols, A, B, C);
+}
+static void mtm32(mat32 *A, mat32 *B, mat32 *C) { /* matrix_transpose_matrix_mult */
+    gemm32('T', 'N', (int)A->ncols, (int)B->ncols, (int)A->nrows, A, B, C);
+}
+static void mmt32(mat32 *A, mat32 *B, mat32 *C) { /* matrix_matrix_transpose_mult */
+    gemm32('N', 'T', (int)A->nrows, (int)B->nrows, (int)A->ncols, A, B, C);
+}
+static void copy32(mat32 *D, mat32 *S) {
+    memcpy(D->d, S->d, (size_t)(S->nrows * S->ncols) * sizeof(float));
+}
+static void sub32(mat32 *A, mat32 *B) {
+    size_t N = (size_t)(A->nrows * A->ncols);
+    for (size_t i = 0; i < N; ++i) A->d[i] -= B->d[i];
+}
+float oref32_frobenius(const float *d, int64_t N) {
+    const int64_t T = g_norm_chunks;
+    float total = 0.0f;
+    for (int64_t t = 0; t < T; ++t) {
+        /* static schedule: chunk t covers [t*N/T, (t+1)*N/T) */
+        const int64_t lo = t * N / T, hi = (t + 1) * N / T;
+        float part = 0.0f;
+        for (int64_t i = lo; i < hi; ++i) part += d[i] * d[i];
+        total += part;
+    }
+    return sqrtf(total);
+}
+static void build_transpose32(mat32 *Mt, mat32 *M) {
+    for (int64_t i = 0; i < M->nrows; ++i)
+        for (int64_t j = 0; j < M->ncols; ++j) MEL32(Mt, j, i) = MEL32(M, i, j);
+}
+static void qr_getQ32(mat32 *M, mat32 *Q) {       /* ONEAPI mvf QR_factorization_getQ */
+    int m = (int)M->nrows, n = (int)M->ncols, k = (int)OMIN(m, n), lwork = -1, info = 0;
+    float wq;
+    copy32(Q, M);
+    float *tau = (float *)calloc((size_t)k + 1, sizeof(float));
+    p_sgeqrf(&m, &n, Q->d, &m, tau, &wq, &lwork, &info);
+    lwork = (int)wq + 1;
+    float *work = (float *)malloc(sizeof(float) * (size_t)lwork);
+    p_sgeqrf(&m, &n, Q->d, &m, tau, work, &lwork, &info);
+    free(work);
+    lwork = -1;
+    p_sorgqr(&m, &n, &n, Q->d, &m, tau, &wq, &lwork, &info);
+    lwork = (int)wq + 1;
+    work = (float *)malloc(sizeof(float) * (size_t)lwork);
+    p_sorgqr(&m, &n, &n, Q->d, &m, tau, work, &lwork, &info);
+    free(work);
+    free(tau);
+}
+static void svd32(mat32 *M, mat32 *U, float *svals, mat32 *Vt) { /* sgesvd('S','S') */
+    int m = (int)M->nrows, n = (int)M->ncols, k = (int)OMIN(m, n), lwork = -1, info = 0;
+    int ldu = OMAX(1, m), ldvt = OMAX(1, k);
+    float wq;
+    p_sgesvd("S", "S", &m, &n, M->d, &m, svals, U->d, &ldu, Vt->d, &ldvt, &wq, &lwork, &info,
+             1, 1);
+    lwork = (int)wq + 1;
+    float *work = (float *)malloc(sizeof(float) * (size_t)lwork);
+    p_sgesvd("S", "S", &m, &n, M->d, &m, svals, U->d, &ldu, Vt->d, &ldvt, work, &lwork, &info,
+             1, 1);
+    free(work);
+}
+static void resize_cols32(mat32 **M, int64_t k) {   /* first k columns */
+    mat32 *R = oref32_matrix_new((*M)->nrows, k);
+    memcpy(R->d, (*M)->d, (size_t)((*M)->nrows * k) * sizeof(float));
+    oref32_matrix_delete(*M);
+    *M = R;
+}
+static void resize_rows32(mat32 **M, int64_t k) {   /* first k rows */
+    mat32 *R = oref32_matrix_new(k, (*M)->ncols);
+    for (int64_t j = 0; j < R->ncols; ++j)
+        for (int64_t i = 0; i < k; ++i) MEL32(R, i, j) = MEL32(*M, i, j);
+    oref32_matrix_delete(*M);
+    *M = R;
+}
+
+/* ONEAPI mvf compute_evals_and_evecs_of_symm_matrix: ssyev('V','U') in place */
+static void syev32(mat32 *S, float *evals) {
+    int n = (int)S->nrows, lda = (int)OMAX(1, S->ncols), lwork = -1, info = 0;
+    float wq;
+    p_ssyev("V", "U", &n, S->d, &lda, evals, &wq, &lwork, &info, 1, 1);
+    lwork = (int)wq + 1;
+    float *work = (float *)malloc(sizeof(float) * (size_t)lwork);
+    p_ssyev("V", "U", &n, S->d, &lda, evals, work, &lwork, &info, 1, 1);
+    free(work);
+}
+/* keep columns [off, off + k) (and rows, for a square S) */
+static void keep_cols32(mat32 **M, int64_t off, int64_t k) {
+    mat32 *R = oref32_matrix_new((*M)->nrows, k);
+    memcpy(R->d, (*M)->d + (size_t)(off * (*M)->nrows), (size_t)((*M)->nrows * k) * sizeof(float));
+    oref32_matrix_delete(*M);
+    *M = R;
+}
+static void keep_square32(mat32 **S, int64_t off, int64_t k) {
+    mat32 *R = oref32_matrix_new(k, k);
+    for (int64_t j = 0; j < k; ++j)
+        for (int64_t i = 0; i < k; ++i) MEL32(R, i, j) = MEL32(*S, off + i, off + j);
+    oref32_matrix_delete(*S);
+    *S = R;
+}
+
+/* ONEAPI/rank_revealing_algorithms_one_api.c:72-240 low_rank_svd_rand_decomp_fixed_rank
+ * in single precision: Y = M RN (:95), q - 1 power iterations with QR on
+ * alternate half steps (:110-135), Q = qr(Y) (:143-144); vnum 1 (or > 2):
+ * B^T = M^T Q, compact QR, sgesvd of R-hat, U = Q V-hat, V = Q-hat U-hat, first
+ * k kept (:146-188); vnum 2: B = Q^T M, ssyev of the upper triangle of BB^T,
+ * sigma = sqrt(lambda) (ascending, no clamp), U = Q U-hat, V = B^T (U-hat
+ * Sigma^-1), last k kept (:190-234). */
+int oref32_fixed_rank(mat32 *M, int64_t k, int64_t p, int64_t vnum, int64_t q, int64_t s,
+                      mat32 **U, mat32 **S, mat32 **V) {
+    if (!p_sgemm) return -2;
+    int64_t m = M->nrows, n = M->ncols, l = k + p;
+    if (k <= 0 || l > OMIN(m, n) || s <= 0) return -1;
+    mat32 *RN = oref32_matrix_new(n, l);
+    {
+        int64_t N = n * l;
+        if (g_inj) {
+            if (g_inj_count < N) { oref32_matrix_delete(RN); return -3; }
+            for (int64_t i = 0; i < N; ++i) RN->d[i] = (float)g_inj[i];
+        } else {
+            double *tmp = (double *)malloc(sizeof(double) * (size_t)N);
+            oref_gaussian(g_seed, 0, N, tmp);
+            for (int64_t i = 0; i < N; ++i) RN->d[i] = (float)tmp[i];
+            free(tmp);
+        }
+    }
+    mat32 *Y = oref32_matrix_new(m, l);
+    mm32(M, RN, Y);
+    oref32_matrix_delete(RN);
+    mat32 *Z = oref32_matrix_new(n, l), *Yo = oref32_matrix_new(m, l), *Zo = oref32_matrix_new(n, l);
+    for (int64_t j = 1; j < q; ++j) {
+        if ((2 * j - 2) % s == 0) { qr_getQ32(Y, Yo); mtm32(M, Yo, Z); }
+        else mtm32(M, Y, Z);
+        if ((2 * j - 1) % s == 0) { qr_getQ32(Z, Zo); mm32(M, Zo, Y); }
+        else mm32(M, Z, Y);
+    }
+    oref32_matrix_delete(Z); oref32_matrix_delete(Yo); oref32_matrix_delete(Zo);
+    mat32 *Q = oref32_matrix_new(m, l);
+    qr_getQ32(Y, Q);
+    *U = oref32_matrix_new(m, l);
+    *S = oref32_matrix_new(l, l);
+    *V = oref32_matrix_new(n, l);
+    if (vnum == 1 || vnum > 2) {
+        mat32 *Bt = oref32_matrix_new(n, l);
+        mtm32(M, Q, Bt);
+        /* compact QR: R = triu(first l rows of the factored Bt), Q-hat = sorgqr */
+        mat32 *Qh = oref32_matrix_new(n, l), *Rh = oref32_matrix_new(l, l);
+        {
+            int mm_ = (int)n, nn_ = (int)l, lwork = -1, info = 0;
+            float wq;
+            copy32(Qh, Bt);
+            float *tau = (float *)calloc((size_t)l + 1, sizeof(float));
+            p_sgeqrf(&mm_, &nn_, Qh->d, &mm_, tau, &wq, &lwork, &info);
+            lwork = (int)wq + 1;
+            float *work = (float *)malloc(sizeof(float) * (size_t)lwork);
+            p_sgeqrf(&mm_, &nn_, Qh->d, &mm_, tau, work, &lwork, &info);
+            free(work);
+            for (int64_t i = 0; i < l; ++i)
+                for (int64_t jj = i; jj < l; ++jj) MEL32(Rh, i, jj) = MEL32(Qh, i, jj);
+            lwork = -1;
+            p_sorgqr(&mm_, &nn_, &nn_, Qh->d, &mm_, tau, &wq, &lwork, &info);
+            lwork = (int)wq + 1;
+            work = (float *)malloc(sizeof(float) * (size_t)lwork);
+            p_sorgqr(&mm_, &nn_, &nn_, Qh->d, &mm_, tau, work, &lwork, &info);
+            free(work);
+            free(tau);
+        }
+        mat32 *Uh = oref32_matrix_new(l, l), *Vht = oref32_matrix_new(l, l);
+        float *sv = (float *)calloc((size_t)l + 1, sizeof(float));
+        svd32(Rh, Uh, sv, Vht);
+        for (int64_t i = 0; i < l; ++i) MEL32(*S, i, i) = sv[i];
+        mmt32(Q, Vht, *U);
+        mm32(Qh, Uh, *V);
+        free(sv);
+        oref32_matrix_delete(Bt); oref32_matrix_delete(Qh); oref32_matrix_delete(Rh);
+        oref32_matrix_delete(Uh); oref32_matrix_delete(Vht);
+        keep_cols32(U, 0, k); keep_cols32(V, 0, k); keep_square32(S, 0, k);
+    } else {
+        mat32 *B = oref32_matrix_new(l, n), *BBt = oref32_matrix_new(l, l);
+        mtm32(Q, M, B);
+        mmt32(B, B, BBt);
+        mat32 *Uh = oref32_matrix_new(l, l);
+        for (int64_t i = 0; i < l; ++i)      /* matrix_copy_symmetric: upper triangle */
+            for (int64_t jj = i; jj < l; ++jj) MEL32(Uh, i, jj) = MEL32(BBt, i, jj);
+        float *ev = (float *)calloc((size_t)l + 1, sizeof(float));
+        syev32(Uh, ev);
+        for (int64_t i = 0; i < l; ++i) MEL32(*S, i, i) = sqrtf(ev[i]);
+        mm32(Q, Uh, *U);
+        mat32 *Sinv = oref32_matrix_new(l, l), *UhSinv = oref32_matrix_new(l, l);
+        for (int64_t i = 0; i < l; ++i) MEL32(Sinv, i, i) = 1.0f / MEL32(*S, i, i);
+        mm32(Uh, Sinv, UhSinv);
+        mtm32(B, UhSinv, *V);
+        free(ev);
+        oref32_matrix_delete(B); oref32_matrix_delete(BBt); oref32_matrix_delete(Uh);
+        oref32_matrix_delete(Sinv); oref32_matrix_delete(UhSinv);
+        keep_cols32(U, l - k, k); keep_cols32(V, l - k, k); keep_square32(S, l - k, k);
+    }
+    oref32_matrix_delete(Y);
+    oref32_matrix_delete(Q);
+    return 0;
+}
+
+/* ONEAPI/rank_revealing_algorithms_one_api.c:605-813 randQB_pb2: the blocked QB
+ * of randQB_pb_new in single precision (kstep clamp :618-620, TOL mode when
+ * nstep <= 0 :623-628, q power iterations with QR on alternate half steps
+ * :678-709, re-orthogonalisation against all earlier blocks on even steps
+ * :725-749, Bp = Qp^T A :757, A -= Qp Bp, frank = (step + 1) kstep :786 and
+ * the strict norm < TOL stop :788-794, first-frank resize in TOL mode
+ * :797-799). */
+int oref32_randQB_pb2(mat32 *M, int64_t kstep, int64_t nstep, float TOL, int64_t q, int64_t s,
+                      int64_t *frank, mat32 **Qo, mat32 **Bo, float *final_norm) {
+    if (!p_sgemm) return -2;
+    int64_t m = M->nrows, n = M->ncols, tolMode = 0;
+    if (kstep > (int64_t)(OMIN(m, n) / 2)) kstep = (int64_t)OMIN(m, n) / 10;
+    if (nstep <= 0) { tolMode = 1; if (kstep <= 0) return -1; nstep = (int64_t)(OMIN(m, n) / kstep); }
+    if (kstep <= 0 || nstep <= 0 || s <= 0) return -1;
+    int64_t l = kstep * nstep;
+    mat32 *RN = oref32_matrix_new(n, l);
+    {   /* initialize_random_matrix: float Gaussians, column-major */
+        int64_t N = n * l;
+        if (g_inj) {
+            if (g_inj_count < N) { oref32_matrix_delete(RN); return -3; }
+            for (int64_t i = 0; i < N; ++i) RN->d[i] = (float)g_inj[i];
+        } else {
+            double *tmp = (double *)malloc(sizeof(double) * (size_t)N);
+            oref_gaussian(g_seed, 0, N, tmp);
+            for (int64_t i = 0; i < N; ++i) RN->d[i] = (float)tmp[i];
+            free(tmp);
+        }
+    }
+    mat32 *A = oref32_matrix_new(m, n);
+    *Qo = oref32_matrix_new(m, l);
+    *Bo = oref32_matrix_new(l, n);
+    mat32 *RNp = oref32_matrix_new(n, kstep), *Yp = oref32_matrix_new(m, kstep);
+    mat32 *Qp = oref32_matrix_new(m, kstep), *Bp = oref32_matrix_new(kstep, n);
+    mat32 *QpBp = oref32_matrix_new(m, n), *AtQp = oref32_matrix_new(n, kstep);
+    mat32 *AtQp2 = oref32_matrix_new(n, kstep), *QptA = oref32_matrix_new(kstep, n);
+    copy32(A, M);
+    int64_t fr = 0;
+    float normval = 0.0f;
+    for (int64_t step = 0; step < nstep; ++step) {
+        const int64_t c0 = step * kstep;
+        memcpy(RNp->d, RN->d + (size_t)(c0 * n), (size_t)(n * kstep) * sizeof(float));
+        mm32(A, RNp, Yp);
+        for (int64_t j = 1; j <= q; ++j) {
+            if ((2 * j - 2) % s == 0) {
+                qr_getQ32(Yp, Qp);
+                mtm32(Qp, A, QptA);
+                build_transpose32(AtQp, QptA);
+            } else {
+                mtm32(A, Yp, AtQp);
+            }
+            if ((2 * j - 1) % s == 0) {
+                qr_getQ32(AtQp, AtQp2);
+                mm32(A, AtQp2, Yp);
+            } else {
+                mm32(A, AtQp, Yp);
+            }
+        }
+        qr_getQ32(Yp, Qp);
+        if (step > 0 && step % 2 == 0) {
+            const int64_t nj = step * kstep;
+            mat32 *Qj = oref32_matrix_new(m, nj), *QjtQp = oref32_matrix_new(nj, kstep);
+            mat32 *QjQjtQp = oref32_matrix_new(m, kstep);
+            memcpy(Qj->d, (*Qo)->d, (size_t)(m * nj) * sizeof(float));
+            mtm32(Qj, Qp, QjtQp);
+            mm32(Qj, QjtQp, QjQjtQp);
+            copy32(Yp, Qp);
+            sub32(Yp, QjQjtQp);
+            qr_getQ32(Yp, Qp);
+            oref32_matrix_delete(Qj); oref32_matrix_delete(QjtQp); oref32_matrix_delete(QjQjtQp);
+        }
+        mtm32(Qp, A, Bp);
+        mm32(Qp, Bp, QpBp);
+        sub32(A, QpBp);
+        memcpy((*Qo)->d + (size_t)(c0 * m), Qp->d, (size_t)(m * kstep) * sizeof(float));
+        for (int64_t jj = 0; jj < n; ++jj)
+            for (int64_t i = 0; i < kstep; ++i) MEL32(*Bo, c0 + i, jj) = MEL32(Bp, i, jj);
+        fr = (step + 1) * kstep;
+        if (tolMode) {
+            normval = oref32_frobenius(A->d, m * n);
+            if (normval < TOL) break;
+        }
+    }
+    if (tolMode) { resize_cols32(Qo, fr); resize_rows32(Bo, fr); }
+    if (frank) *frank = fr;
+    if (final_norm) *final_norm = tolMode ? normval : oref32_frobenius(A->d, m * n);
+    oref32_matrix_delete(A); oref32_matrix_delete(RN); oref32_matrix_delete(RNp);
+    oref32_matrix_delete(Yp); oref32_matrix_delete(Qp); oref32_matrix_delete(Bp);
+    oref32_matrix_delete(QpBp); oref32_matrix_delete(AtQp); oref32_matrix_delete(AtQp2);
+    oref32_matrix_delete(QptA);
+    return 0;
+}
+
+/* ONEAPI/rank_revealing_algorithms_one_api.c:244-304 low_rank_svd_rand_decomp_fromQB:
+ * BB^T (knew x knew, knew = Q->ncols), its SVD by sgesvd (:267), sigma_i =
+ * sqrt(St_ii) (descending, :275-279), U = Q Uhat (:284), V = B^T (Uhat
+ * Sigma^-1) with Uhat Sigma^-1 formed as a product (:291-293).  No
+ * truncation: U m x knew, S knew x knew, V n x knew. */
+int oref32_svd_fromQB(mat32 *Q, mat32 *B, mat32 **U, mat32 **S, mat32 **V) {
+    if (!p_sgemm) return -2;
+    int64_t m = Q->nrows, n = B->ncols, kn = Q->ncols;
+    if (B->nrows != kn) return -1;
+    *U = oref32_matrix_new(m, kn);
+    *S = oref32_matrix_new(kn, kn);
+    *V = oref32_matrix_new(n, kn);
+    mat32 *Uhat = oref32_matrix_new(kn, kn), *Vt = oref32_matrix_new(kn, kn);
+    mat32 *BBt = oref32_matrix_new(kn, kn);
+    mmt32(B, B, BBt);
+    float *st = (float *)calloc((size_t)kn + 1, sizeof(float));
+    svd32(BBt, Uhat, st, Vt);
+    for (int64_t i = 0; i < kn; ++i) MEL32(*S, i, i) = sqrtf(st[i]);
+    mm32(Q, Uhat, *U);
+    mat32 *Sinv = oref32_matrix_new(kn, kn), *UhatSinv = oref32_matrix_new(kn, kn);
+    for (int64_t i = 0; i < kn; ++i) MEL32(Sinv, i, i) = 1.0f / MEL32(*S, i, i);
+    mm32(Uhat, Sinv, UhatSinv);
+    mtm32(B, UhatSinv, *V);
+    free(st);
+    oref32_matrix_delete(BBt); oref32_matrix_delete(Vt); oref32_matrix_delete(Uhat);
+    oref32_matrix_delete(Sinv); oref32_matrix_delete(UhatSinv);
+    return 0;
 }

@@ -23,6 +23,14 @@ class Mat(C.Structure):
 
 
 MatP = C.POINTER(Mat)
+
+
+class Mat32(C.Structure):
+    """The oneAPI variant's mat: int64 dimensions, float data."""
+    _fields_ = [("nrows", C.c_int64), ("ncols", C.c_int64), ("d", C.POINTER(C.c_float))]
+
+
+Mat32P = C.POINTER(Mat32)
 _lib = None
 
 
@@ -85,6 +93,23 @@ def load(nthreads: int | None = None) -> C.CDLL:
         lib.oref_svd_residual.restype = C.c_double
         lib.oref_svd_residual.argtypes = [MatP, MatP, MatP, MatP]
         lib.oref_wall_seconds.restype = C.c_double
+        lib.oref32_matrix_new.restype = Mat32P
+        lib.oref32_matrix_new.argtypes = [C.c_int64, C.c_int64]
+        lib.oref32_matrix_delete.argtypes = [Mat32P]
+        lib.oref32_set_norm_chunks.argtypes = [C.c_int]
+        lib.oref32_frobenius.restype = C.c_float
+        lib.oref32_frobenius.argtypes = [C.POINTER(C.c_float), C.c_int64]
+        lib.oref32_randQB_pb2.restype = C.c_int
+        lib.oref32_randQB_pb2.argtypes = [Mat32P, C.c_int64, C.c_int64, C.c_float, C.c_int64,
+                                          C.c_int64, C.POINTER(C.c_int64), C.POINTER(Mat32P),
+                                          C.POINTER(Mat32P), C.POINTER(C.c_float)]
+        lib.oref32_fixed_rank.restype = C.c_int
+        lib.oref32_fixed_rank.argtypes = [Mat32P, C.c_int64, C.c_int64, C.c_int64, C.c_int64,
+                                          C.c_int64, C.POINTER(Mat32P), C.POINTER(Mat32P),
+                                          C.POINTER(Mat32P)]
+        lib.oref32_svd_fromQB.restype = C.c_int
+        lib.oref32_svd_fromQB.argtypes = [Mat32P, Mat32P, C.POINTER(Mat32P), C.POINTER(Mat32P),
+                                          C.POINTER(Mat32P)]
         nt = nthreads if nthreads is not None else int(os.environ.get("OREF_THREADS", "0"))
         if lib.oref_init(nt, _openblas_path().encode()) == 0:
             raise RuntimeError("oracle: no BLAS/LAPACK library could be bound")
@@ -213,3 +238,80 @@ def blockrand(M, k, p, TOL, vnum, kstep, q, s, seed=777, omega=None):
 
 def wall() -> float:
     return load().oref_wall_seconds()
+
+
+# ---- FP32 second oracle: the oneAPI variant (randQB_pb2, fromQB) ----
+
+def _to_mat32(a: np.ndarray) -> Mat32P:
+    lib = load()
+    a = np.asarray(a, dtype=np.float32)
+    M = lib.oref32_matrix_new(a.shape[0], a.shape[1])
+    if a.size:
+        buf = np.asfortranarray(a)
+        C.memmove(M.contents.d, buf.ctypes.data, 4 * a.size)
+    return M
+
+
+def _from_mat32(M: Mat32P) -> np.ndarray:
+    lib = load()
+    m, n = M.contents.nrows, M.contents.ncols
+    out = np.empty((m, n), dtype=np.float32, order="F")
+    if m * n:
+        C.memmove(out.ctypes.data, M.contents.d, 4 * m * n)
+    lib.oref32_matrix_delete(M)
+    return np.ascontiguousarray(out)
+
+
+def frobenius_f32(a: np.ndarray, chunks: int = 16) -> float:
+    """The FP32 variant's Frobenius norm: float sums over `chunks` contiguous
+    chunks of the column-major data, added in chunk order."""
+    lib = load()
+    lib.oref32_set_norm_chunks(chunks)
+    buf = np.asfortranarray(np.asarray(a, dtype=np.float32)).ravel(order="F")
+    return float(lib.oref32_frobenius(buf.ctypes.data_as(C.POINTER(C.c_float)), buf.size))
+
+
+def randQB_pb2(M, kstep, nstep, TOL, q, s, seed=777, omega=None, norm_chunks=16):
+    """FP32 oracle of randQB_pb2 (oneAPI variant); returns (Q, B, frank,
+    final_norm) as float32 arrays / Python scalars."""
+    lib = load()
+    lib.oref32_set_norm_chunks(norm_chunks)
+    Mm = _to_mat32(M)
+    Q, B = Mat32P(), Mat32P()
+    fr = C.c_int64(0)
+    nrm = C.c_float(0)
+    with _Omega(seed, omega):
+        rc = lib.oref32_randQB_pb2(Mm, kstep, nstep, TOL, q, s, C.byref(fr), C.byref(Q),
+                                   C.byref(B), C.byref(nrm))
+    lib.oref32_matrix_delete(Mm)
+    if rc:
+        raise ValueError(f"oref32_randQB_pb2 rc={rc}")
+    return _from_mat32(Q), _from_mat32(B), fr.value, float(nrm.value)
+
+
+def svd_fromQB_f32(Q, B):
+    """FP32 oracle of low_rank_svd_rand_decomp_fromQB (oneAPI variant):
+    returns (U, S, V), S knew x knew diagonal, values descending."""
+    lib = load()
+    Qm, Bm = _to_mat32(Q), _to_mat32(B)
+    U, S, V = Mat32P(), Mat32P(), Mat32P()
+    rc = lib.oref32_svd_fromQB(Qm, Bm, C.byref(U), C.byref(S), C.byref(V))
+    lib.oref32_matrix_delete(Qm)
+    lib.oref32_matrix_delete(Bm)
+    if rc:
+        raise ValueError(f"oref32_svd_fromQB rc={rc}")
+    return _from_mat32(U), _from_mat32(S), _from_mat32(V)
+
+
+def fixed_rank_f32(M, k, p, vnum, q, s, seed=777, omega=None):
+    """FP32 oracle of the oneAPI variant's low_rank_svd_rand_decomp_fixed_rank;
+    returns float32 (U, S, V)."""
+    lib = load()
+    Mm = _to_mat32(M)
+    U, S, V = Mat32P(), Mat32P(), Mat32P()
+    with _Omega(seed, omega):
+        rc = lib.oref32_fixed_rank(Mm, k, p, vnum, q, s, C.byref(U), C.byref(S), C.byref(V))
+    lib.oref32_matrix_delete(Mm)
+    if rc:
+        raise ValueError(f"oref32_fixed_rank rc={rc}")
+    return _from_mat32(U), _from_mat32(S), _from_mat32(V)
