@@ -10,7 +10,9 @@ iteration, orthonormalisations, small eigenproblem, U and V formation).
 value = F_alg / seconds per step, F_alg the reference algorithm's nominal flop
 count (8 m n l for the four A passes plus the QR / eigen / U / V terms).
 The roofline entry covers the dominant kernel family (the A-streaming GEMMs),
-timed with HIP events on the library's stream inside the timed region.
+timed with HIP events on the library's stream inside the timed region, against
+the algorithmic work of one pass (2 m n l flops over 4 m n bytes of a float32
+A: HBM-bound; 8 m n bytes of a double A: FP64 matrix-core bound).
 The CPU baseline runs the oracle (clean-room restatement of the reference on
 MKL / OpenBLAS) on the same matrix on this host, rank 0, N = 1 only.
 
@@ -68,11 +70,36 @@ def measured_traffic(tag, m, n, l):
     return int(round((2 * vals["nn"] + vals["tn"]) / 3))
 
 
-def physical_cores() -> int:
+def physical_cores() -> tuple[int, int]:
+    """(threads for the CPU baseline, physical cores of the host).
+
+    Physical cores are the distinct (core, socket) pairs `lscpu -p` lists;
+    the baseline runs one thread per physical core this process may run on
+    (its CPU affinity), and no more than OMP_NUM_THREADS when the environment
+    sets it (the host's share for this job)."""
+    import subprocess
     try:
-        return len(os.sched_getaffinity(0))
+        allowed = os.sched_getaffinity(0)
     except Exception:
-        return os.cpu_count() or 1
+        allowed = set(range(os.cpu_count() or 1))
+    host, mine = set(), set()
+    try:
+        out = subprocess.run(["lscpu", "-p=CPU,CORE,SOCKET"], capture_output=True, text=True,
+                             timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("#") or not line.strip():
+                continue
+            cpu, core, sock = (int(x) if x else 0 for x in line.split(",")[:3])
+            host.add((sock, core))
+            if cpu in allowed:
+                mine.add((sock, core))
+    except Exception:
+        pass
+    n = len(mine) or len(allowed)
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(n, 1), len(host) or n
 
 
 def _setup_lib(rp):
@@ -171,7 +198,8 @@ def bench_batched(args, rp, lib) -> int:
     cpu = None
     if not args.no_cpu:
         from oracle import ref
-        threads = args.cpu_threads or min(physical_cores(), 16)
+        threads, host_cores = physical_cores()
+        threads = args.cpu_threads or threads
         ref.load(threads)
         mats = [(i, b.matrix(i).astype(np.float64)) for i in range(0, cnt, 16)]
         t0 = time.perf_counter()
@@ -185,7 +213,7 @@ def bench_batched(args, rp, lib) -> int:
                 break
         cs = time.perf_counter() - t0
         cpu = {"value": round(done / cs, 2), "unit": "solves/s", "cores": threads,
-               "kind": "port", "sample": f"oracle (FP64, {ref.backend().split(':')[0]}) one call "
+               "host_physical_cores": host_cores, "kind": "port", "sample": f"oracle (FP64, {ref.backend().split(':')[0]}) one call "
                                          f"per matrix on {done} matrices of the batch "
                                          f"(every 16th), same Omega"}
     line = {
@@ -507,28 +535,29 @@ def main() -> int:
     # ---- roofline of the dominant A-streaming kernel ----
     t_ms, cnt = ks[tag]
     avg = t_ms / max(cnt, 1)
-    flops = 2.0 * m * n * l * (3 if tag == "apass_bf16x3" else 1)
+    # algorithmic work of one pass: 2 m n l flops over one read of A
+    flops_alg = 2.0 * m * n * l
     if args.precision == "fp64":
-        roofline = {"bound": "mfma", "achieved": round(flops / (avg * 1e-3) / 1e12, 2),
+        # 8 m n bytes, 2 l / 8 = 136 flop/B, above the FP64 ridge (78.6e12 / 8e12)
+        roofline = {"bound": "mfma", "achieved": round(flops_alg / (avg * 1e-3) / 1e12, 2),
                     "peak": PEAK_TFLOPS["fp64"], "unit": "TFLOP/s"}
-    elif tag == "apass_bf16x3":
-        # split-bf16: three bf16 products per fp32-accurate product, fp32 A
-        # streamed once: 3 * 2 l / 4 = 816 flop/B, above the bf16 ridge
-        # (2.5e15 / 8e12 = 312): bound by the matrix cores
-        roofline = {"bound": "mfma", "achieved": round(flops / (avg * 1e-3) / 1e12, 1),
-                    "peak": PEAK_TFLOPS["bf16"], "unit": "TFLOP/s",
-                    "hbm_gbs": round(4.0 * m * n / (avg * 1e-3) / 1e9, 1),
-                    "algorithmic_tflops": round(2.0 * m * n * l / (avg * 1e-3) / 1e12, 1)}
     else:
-        # fp32 A streamed and rounded (bf16) or scaled and quantised (fp8 e4m3)
-        # in-kernel: 4 m n bytes per pass, arithmetic intensity 2 l / 4 = 272
-        # flop/B, below the bf16 (2.5e15 / 8e12) and fp8 (5e15 / 8e12) ridges
+        # float32 A (4 m n bytes) against 2 m n l flops: 2 l / 4 = 272 flop/B,
+        # below the bf16 (2.5e15 / 8e12 = 312) and fp8 ridges, so the pass is
+        # bound by HBM; floor = max(4 m n / 8e12, 2 m n l / peak) = 4 m n / 8e12
         gbs = 4.0 * m * n / (avg * 1e-3) / 1e9
         roofline = {"bound": "hbm", "achieved": round(gbs, 1), "peak": 8000.0, "unit": "GB/s",
-                    "mfma_tflops": round(flops / (avg * 1e-3) / 1e12, 1)}
+                    "algorithmic_tflops": round(flops_alg / (avg * 1e-3) / 1e12, 1)}
+        # what the matrix cores actually execute: split-bf16 issues three bf16
+        # products (hi hi, hi lo, lo hi) per algorithmic product
+        issued = flops_alg * (3 if tag == "apass_bf16x3" else 1)
+        peak_mfma = PEAK_TFLOPS["fp8" if tag == "apass_fp8" else "bf16"]
+        roofline["mfma_issued"] = {"tflops": round(issued / (avg * 1e-3) / 1e12, 1),
+                                   "peak": peak_mfma,
+                                   "frac": round(issued / (avg * 1e-3) / 1e12 / peak_mfma, 4)}
     roofline["frac"] = round(roofline["achieved"] / roofline["peak"], 4)
     roofline["traffic"] = measured_traffic(tag, m, n, l)
-    roofline["kernel"] = f"{tag} ({'3x ' if tag.endswith('x3') else ''}2*m*n*l flops per launch)"
+    roofline["kernel"] = f"{tag} (one pass: 2*m*n*l flops, 4*m*n bytes of A)"
     roofline["avg_launch_ms"] = round(avg, 4)
     roofline["launches"] = cnt
     breakdown = {name: round(v[0] / args.steps if name != "apass_f64" or args.precision == "fp64"
@@ -545,7 +574,8 @@ def main() -> int:
     cpu = None
     if not args.no_cpu:
         from oracle import ref
-        threads = args.cpu_threads or min(physical_cores(), 16)
+        threads, host_cores = physical_cores()
+        threads = args.cpu_threads or threads
         ref.load(threads)
         ms_ = min(args.cpu_rows, m)
         log(f"cpu baseline: oracle on {threads} threads ({ref.backend()}), rows 0..{ms_ - 1}")
@@ -555,7 +585,8 @@ def main() -> int:
         ref.fixed_rank(A_host, k, p, 2, q, 1, omega=Om)
         cpu_s = time.perf_counter() - t0
         cpu = {"value": round(falg(ms_, n, l, q) / cpu_s / 1e12, 4), "unit": "TFLOP/s",
-               "cores": threads, "kind": "port", "seconds": round(cpu_s, 3),
+               "cores": threads, "host_physical_cores": host_cores, "kind": "port",
+               "seconds": round(cpu_s, 3),
                "sample": f"one call of the oracle (FP64, {ref.backend().split(':')[0]}) on rows "
                          f"0..{ms_ - 1} of the same A ({ms_}x{n}), k={k}, p={p}, q={q}, same "
                          f"Omega; rate = F_alg({ms_},{n},{l},{q}) / seconds"}
