@@ -40,6 +40,28 @@ int rsvd_set_omega(const double *host, long long count);
  * on the GPU and copied to host. */
 int rsvd_gaussian(unsigned long long seed, long long offset, long long count, double *host_out);
 
+/* ---- binary matrix files ----
+ * The reference's file layout: two integers (rows, columns), then the
+ * entries row after row.  format 0: int32 header, double entries (MKL/mvf.c:77-133,
+ * make_matrix_binary.m:32-41; what matrix_load_from_binary_file reads);
+ * format 1: int64 header, double entries
+ * (multi_core_mkl_code_64bit/matrix_vector_functions_intel_mkl.c:78-101);
+ * format 2: int64 header, float entries
+ * (oneapi_code/matrix_vector_functions_one_api.c:71-95). */
+/* Rows and columns from the header of a file in `format`. */
+int rsvd_binary_file_shape(const char *fname, int format, long long *m, long long *n);
+/* Loads a file straight into a device matrix (column-major, lda >= rows), as
+ * double (dtype 0) or float (dtype 1): blocks of rows are read into two
+ * pinned host buffers in turn, copied to HBM and transposed on the device, the
+ * read of one block overlapping the copy and transpose of the previous one.
+ * Float entries widen exactly to double; double entries round to float for
+ * dtype 1. */
+int rsvd_dev_load_binary(const char *fname, int format, int dtype, void *dA, long long lda);
+/* Writes an m x n device matrix (double: dtype 0, float: 1; column-major,
+ * lda) to a file in `format` (double entries round to float for format 2). */
+int rsvd_dev_write_binary(const char *fname, int format, int dtype, const void *dA, long long m,
+                          long long n, long long lda);
+
 /* ---- device memory helpers ---- */
 void *rsvd_dev_alloc(size_t bytes);
 void rsvd_dev_free(void *p);

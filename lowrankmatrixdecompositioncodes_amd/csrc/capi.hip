@@ -103,6 +103,46 @@ struct Api {
   }
 };
 
+// Entry of a routine that reads or writes caller device memory.  The library
+// stream is non-blocking, so the work the caller queued on the null stream
+// before the call (and, through it, on every blocking stream) is made to
+// precede the call's own work; every such routine also ends with a stream
+// synchronisation, so its outputs are complete when it returns.
+struct DevApi : Api {
+  DevApi() {
+    if (!rc) rc = order_after_null_stream(c);
+  }
+};
+
+// Device outputs of a decomposition that the library allocates for the
+// caller (U, S, V): all three or none; freed again unless release() hands
+// them over.
+struct OutBufs {
+  void* p[3] = {nullptr, nullptr, nullptr};
+  int alloc(size_t b0, size_t b1, size_t b2) {
+    const size_t b[3] = {b0, b1, b2};
+    for (int i = 0; i < 3; ++i) {
+      hipError_t e = hipMalloc(&p[i], b[i] ? b[i] : 1);
+      if (e != hipSuccess) {
+        p[i] = nullptr;
+        return set_error("hipMalloc(%zu) for an output failed: %s", b[i], hipGetErrorString(e));
+      }
+    }
+    return 0;
+  }
+  template <typename TU, typename TV>
+  void release(TU** u, double** s, TV** v) {
+    *u = (TU*)p[0];
+    *s = (double*)p[1];
+    *v = (TV*)p[2];
+    p[0] = p[1] = p[2] = nullptr;
+  }
+  ~OutBufs() {
+    for (void* q : p)
+      if (q) (void)hipFree(q);
+  }
+};
+
 }  // namespace
 
 extern "C" {
@@ -149,45 +189,7 @@ double matrix_get_element(mat* M, int row_num, int col_num) {
 void vector_set_element(vec* v, int row_num, double val) { v->d[row_num] = val; }
 double vector_get_element(vec* v, int row_num) { return v->d[row_num]; }
 
-mat* matrix_load_from_binary_file(char* fname) {
-  clear_error();
-  FILE* fp = fopen(fname, "rb");
-  if (!fp) { set_error("cannot open %s", fname); return nullptr; }
-  int m = 0, n = 0;
-  if (fread(&m, sizeof(int), 1, fp) != 1 || fread(&n, sizeof(int), 1, fp) != 1 || m < 0 || n < 0) {
-    fclose(fp);
-    set_error("%s: bad header", fname);
-    return nullptr;
-  }
-  mat* M = matrix_new(m, n);
-  std::vector<double> row((size_t)(n > 0 ? n : 1));
-  for (int i = 0; i < m; ++i) {
-    if (fread(row.data(), sizeof(double), (size_t)n, fp) != (size_t)n) {
-      fclose(fp);
-      matrix_delete(M);
-      set_error("%s: truncated at row %d", fname, i);
-      return nullptr;
-    }
-    for (int j = 0; j < n; ++j) M->d[(size_t)j * m + i] = row[j];
-  }
-  fclose(fp);
-  return M;
-}
-
-void matrix_write_to_binary_file(mat* M, char* fname) {
-  clear_error();
-  FILE* fp = fopen(fname, "wb");
-  if (!fp) { set_error("cannot open %s", fname); return; }
-  int m = M->nrows, n = M->ncols;
-  fwrite(&m, sizeof(int), 1, fp);
-  fwrite(&n, sizeof(int), 1, fp);
-  std::vector<double> row((size_t)(n > 0 ? n : 1));
-  for (int i = 0; i < m; ++i) {
-    for (int j = 0; j < n; ++j) row[j] = M->d[(size_t)j * m + i];
-    fwrite(row.data(), sizeof(double), (size_t)n, fp);
-  }
-  fclose(fp);
-}
+// matrix_load_from_binary_file / matrix_write_to_binary_file: binio.hip
 
 void matrix_print(mat* M) {
   for (int i = 0; i < M->nrows; ++i) {
@@ -695,27 +697,28 @@ void rsvd_dev_free(void* p) {
   (void)hipFree(p);
 }
 int rsvd_memcpy_h2d(void* dst, const void* src, size_t bytes) {
-  Api api;
+  DevApi api;
   if (api.rc) return api.rc;
   RSVD_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, api.c->stream));
   return stream_sync(api.c);
 }
 int rsvd_memcpy_d2h(void* dst, const void* src, size_t bytes) {
-  Api api;
+  DevApi api;
   if (api.rc) return api.rc;
   RSVD_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, api.c->stream));
   return stream_sync(api.c);
 }
 int rsvd_memset_dev(void* dst, int value, size_t bytes) {
-  Api api;
+  DevApi api;
   if (api.rc) return api.rc;
   RSVD_HIP(hipMemsetAsync(dst, value, bytes, api.c->stream));
-  return 0;
+  return stream_sync(api.c);
 }
 int rsvd_dev_gaussian(unsigned long long seed, long long m, long long n, double* dst, long long ld) {
-  Api api;
+  DevApi api;
   if (api.rc) return api.rc;
-  return gaussian_f64(api.c, seed, m, n, 0, m, dst, ld);
+  RSVD_TRY(gaussian_f64(api.c, seed, m, n, 0, m, dst, ld));
+  return stream_sync(api.c);
 }
 int rsvd_sync(void) {
   Api api;
@@ -732,13 +735,15 @@ void rsvd_release_cache(void) {
 int rsvd_dev_fixed_rank(const double* dA, long long m, long long n, long long lda, int k, int p,
                         int vnum, int q, int s, unsigned long long seed, const double* dOmega,
                         double* dU, long long ldu, double* dS, double* dV, long long ldv) {
-  Api api;
+  DevApi api;
   if (api.rc) return api.rc;
   OmegaSrc om;
   om.seed = seed;
   om.dev = dOmega;
   om.ld = n;
-  return fixed_rank_f64(api.c, dA, m, n, lda, k, p, vnum, q, s, om, dU, ldu, dS, dV, ldv, nullptr);
+  RSVD_TRY(fixed_rank_f64(api.c, dA, m, n, lda, k, p, vnum, q, s, om, dU, ldu, dS, dV, ldv,
+                          nullptr));
+  return stream_sync(api.c);
 }
 
 long long rsvd_qb_max_cols(long long m, long long n, int kstep, int nstep, int is_new) {
@@ -751,7 +756,7 @@ int rsvd_dev_randqb(const double* dA, long long m, long long n, long long lda, i
                     double TOL, int q, int s, int is_new, unsigned long long seed, const double* dRN,
                     double* dQ, long long ldq, double* dB, long long ldb, int* frank,
                     double* final_norm) {
-  Api api;
+  DevApi api;
   if (api.rc) return api.rc;
   OmegaSrc om;
   om.seed = seed;
@@ -769,43 +774,43 @@ int rsvd_dev_randqb(const double* dA, long long m, long long n, long long lda, i
 int rsvd_dev_blockrand(const double* dA, long long m, long long n, long long lda, int k, int p,
                        double TOL, int vnum, int kstep, int q, int s, unsigned long long seed,
                        const double* dRN, int* frank, double** dU, double** dS, double** dV) {
-  Api api;
+  DevApi api;
   if (api.rc) return api.rc;
   OmegaSrc om;
   om.seed = seed;
   om.dev = dRN;
   om.ld = n;
+  if (!frank || !dU || !dS || !dV) return set_error("rsvd_dev_blockrand: null output pointer");
+  *dU = *dS = *dV = nullptr;
   BlockrandOut out;
   RSVD_TRY(blockrand_f64(api.c, dA, m, n, lda, k, p, TOL, vnum, kstep, q, s, om, &out));
   const int fr = out.frank;
-  double *u = nullptr, *sv = nullptr, *v = nullptr;
-  RSVD_HIP(hipMalloc(&u, sizeof(double) * (size_t)m * fr));
-  RSVD_HIP(hipMalloc(&sv, sizeof(double) * (size_t)fr));
-  RSVD_HIP(hipMalloc(&v, sizeof(double) * (size_t)n * fr));
-  RSVD_HIP(hipMemcpyAsync(u, out.U.p, sizeof(double) * (size_t)m * fr, hipMemcpyDeviceToDevice, api.c->stream));
-  RSVD_HIP(hipMemcpyAsync(sv, out.S.p, sizeof(double) * (size_t)fr, hipMemcpyDeviceToDevice, api.c->stream));
-  RSVD_HIP(hipMemcpyAsync(v, out.V.p, sizeof(double) * (size_t)n * fr, hipMemcpyDeviceToDevice, api.c->stream));
+  OutBufs ob;
+  RSVD_TRY(ob.alloc(sizeof(double) * (size_t)m * fr, sizeof(double) * (size_t)fr,
+                    sizeof(double) * (size_t)n * fr));
+  RSVD_HIP(hipMemcpyAsync(ob.p[0], out.U.p, sizeof(double) * (size_t)m * fr,
+                          hipMemcpyDeviceToDevice, api.c->stream));
+  RSVD_HIP(hipMemcpyAsync(ob.p[1], out.S.p, sizeof(double) * (size_t)fr, hipMemcpyDeviceToDevice,
+                          api.c->stream));
+  RSVD_HIP(hipMemcpyAsync(ob.p[2], out.V.p, sizeof(double) * (size_t)n * fr,
+                          hipMemcpyDeviceToDevice, api.c->stream));
   RSVD_TRY(stream_sync(api.c));
   *frank = fr;
-  *dU = u;
-  *dS = sv;
-  *dV = v;
+  ob.release(dU, dS, dV);
   return 0;
 }
 
-int rsvd_dev_gemm(int dtype, char ta, char tb, long long M, long long N, long long K, double alpha,
-                  const void* A, long long lda, const void* B, long long ldb, double beta, void* C,
-                  long long ldc) {
-  Api api;
-  if (api.rc) return api.rc;
+static int dev_gemm(Context* c, int dtype, char ta, char tb, long long M, long long N, long long K,
+                    double alpha, const void* A, long long lda, const void* B, long long ldb,
+                    double beta, void* C, long long ldc) {
   if (dtype == 0)
-    return gemm_f64(api.c, ta, tb, M, N, K, alpha, (const double*)A, lda, (const double*)B, ldb,
+    return gemm_f64(c, ta, tb, M, N, K, alpha, (const double*)A, lda, (const double*)B, ldb,
                     beta, (double*)C, ldc);
   if (dtype == 1)
-    return gemm_f32(api.c, ta, tb, M, N, K, (float)alpha, (const float*)A, lda, (const float*)B,
+    return gemm_f32(c, ta, tb, M, N, K, (float)alpha, (const float*)A, lda, (const float*)B,
                     ldb, (float)beta, (float*)C, ldc);
   if (dtype == 2 || dtype == 3)
-    return gemm_bf16(api.c, "gemm_bf16", ta, tb, M, N, K, (float)alpha, (const float*)A, lda,
+    return gemm_bf16(c, "gemm_bf16", ta, tb, M, N, K, (float)alpha, (const float*)A, lda,
                      (const float*)B, ldb, (float)beta, (float*)C, ldc, dtype == 2 ? 1 : 3);
   if (dtype == 4) {
     if (tb != 'N' && tb != 'n') return set_error("rsvd_dev_gemm: fp8 needs tb = 'N'");
@@ -813,14 +818,23 @@ int rsvd_dev_gemm(int dtype, char ta, char tb, long long M, long long N, long lo
     const int64_t ar = t ? K : M, ac = t ? M : K;  // A is ar x ac
     DBuf<float> sc;
     RSVD_TRY(sc.alloc((size_t)((ac + 127) / 128 + 1)));
-    RSVD_TRY(fp8_column_scales(api.c, ar, ac, (const float*)A, lda, sc.p));
-    return apass_fp8(api.c, "gemm_fp8", t, M, N, K, (float)alpha, (const float*)A, lda, sc.p,
+    RSVD_TRY(fp8_column_scales(c, ar, ac, (const float*)A, lda, sc.p));
+    return apass_fp8(c, "gemm_fp8", t, M, N, K, (float)alpha, (const float*)A, lda, sc.p,
                      (const float*)B, ldb, (float)beta, (float*)C, ldc);
   }
   if (dtype == 5)
-    return gemm_bf16_general(api.c, "gemm_bf16x6", ta, tb, M, N, K, (float)alpha, (const float*)A,
+    return gemm_bf16_general(c, "gemm_bf16x6", ta, tb, M, N, K, (float)alpha, (const float*)A,
                              lda, (const float*)B, ldb, (float)beta, (float*)C, ldc, 6);
   return set_error("rsvd_dev_gemm: dtype %d not supported", dtype);
+}
+
+int rsvd_dev_gemm(int dtype, char ta, char tb, long long M, long long N, long long K, double alpha,
+                  const void* A, long long lda, const void* B, long long ldb, double beta, void* C,
+                  long long ldc) {
+  DevApi api;
+  if (api.rc) return api.rc;
+  RSVD_TRY(dev_gemm(api.c, dtype, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc));
+  return stream_sync(api.c);
 }
 
 int rsvd_dev_fixed_rank_batched(int count, const int* m, const int* n, const int* k, int p, int q,
@@ -828,7 +842,7 @@ int rsvd_dev_fixed_rank_batched(int count, const int* m, const int* n, const int
                                 float* dU, const long long* u_off, double* dS,
                                 const long long* s_off, float* dV, const long long* v_off,
                                 int* status) {
-  Api api;
+  DevApi api;
   if (api.rc) return api.rc;
   RSVD_TRY(fixed_rank_batched(api.c, count, m, n, k, p, q, seed, dA, a_off, dU, u_off, dS, s_off,
                               dV, v_off, status));
@@ -839,7 +853,7 @@ const char* rsvd_batched_last_config(void) { return batched_last_config(); }
 
 int rsvd_dev_batch_test_matrices(int count, const int* m, const int* n, const double* tau,
                                  unsigned long long seed, float* dA, const long long* a_off) {
-  Api api;
+  DevApi api;
   if (api.rc) return api.rc;
   if (count < 0 || (count > 0 && (!m || !n || !tau || !a_off || !dA)))
     return set_error("rsvd_dev_batch_test_matrices: bad arguments");
@@ -861,14 +875,14 @@ int rsvd_dev_batch_test_matrices(int count, const int* m, const int* n, const do
 int rsvd_mlp_num_params(void) { return mlp_num_params(); }
 
 int rsvd_dev_mlp_init(unsigned long long seed, float* dTheta) {
-  Api api;
+  DevApi api;
   if (api.rc) return api.rc;
   RSVD_TRY(mlp_init(api.c, seed, dTheta));
   return stream_sync(api.c);
 }
 
 int rsvd_dev_mlp_forward(const float* dTheta, const float* dX, int batch, float* dOut) {
-  Api api;
+  DevApi api;
   if (api.rc) return api.rc;
   RSVD_TRY(mlp_forward(api.c, dTheta, dX, batch, dOut));
   return stream_sync(api.c);
@@ -876,7 +890,7 @@ int rsvd_dev_mlp_forward(const float* dTheta, const float* dX, int batch, float*
 
 int rsvd_dev_mlp_grad(const float* dTheta, const float* dX, const int* dq, const float* dr,
                       float lam, int batch, float* dGrad, float* dLoss) {
-  Api api;
+  DevApi api;
   if (api.rc) return api.rc;
   RSVD_TRY(mlp_grad(api.c, dTheta, dX, nullptr, dq, dr, lam, batch, dGrad, dLoss));
   return stream_sync(api.c);
@@ -885,7 +899,7 @@ int rsvd_dev_mlp_grad(const float* dTheta, const float* dX, const int* dq, const
 int rsvd_dev_mlp_train(float* dTheta, const float* dX, const int* dq, const float* dr, int n,
                        float lam, int epochs, int batch, float lr, unsigned long long seed,
                        float* loss_hist) {
-  Api api;
+  DevApi api;
   if (api.rc) return api.rc;
   RSVD_TRY(mlp_train(api.c, dTheta, dX, dq, dr, n, lam, epochs, batch, lr, seed, loss_hist));
   return stream_sync(api.c);
@@ -893,7 +907,7 @@ int rsvd_dev_mlp_train(float* dTheta, const float* dX, const int* dq, const floa
 
 int rsvd_dev_sketch_features(const float* dA, long long m, long long n, long long lda,
                              unsigned long long seed, float* dFeat) {
-  Api api;
+  DevApi api;
   if (api.rc) return api.rc;
   RSVD_TRY(sketch_features(api.c, dA, m, n, lda, seed, dFeat));
   return stream_sync(api.c);
@@ -904,7 +918,7 @@ int rsvd_dev_learned_lowrank(const float* dTheta, const double* dA, const float*
                              double margin, int kstep, int q_fallback, unsigned long long seed,
                              int* frank, double** dU, double** dS, double** dV, int* q_pred,
                              int* k_pred, int* fell_back, double* resid_rel) {
-  Api api;
+  DevApi api;
   if (api.rc) return api.rc;
   Context* c = api.c;
   if (!dTheta || !dA || !dA32 || !frank || !dU || !dS || !dV)
@@ -959,40 +973,30 @@ int rsvd_dev_learned_lowrank(const float* dTheta, const double* dA, const float*
     om.ld = n;
   }
   if (try_fixed) {
-    RSVD_HIP(hipMalloc(&u, sizeof(double) * (size_t)(m * k)));
-    RSVD_HIP(hipMalloc(&sv, sizeof(double) * (size_t)k));
-    RSVD_HIP(hipMalloc(&v, sizeof(double) * (size_t)(n * k)));
-    int rc = fixed_rank_f64(c, dA, m, n, lda, (int)k, pp, 2, qq, 1, om, u, m, sv, v, n, nullptr);
+    OutBufs ob;
+    RSVD_TRY(ob.alloc(sizeof(double) * (size_t)(m * k), sizeof(double) * (size_t)k,
+                      sizeof(double) * (size_t)(n * k)));
+    u = (double*)ob.p[0];
+    sv = (double*)ob.p[1];
+    v = (double*)ob.p[2];
+    RSVD_TRY(fixed_rank_f64(c, dA, m, n, lda, (int)k, pp, 2, qq, 1, om, u, m, sv, v, n, nullptr));
     double s2 = 0, res = 0;
     if (full) {
       // ||A||^2 - sum s^2 cancels for an exact decomposition: form the
       // residual explicitly (one m x n x n product)
-      if (rc == 0) rc = lowrank_residual(c, dA, m, n, lda, k, u, m, sv, v, n, 0, &res);
+      RSVD_TRY(lowrank_residual(c, dA, m, n, lda, k, u, m, sv, v, n, 0, &res));
     } else {
-      if (rc == 0) rc = sumsq_f64(c, k, 1, sv, k, nrm.p);
-      if (rc == 0 && hipMemcpyAsync(&s2, nrm.p, sizeof s2, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
-        rc = set_error("learned: copy failed");
-      if (rc == 0) rc = stream_sync(c);
+      RSVD_TRY(sumsq_f64(c, k, 1, sv, k, nrm.p));
+      RSVD_TRY(read_small(c, &s2, nrm.p, sizeof s2));
       res = sqrt(a2 > s2 ? a2 - s2 : 0.0);
-    }
-    if (rc) {
-      (void)hipFree(u);
-      (void)hipFree(sv);
-      (void)hipFree(v);
-      return rc;
     }
     if (res <= tol_rel * sqrt(a2)) {
       *frank = (int)k;
-      *dU = u;
-      *dS = sv;
-      *dV = v;
+      ob.release(dU, dS, dV);
       if (fell_back) *fell_back = 0;
       if (resid_rel) *resid_rel = res / sqrt(a2);
       return 0;
     }
-    (void)hipFree(u);
-    (void)hipFree(sv);
-    (void)hipFree(v);
   }
   // fallback: the adaptive blocked method in TOL mode
   om.dev = nullptr;
@@ -1002,28 +1006,30 @@ int rsvd_dev_learned_lowrank(const float* dTheta, const double* dA, const float*
   RSVD_TRY(blockrand_f64(c, dA, m, n, lda, -1, p, tol_rel * sqrt(a2), 2, kstep, q_fallback, 1, om,
                          &bo));
   const int fr = bo.frank;
-  RSVD_HIP(hipMalloc(&u, sizeof(double) * (size_t)m * (fr > 0 ? fr : 1)));
-  RSVD_HIP(hipMalloc(&sv, sizeof(double) * (size_t)(fr > 0 ? fr : 1)));
-  RSVD_HIP(hipMalloc(&v, sizeof(double) * (size_t)n * (fr > 0 ? fr : 1)));
+  OutBufs ob;
+  RSVD_TRY(ob.alloc(sizeof(double) * (size_t)m * (fr > 0 ? fr : 1),
+                    sizeof(double) * (size_t)(fr > 0 ? fr : 1),
+                    sizeof(double) * (size_t)n * (fr > 0 ? fr : 1)));
   if (fr > 0) {
-    RSVD_HIP(hipMemcpyAsync(u, bo.U.p, sizeof(double) * (size_t)m * fr, hipMemcpyDeviceToDevice, c->stream));
-    RSVD_HIP(hipMemcpyAsync(sv, bo.S.p, sizeof(double) * (size_t)fr, hipMemcpyDeviceToDevice, c->stream));
-    RSVD_HIP(hipMemcpyAsync(v, bo.V.p, sizeof(double) * (size_t)n * fr, hipMemcpyDeviceToDevice, c->stream));
-    RSVD_TRY(sumsq_f64(c, fr, 1, sv, fr, nrm.p));
-    RSVD_HIP(hipMemcpyAsync(&s2, nrm.p, sizeof s2, hipMemcpyDeviceToHost, c->stream));
+    RSVD_HIP(hipMemcpyAsync(ob.p[0], bo.U.p, sizeof(double) * (size_t)m * fr,
+                            hipMemcpyDeviceToDevice, c->stream));
+    RSVD_HIP(hipMemcpyAsync(ob.p[1], bo.S.p, sizeof(double) * (size_t)fr,
+                            hipMemcpyDeviceToDevice, c->stream));
+    RSVD_HIP(hipMemcpyAsync(ob.p[2], bo.V.p, sizeof(double) * (size_t)n * fr,
+                            hipMemcpyDeviceToDevice, c->stream));
+    RSVD_TRY(sumsq_f64(c, fr, 1, (const double*)ob.p[1], fr, nrm.p));
+    RSVD_TRY(read_small(c, &s2, nrm.p, sizeof s2));
   }
   RSVD_TRY(stream_sync(c));
   *frank = fr;
-  *dU = u;
-  *dS = sv;
-  *dV = v;
+  ob.release(dU, dS, dV);
   if (fell_back) *fell_back = 1;
   if (resid_rel) *resid_rel = sqrt(a2 > s2 ? a2 - s2 : 0.0) / sqrt(a2);
   return 0;
 }
 
 int rsvd_dev_copy(const void* src, void* dst, long long bytes) {
-  Api api;
+  DevApi api;
   if (api.rc) return api.rc;
   if (bytes < 0) return set_error("rsvd_dev_copy: negative size");
   RSVD_TRY(stream_copy(api.c, src, dst, bytes));
@@ -1033,26 +1039,28 @@ int rsvd_dev_copy(const void* src, void* dst, long long bytes) {
 int rsvd_dev_lowrank_test_matrix(long long m, long long n, int r, const double* sigma,
                                  double noise_std, unsigned long long seed, double* dA64,
                                  long long lda64, float* dA32, long long lda32) {
-  Api api;
+  DevApi api;
   if (api.rc) return api.rc;
-  return lowrank_test_matrix(api.c, m, n, r, sigma, noise_std, seed, dA64, lda64, dA32, lda32);
+  RSVD_TRY(lowrank_test_matrix(api.c, m, n, r, sigma, noise_std, seed, dA64, lda64, dA32, lda32));
+  return stream_sync(api.c);
 }
 
 int rsvd_dev_fixed_rank_mp(const float* dA, long long m, long long n, long long lda, int k, int p,
                            int vnum, int q, int s, unsigned long long seed, const float* dOmega,
                            int precision, float* dU, long long ldu, double* dS, float* dV,
                            long long ldv) {
-  Api api;
+  DevApi api;
   if (api.rc) return api.rc;
-  return fixed_rank_mp(api.c, dA, m, n, lda, k, p, vnum, q, s, seed, dOmega, precision, dU, ldu,
-                       dS, dV, ldv, nullptr);
+  RSVD_TRY(fixed_rank_mp(api.c, dA, m, n, lda, k, p, vnum, q, s, seed, dOmega, precision, dU,
+                         ldu, dS, dV, ldv, nullptr));
+  return stream_sync(api.c);
 }
 
 int rsvd_dev_randqb_mp(const float* dA, long long m, long long n, long long lda, int kstep,
                        int nstep, double TOL, int q, int s, int is_new, unsigned long long seed,
                        const float* dRN, int precision, float* dQ, long long ldq, float* dBt,
                        long long ldbt, int* frank, double* final_norm) {
-  Api api;
+  DevApi api;
   if (api.rc) return api.rc;
   int fr = 0, la = 0;
   QbInfo qi{};
@@ -1067,43 +1075,43 @@ int rsvd_dev_blockrand_mp(const float* dA, long long m, long long n, long long l
                           double TOL, int vnum, int kstep, int q, int s, unsigned long long seed,
                           const float* dRN, int precision, int* frank, float** dU, double** dS,
                           float** dV) {
-  Api api;
+  DevApi api;
   if (api.rc) return api.rc;
   if (!frank || !dU || !dS || !dV) return set_error("rsvd_dev_blockrand_mp: null output pointer");
+  *dU = nullptr;
+  *dS = nullptr;
+  *dV = nullptr;
   BlockrandMpOut out;
   RSVD_TRY(blockrand_mp(api.c, dA, m, n, lda, k, p, TOL, vnum, kstep, q, s, seed, dRN, precision,
                         &out));
   const int fr = out.frank;
-  float *u = nullptr, *v = nullptr;
-  double* sv = nullptr;
-  RSVD_HIP(hipMalloc(&u, sizeof(float) * (size_t)m * fr));
-  RSVD_HIP(hipMalloc(&sv, sizeof(double) * (size_t)fr));
-  RSVD_HIP(hipMalloc(&v, sizeof(float) * (size_t)n * fr));
-  RSVD_HIP(hipMemcpyAsync(u, out.U.p, sizeof(float) * (size_t)m * fr, hipMemcpyDeviceToDevice,
+  OutBufs ob;
+  RSVD_TRY(ob.alloc(sizeof(float) * (size_t)m * fr, sizeof(double) * (size_t)fr,
+                    sizeof(float) * (size_t)n * fr));
+  RSVD_HIP(hipMemcpyAsync(ob.p[0], out.U.p, sizeof(float) * (size_t)m * fr,
+                          hipMemcpyDeviceToDevice, api.c->stream));
+  RSVD_HIP(hipMemcpyAsync(ob.p[1], out.S.p, sizeof(double) * (size_t)fr, hipMemcpyDeviceToDevice,
                           api.c->stream));
-  RSVD_HIP(hipMemcpyAsync(sv, out.S.p, sizeof(double) * (size_t)fr, hipMemcpyDeviceToDevice,
-                          api.c->stream));
-  RSVD_HIP(hipMemcpyAsync(v, out.V.p, sizeof(float) * (size_t)n * fr, hipMemcpyDeviceToDevice,
-                          api.c->stream));
+  RSVD_HIP(hipMemcpyAsync(ob.p[2], out.V.p, sizeof(float) * (size_t)n * fr,
+                          hipMemcpyDeviceToDevice, api.c->stream));
   RSVD_TRY(stream_sync(api.c));
   *frank = fr;
-  *dU = u;
-  *dS = sv;
-  *dV = v;
+  ob.release(dU, dS, dV);
   return 0;
 }
 
 int rsvd_dev_lowrank_residual(const double* dA, long long m, long long n, long long lda,
                               long long k, const void* dU, long long ldu, const double* dS,
                               const void* dV, long long ldv, int uv_f32, double* out) {
-  Api api;
+  DevApi api;
   if (api.rc) return api.rc;
-  return lowrank_residual(api.c, dA, m, n, lda, k, dU, ldu, dS, dV, ldv, uv_f32, out);
+  RSVD_TRY(lowrank_residual(api.c, dA, m, n, lda, k, dU, ldu, dS, dV, ldv, uv_f32, out));
+  return stream_sync(api.c);
 }
 
 int rsvd_dev_sym_eig_psd(long long n, const double* dG, long long ldg, double* dEvals,
                          double* dEvecs, long long ldv, int* sweeps) {
-  Api api;
+  DevApi api;
   if (api.rc) return api.rc;
   Context* c = api.c;
   DBuf<double> W;

@@ -11,12 +11,16 @@ namespace rsvd {
 constexpr double kRotTol = 1e-15;
 // a sweep that met no off-diagonal above this relative size ends the iteration
 constexpr double kStopTol = 1e-10;
-// the same for callers that need eigenvectors to fp32 level only: Jacobi
-// converges quadratically, so a sweep whose rotations were all below 1e-4
-// leaves off-diagonals near 1e-8 of sqrt(g_pp g_qq); the eigenvalues are then
-// exact to ~1e-16 relative (their error is second order in the off-diagonals)
-// and the eigenvectors, and the orthogonality of B^T V Sigma^-1, to ~1e-8,
-// below the fp32 resolution of the outputs
+// the same for callers that need eigenvectors to fp32 level only, on the
+// rotation sines |s| rather than the off-diagonal ratio: near convergence a
+// pair with relative eigenvalue gap d and ratio r turns by |s| ~ r / d, and
+// the next sweep leaves ratios ~ r^2 / d = s^2 d.  A sweep whose rotations
+// all had |s| <= 1e-4 therefore leaves off-diagonals below ~1e-8 of
+// sqrt(g_pp g_qq) for every gap, clusters of nearly equal eigenvalues included
+// (there the ratio test alone stops while rotations inside the cluster are
+// still large and refill each other); the eigenvalues are then exact to
+// ~1e-16 relative (second order) and the eigenvectors, and the orthogonality
+// of B^T V Sigma^-1, to ~1e-8, below the fp32 resolution of the outputs
 constexpr double kLooseStopTol = 1e-4;
 
 // 1/x and 1/sqrt(x) from the hardware estimates plus two Newton steps
@@ -46,8 +50,8 @@ static __device__ __forceinline__ double rsq_nr(double x) {
 // serial path and c^2 + s^2 = (den^2 + 4g^2) / (2 r den) = 1 identically.
 // Evaluated on the entries scaled by a power of two so that the squares stay
 // in range.  Branch-free.  No rotation when |g| <= flo or |g| <= kRotTol
-// sqrt|ab|; *sig bit 0 reports |g| > kStopTol sqrt|ab|, bit 1 |g| >
-// kLooseStopTol sqrt|ab| (so *sig != 0 exactly when bit 0 is set).
+// sqrt|ab|; *sig bit 0 reports |g| > kStopTol sqrt|ab|, bit 1 a rotation
+// with |s| > kLooseStopTol (|s| <= |g| / sqrt|ab|, so bit 1 implies bit 0).
 static __device__ __forceinline__ void rotation(double a, double b, double g, double flo, double* cs,
                                          double* sn, int* sig) {
   const double ag = fabs(g);
@@ -56,8 +60,6 @@ static __device__ __forceinline__ void rotation(double a, double b, double g, do
   const double sa = ldexp(a, -e), sb = ldexp(b, -e), sg = ldexp(g, -e);
   const double ab = fabs(sa * sb), g2 = sg * sg;
   const bool big = ag > flo;
-  *sig = ((big && g2 > kStopTol * kStopTol * ab) ? 1 : 0) |
-         ((big && g2 > kLooseStopTol * kLooseStopTol * ab) ? 2 : 0);
   const bool rot = big && g2 > kRotTol * kRotTol * ab;
   const double d = sb - sa;
   const double r2 = rot ? fma(d, d, 4.0 * g2) : 1.0;
@@ -67,6 +69,8 @@ static __device__ __forceinline__ void rotation(double a, double b, double g, do
   const double t2g = (d < 0.0) ? -2.0 * sg : 2.0 * sg;
   *cs = rot ? den * q : 1.0;
   *sn = rot ? t2g * q : 0.0;
+  *sig = ((big && g2 > kStopTol * kStopTol * ab) ? 1 : 0) |
+         ((rot && fabs(*sn) > kLooseStopTol) ? 2 : 0);
 }
 
 }  // namespace rsvd

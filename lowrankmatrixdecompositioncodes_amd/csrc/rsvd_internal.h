@@ -62,10 +62,16 @@ struct Context {
   // pinned host slot for the small device -> host reads of the control flow
   // (convergence and breakdown flags, norms); allocated on first use, kept
   void* pinned = nullptr;
+  // recorded on the null stream at the entry of device-pointer routines
+  hipEvent_t entry_ev = nullptr;
 };
+
 
 // Lazily initialises the current device and the library stream.
 int ctx_get(Context** out);
+
+// Makes the library stream wait for the work queued on the null stream so far.
+int order_after_null_stream(Context* c);
 
 // dst <- bytes (<= 64) at device address src, after the work queued on the
 // library stream: one direct copy into pinned host memory, then a wait.

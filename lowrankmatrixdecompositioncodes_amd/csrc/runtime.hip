@@ -42,6 +42,8 @@ int ctx_get(Context** out) {
       return set_error("librsvdpack is built for gfx950 (MI355X); device %d is %s", dev,
                        prop.gcnArchName);
     RSVD_HIP(hipStreamCreateWithFlags(&g_ctx.stream, hipStreamNonBlocking));
+    RSVD_HIP(hipEventCreateWithFlags(&g_ctx.entry_ev, hipEventDisableTiming));
+    RSVD_HIP(hipHostMalloc(&g_ctx.pinned, 64, hipHostMallocDefault));
     g_ctx.num_cus = prop.multiProcessorCount;
     g_ctx.device = dev;
   }
@@ -50,9 +52,14 @@ int ctx_get(Context** out) {
   return 0;
 }
 
+int order_after_null_stream(Context* c) {
+  RSVD_HIP(hipEventRecord(c->entry_ev, nullptr));
+  RSVD_HIP(hipStreamWaitEvent(c->stream, c->entry_ev, 0));
+  return 0;
+}
+
 int read_small(Context* c, void* dst, const void* src, size_t bytes) {
   if (bytes > 64) return set_error("read_small: %zu bytes", bytes);
-  if (!c->pinned) RSVD_HIP(hipHostMalloc(&c->pinned, 64, hipHostMallocDefault));
   RSVD_HIP(hipMemcpyAsync(c->pinned, src, bytes, hipMemcpyDeviceToHost, c->stream));
   RSVD_HIP(hipStreamSynchronize(c->stream));
   memcpy(dst, c->pinned, bytes);

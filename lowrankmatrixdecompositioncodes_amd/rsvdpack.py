@@ -112,6 +112,11 @@ _SIGS = {
         C.c_int, [C.c_void_p, C.c_longlong, C.c_longlong, C.c_longlong, C.c_int, C.c_int,
                   C.c_int, C.c_int, C.c_int, C.c_ulonglong, C.c_void_p, C.c_void_p,
                   C.c_longlong, C.c_void_p, C.c_void_p, C.c_longlong]),
+    "rsvd_binary_file_shape": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_longlong),
+                                         C.POINTER(C.c_longlong)]),
+    "rsvd_dev_load_binary": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_void_p, C.c_longlong]),
+    "rsvd_dev_write_binary": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_void_p, C.c_longlong,
+                                        C.c_longlong, C.c_longlong]),
     "rsvd_qb_max_cols": (C.c_longlong, [C.c_longlong, C.c_longlong, C.c_int, C.c_int, C.c_int]),
     "rsvd_dev_randqb": (
         C.c_int, [C.c_void_p, C.c_longlong, C.c_longlong, C.c_longlong, C.c_int, C.c_int,

@@ -157,6 +157,44 @@ def test_fixed_rank_mp_against_fp64(gpu_lib, prec, bound):
     assert abs(res.value - res64.value) <= max(bound, 1e-5) * res64.value
 
 
+@pytest.mark.gpu
+def test_fixed_rank_mp_near_degenerate_cluster(gpu_lib):
+    """A cluster of 24 nearly equal singular values (sigma = 1 + 1e-6 j, relative
+    gaps 1e-6) above a decaying tail, 4000 x 1500, k = 64, p = 16, q = 2.  The
+    eigensolver's early stop for the mixed path must not leave rotations
+    inside the cluster unfinished: the returned sigma agree with the FP64 path
+    to 1e-6 and the cluster's columns of V (V = B^T Uhat Sigma^-1, sensitive to
+    an unconverged Uhat) and U are orthonormal to fp32 level."""
+    rp = gpu_lib
+    lib = rp.load()
+    m, n, k, p, q, seed = 4000, 1500, 64, 16, 2, 99
+    sigma = np.concatenate([1.0 + 1e-6 * np.arange(24)[::-1],
+                            0.5 * np.exp(-np.arange(200) / 20.0)])
+    A64 = rp.DeviceMatrix(m, n, np.float64)
+    A32 = rp.DeviceMatrix(m, n, np.float32)
+    assert lib.rsvd_dev_lowrank_test_matrix(m, n, len(sigma), sigma.ctypes.data, 0.0, 5, A64.ptr,
+                                            m, A32.ptr, m) == 0, rp.last_error()
+    om = rp.gaussian(seed, 0, n * (k + p)).astype(np.float32).astype(np.float64)
+    dOm = rp.DeviceMatrix.from_host(om.reshape(k + p, n).T)
+    U64, S64, V64 = rp.DeviceMatrix(m, k), rp.DeviceMatrix(k, 1), rp.DeviceMatrix(n, k)
+    assert lib.rsvd_dev_fixed_rank(A64.ptr, m, n, m, k, p, 2, q, 1, seed, dOm.ptr, U64.ptr, m,
+                                   S64.ptr, V64.ptr, n) == 0, rp.last_error()
+    Um, Sm, Vm = (rp.DeviceMatrix(m, k, np.float32), rp.DeviceMatrix(k, 1),
+                  rp.DeviceMatrix(n, k, np.float32))
+    assert lib.rsvd_dev_fixed_rank_mp(A32.ptr, m, n, m, k, p, 2, q, 1, seed, None, 2, Um.ptr, m,
+                                      Sm.ptr, Vm.ptr, n) == 0, rp.last_error()
+    s64, sm = S64.to_host()[:, 0], Sm.to_host()[:, 0]
+    assert np.abs(sm - s64).max() <= 1e-6 * s64.max()
+    # vnum = 2 returns ascending values: the cluster is the last 24 columns
+    V = Vm.to_host().astype(np.float64)[:, -24:]
+    U = Um.to_host().astype(np.float64)[:, -24:]
+    ov = np.abs(V.T @ V - np.eye(24)).max()
+    ou = np.abs(U.T @ U - np.eye(24)).max()
+    print(f"cluster: sigma diff {np.abs(sm - s64).max():.2e}, V^T V - I {ov:.2e}, "
+          f"U^T U - I {ou:.2e}")
+    assert ov <= 2e-6 and ou <= 2e-6, (ov, ou)
+
+
 def _blockrand_case(rp, m, n, seed):
     """A = U diag(sigma) V^T with a 50-value plateau at 1, then exp(-(j - 50)/40)
     down to 1e-8 (float32 and its exact double widening)."""
