@@ -84,6 +84,24 @@ int rsvd_dev_fixed_rank(const double *dA, long long m, long long n, long long ld
                         int vnum, int q, int s, unsigned long long seed, const double *dOmega,
                         double *dU, long long ldu, double *dS, double *dV, long long ldv);
 
+/* rsvd_dev_fixed_rank without any device allocation: no counterpart in the
+ * reference; for callers that cannot allocate device memory during a call
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, arithmetic and results as rsvd_dev_fixed_rank;
+ * every buffer the computation needs beyond the caller's inputs and outputs
+ * comes from one static device array of 256 MiB that the library holds on the
+ * device from its load until the process ends.  Problems whose buffers exceed
+ * it (roughly 8 (k + p) (3 n + 2 m) bytes plus O((k + p)^2) for the small
+ * solves) are rejected with an error.  Every call on a device shares that
+ * device's array and finds the previous call's contents in it (nothing there
+ * is assumed zero); two calls that overlapped on one device would race.
+ * Within a process they cannot: calls are serialised by the library's lock,
+ * run on its one stream and return only when finished. */
+int rsvd_dev_fixed_rank_noalloc(const double *dA, long long m, long long n, long long lda, int k,
+                                int p, int vnum, int q, int s, unsigned long long seed,
+                                const double *dOmega, double *dU, long long ldu, double *dS,
+                                double *dV, long long ldv);
+
 /* Mixed-precision fixed-rank rSVD of a float32 device matrix A (m x n, lda).
  * precision 1: sketch and power-iteration passes on the bf16 matrix cores
  * (fp32 accumulation); 2: the same passes in split-bf16 (16-bit operands, about

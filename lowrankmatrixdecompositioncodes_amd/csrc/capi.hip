@@ -746,6 +746,31 @@ int rsvd_dev_fixed_rank(const double* dA, long long m, long long n, long long ld
   return stream_sync(api.c);
 }
 
+int rsvd_dev_fixed_rank_noalloc(const double* dA, long long m, long long n, long long lda, int k,
+                                int p, int vnum, int q, int s, unsigned long long seed,
+                                const double* dOmega, double* dU, long long ldu, double* dS,
+                                double* dV, long long ldv) {
+  DevApi api;
+  if (api.rc) return api.rc;
+  // The panels the path holds at once (Omega, Y, Z, the orthonormalisation's
+  // copy, B, in doubles) must fit the scratch array; the remaining buffers
+  // are checked by the scratch allocator before the kernel that needs them.
+  const double l = (double)k + (double)p;
+  const double panels = 8.0 * l * ((dOmega ? 0.0 : (double)n) + 2.0 * (double)m + 2.0 * (double)n);
+  if (k > 0 && p >= 0 && panels > (double)kScratchBytes)
+    return set_error("rsvd_dev_fixed_rank_noalloc: %lld x %lld with k + p = %.0f needs at least "
+                     "%.0f MiB of device buffers, more than its %zu MiB scratch; use "
+                     "rsvd_dev_fixed_rank", m, n, l, panels / 1048576.0, kScratchBytes >> 20);
+  OmegaSrc om;
+  om.seed = seed;
+  om.dev = dOmega;
+  om.ld = n;
+  ScratchScope scope(api.c);
+  RSVD_TRY(fixed_rank_f64(api.c, dA, m, n, lda, k, p, vnum, q, s, om, dU, ldu, dS, dV, ldv,
+                          nullptr));
+  return stream_sync(api.c);
+}
+
 long long rsvd_qb_max_cols(long long m, long long n, int kstep, int nstep, int is_new) {
   int ks = kstep, ns = nstep, tm = 0;
   if (qb_adjust(m, n, &ks, &ns, is_new, &tm)) return -1;

@@ -64,8 +64,32 @@ struct Context {
   void* pinned = nullptr;
   // recorded on the null stream at the entry of device-pointer routines
   hipEvent_t entry_ev = nullptr;
+  // Allocation-free mode: while scratch_mode is set, dmalloc hands out
+  // first-fit ranges of the library's static device scratch array (device
+  // address `scratch`, kScratchBytes long) instead of pooled hipMalloc
+  // memory, and dfree returns them.  Live ranges: (offset, bytes), sorted.
+  char* scratch = nullptr;
+  bool scratch_mode = false;
+  std::vector<std::pair<size_t, size_t>> scratch_live;
 };
 
+// Size of the static device scratch array behind the allocation-free entry
+// point (rsvd_dev_fixed_rank_noalloc).
+constexpr size_t kScratchBytes = size_t(256) << 20;
+
+// Scope of an allocation-free call: every device buffer the library takes
+// inside it comes from the scratch array; none is left live at its end.
+struct ScratchScope {
+  Context* c;
+  explicit ScratchScope(Context* ctx) : c(ctx) {
+    c->scratch_live.clear();
+    c->scratch_mode = true;
+  }
+  ~ScratchScope() {
+    c->scratch_mode = false;
+    c->scratch_live.clear();
+  }
+};
 
 // Lazily initialises the current device and the library stream.
 int ctx_get(Context** out);

@@ -26,6 +26,13 @@ const char* last_error() { return g_err; }
 static Context g_ctx;
 static std::mutex g_init_mu;
 
+// Device memory of the allocation-free entry point: a file-scope array the
+// code object carries, so it exists on the device from the library's load
+// until the process ends.  Host code reaches it only through the address
+// hipGetSymbolAddress returns (kept in Context::scratch), never through the
+// symbol itself.
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
 int ctx_get(Context** out) {
   std::lock_guard<std::mutex> lk(g_init_mu);
   if (g_ctx.device < 0) {
@@ -44,6 +51,10 @@ int ctx_get(Context** out) {
     RSVD_HIP(hipStreamCreateWithFlags(&g_ctx.stream, hipStreamNonBlocking));
     RSVD_HIP(hipEventCreateWithFlags(&g_ctx.entry_ev, hipEventDisableTiming));
     RSVD_HIP(hipHostMalloc(&g_ctx.pinned, 64, hipHostMallocDefault));
+    void* sp = nullptr;
+    RSVD_HIP(hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch)));
+    if (!sp) return set_error("hipGetSymbolAddress(scratch) returned NULL");
+    g_ctx.scratch = static_cast<char*>(sp);
     g_ctx.num_cus = prop.multiProcessorCount;
     g_ctx.device = dev;
   }
@@ -85,8 +96,40 @@ static size_t round_size(size_t b) {
 // that can be handed out again.
 static std::unordered_map<void*, size_t> g_live;
 
+// First fit in the scratch array, 256-byte aligned; no device call.
+static int scratch_alloc(Context* c, void** p, size_t bytes) {
+  const size_t want = (bytes + 255) / 256 * 256;
+  size_t at = 0;
+  size_t pos = 0;
+  for (; pos < c->scratch_live.size(); ++pos) {
+    const auto& r = c->scratch_live[pos];
+    if (r.first >= at + want) break;
+    at = r.first + r.second;
+  }
+  if (at + want > kScratchBytes)
+    return set_error("allocation-free call: its device buffers need more than the %zu MiB "
+                     "scratch (%zu bytes requested with %zu in use)",
+                     kScratchBytes >> 20, bytes, at);
+  c->scratch_live.insert(c->scratch_live.begin() + pos, {at, want});
+  *p = c->scratch + at;
+  return 0;
+}
+
+static bool scratch_free(Context* c, void* p) {
+  if (!c->scratch || p < (void*)c->scratch || p >= (void*)(c->scratch + kScratchBytes))
+    return false;
+  const size_t off = (size_t)((char*)p - c->scratch);
+  for (size_t i = 0; i < c->scratch_live.size(); ++i)
+    if (c->scratch_live[i].first == off) {
+      c->scratch_live.erase(c->scratch_live.begin() + i);
+      break;
+    }
+  return true;
+}
+
 int dmalloc(void** p, size_t bytes) {
   Context* c = &g_ctx;
+  if (c->scratch_mode) return scratch_alloc(c, p, bytes);
   size_t want = round_size(bytes);
   int best = -1;
   for (int i = 0; i < (int)c->free_blocks.size(); ++i) {
@@ -117,6 +160,7 @@ int dmalloc(void** p, size_t bytes) {
 void dfree(void* p) {
   if (!p) return;
   Context* c = &g_ctx;
+  if (scratch_free(c, p)) return;
   auto it = g_live.find(p);
   if (it == g_live.end()) return;
   c->free_blocks.push_back({p, it->second});

@@ -112,6 +112,10 @@ _SIGS = {
         C.c_int, [C.c_void_p, C.c_longlong, C.c_longlong, C.c_longlong, C.c_int, C.c_int,
                   C.c_int, C.c_int, C.c_int, C.c_ulonglong, C.c_void_p, C.c_void_p,
                   C.c_longlong, C.c_void_p, C.c_void_p, C.c_longlong]),
+    "rsvd_dev_fixed_rank_noalloc": (
+        C.c_int, [C.c_void_p, C.c_longlong, C.c_longlong, C.c_longlong, C.c_int, C.c_int,
+                  C.c_int, C.c_int, C.c_int, C.c_ulonglong, C.c_void_p, C.c_void_p,
+                  C.c_longlong, C.c_void_p, C.c_void_p, C.c_longlong]),
     "rsvd_binary_file_shape": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_longlong),
                                          C.POINTER(C.c_longlong)]),
     "rsvd_dev_load_binary": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_void_p, C.c_longlong]),
