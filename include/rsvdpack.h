@@ -160,6 +160,40 @@ void randQB_pb_new(mat *M, int kstep, int nstep, double TOL, int q, int s, int *
 void use_low_rank_svd_for_approximation(mat *M, mat *U, mat *S, mat *V);
 void use_QB_decomp_for_approximation(mat *M, mat *Q, mat *B);
 
+/* ---------------- L2: interpolative and CUR decompositions ---------------- */
+/* The index vectors hold 0-based column (row) indices as doubles, as the
+ * reference's vec does. */
+
+/* MKL/rra.h:41, rra.c:924-975 -- column-pivoted QR, M(:, I) = Q R with every
+ * column free to pivot (LAPACK dgeqp3 rules); k = min(m, n): Q m x k, R k x n
+ * for m <= n (else k x k), I the n pivots. */
+void pivotedQR_mkl(mat *M, mat **Q, mat **R, vec **I);
+/* MKL/rra.h:71, rra.c:1969-2027 -- column ID from the blocked QB:
+ * randQB_pb_new with nstep = (k + p) / kstep (TOL mode when k <= 0), the
+ * column-pivoted QR of B, rank k or (TOL mode) round(frank / (frank + p +
+ * 1e-6) * frank); I: the n pivots, T = Rk1^-1 Rk2 (frank x (n - frank)), so
+ * M(:, I) ~ M(:, I(1:frank)) [I T]. */
+void id_blockrand_decomp_fixed_rank_or_prec(mat *M, int k, int p, double TOL, int kstep, int q,
+    int s, int *frank, vec **I, mat **T);
+/* MKL/rra.h:81, rra.c:2086-2112 -- the column ID, then the full ID of
+ * M(:, Icol(1:frank))^T: Irow (m row indices), S (frank x (m - frank)). */
+void id_two_sided_blockrand_decomp_fixed_rank_or_prec(mat *M, int k, int p, double TOL,
+    int kstep, int q, int s, int *frank, vec **Icol, vec **Irow, mat **T, mat **S);
+/* MKL/rra.h:91, rra.c:2262-2332 -- C = M(:, Icol(1:frank)), R = M(Irow(1:frank), :),
+ * U (frank x frank) solving (R R^T) U^T = R V for the column ID's
+ * interpolation matrix V; M ~ C U R.  The reference solves with dgesv; here
+ * R R^T (symmetric positive definite) is factored by Cholesky. */
+void cur_blockrand_decomp_fixed_rank_or_prec(mat *M, int k, int p, double TOL, int kstep, int q,
+    int s, int *frank, mat **C, mat **U, mat **R);
+/* MKL/mvf.h:351, mvf.c:1322-1336 -- P = C U R. */
+void form_cur_product_matrix(mat *C, mat *U, mat *R, mat *P);
+/* MKL/rra.h:104,107,110; rra.c:2402-2573 -- print norm(M), norm of the
+ * approximation and the percent error, as use_QB_decomp_for_approximation. */
+void use_id_decomp_for_approximation(mat *M, mat *T, vec *I, int k);
+void use_id_two_sided_decomp_for_approximation(mat *M, mat *T, mat *S, vec *Icol, vec *Irow,
+    int k);
+void use_cur_decomp_for_approximation(mat *M, mat *C, mat *U, mat *R);
+
 #ifdef __cplusplus
 }
 #endif

@@ -108,9 +108,13 @@ int rsvd_dev_fixed_rank_noalloc(const double *dA, long long m, long long n, long
  * 1e-7 relative error on long sums); 3:
  * the same passes in fp8 e4m3 (A scaled per 128 columns, the skinny operand
  * per column; fp32 accumulation); 4 / 5: every pass but the last in bf16 /
- * fp8, the last power-iteration pass (whose range becomes Q) in split-bf16.
- * Always: CholeskyQR with FP64 Cholesky, split-bf16 projection B^T = A^T Q,
- * FP64 BB^T and eigen/SVD solve.  Omega: dOmega (n x (k+p) float, ld n) or the
+ * fp8, the last power-iteration pass (whose range becomes Q) in split-bf16;
+ * 6: every pass over A, the projection B^T = A^T Q included, on the fp16
+ * matrix cores with A divided by a per-128-column scale and rounded to fp16
+ * and the skinny operand in two fp16 planes (hi + lo, per-column scales): two
+ * products per product, fp32 accumulation (about 2e-6 relative error of
+ * sigma at 65536 x 16384, k = 512).  Otherwise: CholeskyQR with FP64
+ * Cholesky, split-bf16 projection B^T = A^T Q, FP64 BB^T and eigen/SVD solve.  Omega: dOmega (n x (k+p) float, ld n) or the
  * seeded stream rounded to float.  Outputs: dU m x k float (ldu), dS k doubles
  * (reference order), dV n x k float (ldv). */
 int rsvd_dev_fixed_rank_mp(const float *dA, long long m, long long n, long long lda, int k, int p,
@@ -180,7 +184,10 @@ int rsvd_dev_blockrand(const double *dA, long long m, long long n, long long lda
  * e4m3 with one scale per 128 columns of A and B in e4m3 with one scale per
  * column, fp8 matrix cores, fp32 accumulation (tb = 'N' only); 5: float
  * storage, fp32-exact products from three bf16 planes per operand (six bf16
- * products), fp32 accumulation. */
+ * products), fp32 accumulation; 6: float storage, op(A) / a_c in fp16 (a_c =
+ * max |A| over each 128-column chunk of A / 32768) times B in two fp16 planes
+ * (hi + lo of B a_c / b_n, b_n = the column's max / 32768), two fp16 products,
+ * fp32 accumulation (tb = 'N' only). */
 int rsvd_dev_gemm(int dtype, char ta, char tb, long long M, long long N, long long K, double alpha,
                   const void *A, long long lda, const void *B, long long ldb, double beta, void *C,
                   long long ldc);
@@ -250,10 +257,12 @@ int rsvd_dev_sketch_features(const float *dA, long long m, long long n, long lon
  * Omega = I (an exact thin QR of A and the small eigenproblem), its residual
  * formed explicitly.  A result above TOL falls back to
  * low_rank_svd_blockrand_decomp_fixed_rank_or_prec in TOL mode (TOL = tol_rel
- * ||A||_F, kstep, q_fallback) and returns that method's result, which may itself
- * miss TOL when TOL is not reachable below full rank.  Outputs as
- * rsvd_dev_blockrand; *q_pred, *k_pred the prediction, *fell_back 1 when the
- * fallback ran, *resid_rel the relative residual of the accepted result. */
+ * ||A||_F, kstep, q_fallback) and returns that method's result; when that too
+ * misses TOL (TOL not reachable below full rank) and m >= n, the exact
+ * full-rank decomposition (k = n, Omega = I) is returned instead, so no result
+ * exceeds TOL.  Outputs as rsvd_dev_blockrand; *q_pred, *k_pred the
+ * prediction, *fell_back 1 when the adaptive fallback's result is returned, 2
+ * when the exact one is, *resid_rel the relative residual of the result. */
 int rsvd_dev_learned_lowrank(const float *dTheta, const double *dA, const float *dA32,
                              long long m, long long n, long long lda, double tol_rel, int p,
                              double margin, int kstep, int q_fallback, unsigned long long seed,

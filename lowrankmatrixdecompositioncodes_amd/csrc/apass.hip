@@ -35,6 +35,7 @@ namespace rsvd {
 namespace {
 
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8_t __attribute__((ext_vector_type(8)));
 typedef float f4v_t __attribute__((ext_vector_type(4)));
 typedef float f4s_t __attribute__((ext_vector_type(4)));
 
@@ -263,13 +264,31 @@ constexpr int kNTW2 = 9;
 constexpr int kBNT2 = 2 * kNTW2;
 constexpr int kST2 = 3;
 
+// v_mfma_f32_16x16x32 on 8 16-bit values per lane: bf16 (SPLIT 1, 3) or fp16
+// (SPLIT 2), the fp16 fragments carried in the bf16 vector type bit for bit.
+template <int SPLIT>
+__device__ __forceinline__ f4v_t mfma16(bf16x8_t a, bf16x8_t b, f4v_t c) {
+  if constexpr (SPLIT == 2)
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(f16x8_t, a),
+                                                  __builtin_bit_cast(f16x8_t, b), c, 0, 0, 0);
+  else
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// SPLIT 1: bf16 operands, one product.  SPLIT 3: split-bf16, hi hi + hi lo +
+// lo hi.  SPLIT 2: fp16, A scaled by 1 / a_c per 128-column chunk of A
+// (ascale) and rounded to fp16, the skinny operand in two fp16 planes (hi +
+// lo, scaled per column by 1 / b_n, with a_c folded in for op(A) = A), two
+// products hi hi + lo hi; the epilogue multiplies by b_n (bscale) and, for
+// op(A) = A^T, by the output row's a_c.
 template <bool TA, int SPLIT, bool VEC, int MT>
 __global__ void __launch_bounds__(kT2)
 apass2_kernel(int64_t M, int64_t N, int64_t KBtot, int64_t KBper, float alpha,
               const float* __restrict__ A, int64_t lda, const bf16x8_t* __restrict__ St,
               int64_t NTP, float beta, float* __restrict__ C, int64_t ldc,
-              float* __restrict__ Wk, int gx, int gy, int ks) {
-  constexpr int NP = (SPLIT == 3) ? 2 : 1;
+              float* __restrict__ Wk, int gx, int gy, int ks, const float* __restrict__ ascale,
+              const float* __restrict__ bscale) {
+  constexpr int NP = (SPLIT >= 2) ? 2 : 1;
   constexpr int NCH = NP * kBNT2;             // 1 KiB chunks per stage
   constexpr int NLD = (NCH + kW2 - 1) / kW2;  // chunk loads per wave per stage
   __shared__ bf16x8_t Ss[kST2][NP][kBNT2][64];
@@ -355,8 +374,10 @@ apass2_kernel(int64_t M, int64_t N, int64_t KBtot, int64_t KBper, float alpha,
     Ar[i] = TA ? A + r * lda + 8 * (lane >> 4) : A + r + (int64_t)(8 * (lane >> 4)) * lda;
   }
   float raw[MT][8];
+  int64_t kb_raw = kb0;  // k block held in raw
   auto loadA = [&](int64_t kb) {
     kb = kb < klast ? kb : klast;
+    kb_raw = kb;
 #pragma unroll
     for (int i = 0; i < MT; ++i) {
       if (TA) {
@@ -381,15 +402,28 @@ apass2_kernel(int64_t M, int64_t N, int64_t KBtot, int64_t KBper, float alpha,
     }
   };
   bf16x8_t ah[MT], al[MT];
+  // fp16: 1 / a_c of the chunk of A's columns the values in raw come from --
+  // the k block's chunk (op(A) = A) or the workgroup's 128 rows (op(A) = A^T)
+  static_assert(SPLIT != 2 || !TA || 64 * MT == 128, "fp16: one chunk of A per row block");
+  const float inv_row = (SPLIT == 2 && TA) ? 1.0f / ascale[bx] : 1.0f;
   auto split = [&]() {
+    if constexpr (SPLIT == 2) {
+      const float sc = TA ? inv_row : 1.0f / ascale[kb_raw >> 2];
 #pragma unroll
-    for (int i = 0; i < MT; ++i)
+      for (int i = 0; i < MT; ++i)
 #pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        const __bf16 hb = (__bf16)raw[i][e];
-        ah[i][e] = hb;
-        if (SPLIT == 3) al[i][e] = (__bf16)(raw[i][e] - (float)hb);
-      }
+        for (int e = 0; e < 8; ++e)
+          ah[i][e] = __builtin_bit_cast(__bf16, (_Float16)(raw[i][e] * sc));
+    } else {
+#pragma unroll
+      for (int i = 0; i < MT; ++i)
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          const __bf16 hb = (__bf16)raw[i][e];
+          ah[i][e] = hb;
+          if (SPLIT == 3) al[i][e] = (__bf16)(raw[i][e] - (float)hb);
+        }
+    }
   };
 
   f4v_t acc[kNTW2][MT];
@@ -412,7 +446,7 @@ apass2_kernel(int64_t M, int64_t N, int64_t KBtot, int64_t KBper, float alpha,
 #pragma unroll
       for (int q = 0; q < 2; ++q) {
         sh[0][q] = cur[0][wn * kNTW2 + q][lane];
-        if (SPLIT == 3) sl[0][q] = cur[NP - 1][wn * kNTW2 + q][lane];
+        if (SPLIT >= 2) sl[0][q] = cur[NP - 1][wn * kNTW2 + q][lane];
       }
 #pragma unroll
       for (int jp = 0; jp < NPR; ++jp) {
@@ -423,35 +457,37 @@ apass2_kernel(int64_t M, int64_t N, int64_t KBtot, int64_t KBper, float alpha,
             const int jn = 2 * (jp + 1) + q;
             if (jn < kNTW2) {
               sh[b ^ 1][q] = cur[0][wn * kNTW2 + jn][lane];
-              if (SPLIT == 3) sl[b ^ 1][q] = cur[NP - 1][wn * kNTW2 + jn][lane];
+              if (SPLIT >= 2) sl[b ^ 1][q] = cur[NP - 1][wn * kNTW2 + jn][lane];
             }
           }
         }
         const int j0 = 2 * jp, j1 = 2 * jp + 1;
 #pragma unroll
         for (int i = 0; i < MT; ++i)
-          acc[j0][i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(sh[b][0], ah[i], acc[j0][i], 0, 0, 0);
+          acc[j0][i] = mfma16<SPLIT>(sh[b][0], ah[i], acc[j0][i]);
         if (j1 < kNTW2) {
 #pragma unroll
           for (int i = 0; i < MT; ++i)
-            acc[j1][i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(sh[b][1], ah[i], acc[j1][i], 0, 0, 0);
+            acc[j1][i] = mfma16<SPLIT>(sh[b][1], ah[i], acc[j1][i]);
         }
         if (SPLIT == 3) {
 #pragma unroll
           for (int i = 0; i < MT; ++i)
-            acc[j0][i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(sh[b][0], al[i], acc[j0][i], 0, 0, 0);
+            acc[j0][i] = mfma16<SPLIT>(sh[b][0], al[i], acc[j0][i]);
           if (j1 < kNTW2) {
 #pragma unroll
             for (int i = 0; i < MT; ++i)
-              acc[j1][i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(sh[b][1], al[i], acc[j1][i], 0, 0, 0);
+              acc[j1][i] = mfma16<SPLIT>(sh[b][1], al[i], acc[j1][i]);
           }
+        }
+        if (SPLIT >= 2) {
 #pragma unroll
           for (int i = 0; i < MT; ++i)
-            acc[j0][i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(sl[b][0], ah[i], acc[j0][i], 0, 0, 0);
+            acc[j0][i] = mfma16<SPLIT>(sl[b][0], ah[i], acc[j0][i]);
           if (j1 < kNTW2) {
 #pragma unroll
             for (int i = 0; i < MT; ++i)
-              acc[j1][i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(sl[b][1], ah[i], acc[j1][i], 0, 0, 0);
+              acc[j1][i] = mfma16<SPLIT>(sl[b][1], ah[i], acc[j1][i]);
           }
         }
         __builtin_amdgcn_sched_barrier(0);
@@ -459,24 +495,26 @@ apass2_kernel(int64_t M, int64_t N, int64_t KBtot, int64_t KBper, float alpha,
     } else {
       bf16x8_t sh[2], sl[2];
       sh[0] = cur[0][wn * kNTW2][lane];
-      if (SPLIT == 3) sl[0] = cur[NP - 1][wn * kNTW2][lane];
+      if (SPLIT >= 2) sl[0] = cur[NP - 1][wn * kNTW2][lane];
 #pragma unroll
       for (int j = 0; j < kNTW2; ++j) {
         const int b = j & 1;
         if (j + 1 < kNTW2) {
           sh[b ^ 1] = cur[0][wn * kNTW2 + j + 1][lane];
-          if (SPLIT == 3) sl[b ^ 1] = cur[NP - 1][wn * kNTW2 + j + 1][lane];
+          if (SPLIT >= 2) sl[b ^ 1] = cur[NP - 1][wn * kNTW2 + j + 1][lane];
         }
 #pragma unroll
         for (int i = 0; i < MT; ++i)
-          acc[j][i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(sh[b], ah[i], acc[j][i], 0, 0, 0);
+          acc[j][i] = mfma16<SPLIT>(sh[b], ah[i], acc[j][i]);
         if (SPLIT == 3) {
 #pragma unroll
           for (int i = 0; i < MT; ++i)
-            acc[j][i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(sh[b], al[i], acc[j][i], 0, 0, 0);
+            acc[j][i] = mfma16<SPLIT>(sh[b], al[i], acc[j][i]);
+        }
+        if (SPLIT >= 2) {
 #pragma unroll
           for (int i = 0; i < MT; ++i)
-            acc[j][i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(sl[b], ah[i], acc[j][i], 0, 0, 0);
+            acc[j][i] = mfma16<SPLIT>(sl[b], ah[i], acc[j][i]);
         }
         // keep the fragment reads one tile ahead (not all hoisted: registers)
         __builtin_amdgcn_sched_barrier(0);
@@ -504,6 +542,7 @@ apass2_kernel(int64_t M, int64_t N, int64_t KBtot, int64_t KBper, float alpha,
   // D[n][m] of tile (j, i): lane holds n = 4 (lane / 16) + rr, m = lane % 16
   float* out = (ks > 1) ? Wk + (int64_t)kz * M * (NTP * 16) : C;
   const int64_t ldo = (ks > 1) ? M : ldc;
+  const float rowsc = (SPLIT == 2 && TA) ? ascale[bx] : 1.0f;
 #pragma unroll
   for (int j = 0; j < kNTW2; ++j) {
 #pragma unroll
@@ -514,10 +553,11 @@ apass2_kernel(int64_t M, int64_t N, int64_t KBtot, int64_t KBper, float alpha,
         const int64_t n = (ntw + wn * kNTW2 + j) * 16 + 4 * (lane >> 4) + rr;
         if (m < M && n < N) {
           float* cp = out + m + n * ldo;
+          const float a = (SPLIT == 2) ? acc[j][i][rr] * (bscale[n] * rowsc) : acc[j][i][rr];
           if (ks > 1) {
-            *cp = acc[j][i][rr];
+            *cp = a;
           } else {
-            float v = alpha * acc[j][i][rr];
+            float v = alpha * a;
             if (beta != 0.0f) v += beta * (*cp);
             *cp = v;
           }
@@ -580,19 +620,21 @@ __global__ void chunk_absmax_kernel(int64_t M, int64_t N, const float* __restric
   if ((threadIdx.x & 63) == 0) atomicMax(scale_bits + c, __float_as_uint(mx));
 }
 
-// scale_bits -> a_c = max / 448 (1 for an all-zero chunk)
+// scale_bits -> a_c = max / top (1 for an all-zero chunk); top = the largest
+// value the scaled chunk may take (448 for e4m3, 32768 for fp16)
 __global__ void chunk_scale_finish_kernel(int64_t nc, const unsigned* __restrict__ bits,
-                                          float* __restrict__ scale) {
+                                          float top, float* __restrict__ scale) {
   const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (c < nc) {
     const float mx = __uint_as_float(bits[c]);
-    scale[c] = (mx > 0.0f) ? mx / 448.0f : 1.0f;
+    scale[c] = (mx > 0.0f) ? mx / top : 1.0f;
   }
 }
 
-// b[n] = max_k |S[k, n] a_{k/128}| / 448 (a optional), one workgroup per column
+// b[n] = max_k |S[k, n] a_{k/128}| / top (a optional), one workgroup per column
 __global__ void skinny_colscale_kernel(int64_t K, const float* __restrict__ S, int64_t lds,
-                                       const float* __restrict__ ascale, float* __restrict__ b) {
+                                       const float* __restrict__ ascale, float top,
+                                       float* __restrict__ b) {
   __shared__ float red[4];
   const int64_t n = blockIdx.x;
   float mx = 0.0f;
@@ -605,7 +647,7 @@ __global__ void skinny_colscale_kernel(int64_t K, const float* __restrict__ S, i
   __syncthreads();
   if (threadIdx.x == 0) {
     mx = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
-    b[n] = (mx > 0.0f) ? mx / 448.0f : 1.0f;
+    b[n] = (mx > 0.0f) ? mx / top : 1.0f;
   }
 }
 
@@ -630,6 +672,37 @@ __global__ void tile_skinny8_kernel(int64_t K, int64_t N, const float* __restric
       v[e] = (n < N && k < K) ? S[k + n * lds] * (ascale ? ascale[k >> 7] : 1.0f) : 0.0f;
     }
     St[t * 64 + lane] = pack_fp8x8(v, (n < N) ? 1.0f / bscale[n] : 0.0f);
+  }
+}
+
+// Two fp16 planes of S' = S a_{k/128} / b_n (a optional) in the fragment
+// order of tile_skinny_kernel: hi = fp16(S'), lo = fp16(S' - hi).
+__global__ void tile_skinny16_kernel(int64_t K, int64_t N, const float* __restrict__ S,
+                                     int64_t lds, int64_t KB, int64_t NTP,
+                                     const float* __restrict__ ascale,
+                                     const float* __restrict__ bscale,
+                                     bf16x8_t* __restrict__ St) {
+  const int64_t total = KB * NTP * 64;
+  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int lane = (int)(idx & 63);
+    const int64_t t = idx >> 6;
+    const int64_t nt = t % NTP, kb = t / NTP;
+    const int64_t n = nt * 16 + (lane & 15);
+    const int64_t k0 = kb * 32 + 8 * (lane >> 4);
+    const float inv_b = (n < N) ? 1.0f / bscale[n] : 0.0f;
+    bf16x8_t h, l;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int64_t k = k0 + e;
+      const float v =
+          (n < N && k < K) ? S[k + n * lds] * (ascale ? ascale[k >> 7] : 1.0f) * inv_b : 0.0f;
+      const _Float16 hv = (_Float16)v;
+      h[e] = __builtin_bit_cast(__bf16, hv);
+      l[e] = __builtin_bit_cast(__bf16, (_Float16)(v - (float)hv));
+    }
+    St[t * 64 + lane] = h;
+    St[(KB * NTP + t) * 64 + lane] = l;
   }
 }
 
@@ -821,7 +894,7 @@ int apass_bf16(Context* c, const char* tag, bool ta, int64_t M, int64_t N, int64
   hipLaunchKernelGGL((apass2_kernel<TA_, S_, V_, TA_ ? kMT2T : kMT2>), dim3(blocks), dim3(kT2), 0, \
                      c->stream, M, N, KB,                                                      \
                      KBper, alpha, A, lda, (const bf16x8_t*)St.p, NTP, beta, C, ldc, Wk.p,        \
-                     (int)gxp, (int)gy, ks)
+                     (int)gxp, (int)gy, ks, (const float*)nullptr, (const float*)nullptr)
       if (ta) {
         if (split == 3) {
           if (vec) RSVD_APASS2(true, 3, true);
@@ -876,7 +949,20 @@ int apass_bf16(Context* c, const char* tag, bool ta, int64_t M, int64_t N, int64
   return 0;
 }
 
+static int column_scales(Context* c, int64_t M, int64_t N, const float* A, int64_t lda, float top,
+                         float* scale);
+
 int fp8_column_scales(Context* c, int64_t M, int64_t N, const float* A, int64_t lda, float* scale) {
+  return column_scales(c, M, N, A, lda, 448.0f, scale);
+}
+
+int fp16_column_scales(Context* c, int64_t M, int64_t N, const float* A, int64_t lda,
+                       float* scale) {
+  return column_scales(c, M, N, A, lda, 32768.0f, scale);
+}
+
+static int column_scales(Context* c, int64_t M, int64_t N, const float* A, int64_t lda, float top,
+                         float* scale) {
   const int64_t nc = (N + 127) / 128;
   if (nc == 0) return 0;
   DBuf<unsigned> bits;
@@ -889,7 +975,7 @@ int fp8_column_scales(Context* c, int64_t M, int64_t N, const float* A, int64_t 
                      c->stream, M, N, A, lda, bits.p);
   RSVD_LAUNCH_CHECK();
   hipLaunchKernelGGL(chunk_scale_finish_kernel, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0,
-                     c->stream, nc, (const unsigned*)bits.p, scale);
+                     c->stream, nc, (const unsigned*)bits.p, top, scale);
   RSVD_LAUNCH_CHECK();
   return 0;
 }
@@ -915,7 +1001,7 @@ int apass_fp8(Context* c, const char* tag, bool ta, int64_t M, int64_t N, int64_
   RSVD_TRY(St.alloc((size_t)(KB * NTP * 64)));
   const float* afold = ta ? nullptr : ascale;  // op(A) = A: a_c folded into S
   hipLaunchKernelGGL(skinny_colscale_kernel, dim3((unsigned)N), dim3(256), 0, c->stream, K32, S,
-                     lds, afold, b.p);
+                     lds, afold, 448.0f, b.p);
   RSVD_LAUNCH_CHECK();
   {
     const int64_t total = KB * NTP * 64;
@@ -947,6 +1033,88 @@ int apass_fp8(Context* c, const char* tag, bool ta, int64_t M, int64_t N, int64_
   }
   if (K32 < K) {
     // the last K % 32 columns of op(A) in split-bf16: C += alpha op(A)(:, K32:) S(K32:, :)
+    const float* Ar = ta ? A + K32 : A + K32 * lda;
+    RSVD_TRY(gemm_bf16_general(c, tag, ta ? 'T' : 'N', 'N', M, N, K - K32, alpha, Ar, lda,
+                               S + K32, lds, 1.0f, C, ldc, 3));
+  }
+  return 0;
+}
+
+int apass_fp16(Context* c, const char* tag, bool ta, int64_t M, int64_t N, int64_t K, float alpha,
+               const float* A, int64_t lda, const float* ascale, const float* S, int64_t lds,
+               float beta, float* C, int64_t ldc) {
+  if (M < 0 || N < 0 || K < 0) return set_error("apass_fp16: negative dimension");
+  if (!ascale) return set_error("apass_fp16: column scales of A required");
+  if (M == 0 || N == 0) return 0;
+  const int64_t K32 = K - K % 32;
+  const int64_t KB = K32 / 32;
+  const int64_t cus = c->num_cus > 0 ? c->num_cus : 256;
+  const int64_t bm2 = ta ? 64 * kMT2T : kBM2;
+  const int64_t gx2 = (M + bm2 - 1) / bm2;
+  const int64_t gy2 = ((N + 15) / 16 + kBNT2 - 1) / kBNT2;
+  int ks = 1;
+  {
+    double best = 0.0;
+    double fill[9] = {0.0};
+    for (int t = 1; t <= 8 && KB / t >= 8; ++t) {
+      const double w = (double)(gx2 * gy2 * t) / (double)cus;
+      fill[t] = w / std::ceil(w);
+      if (fill[t] > best) best = fill[t];
+    }
+    for (int t = 1; t <= 8 && KB / t >= 8; ++t)
+      if (fill[t] >= best - 0.03) { ks = t; break; }
+  }
+  // the LDS-ring kernel only; shapes it does not fill run in split-bf16
+  if (K32 == 0 || KB < 8 || gx2 * gy2 * ks < cus || gy2 > 65535 || gx2 * gy2 * ks > 0x7fffffff)
+    return gemm_bf16(c, tag, ta ? 'T' : 'N', 'N', M, N, K, alpha, A, lda, S, lds, beta, C, ldc,
+                     3);
+  const int64_t NTP = gy2 * kBNT2;
+  DBuf<float> b;
+  DBuf<bf16x8_t> St;
+  RSVD_TRY(b.alloc((size_t)N));
+  RSVD_TRY(St.alloc((size_t)(2 * KB * NTP * 64)));
+  const float* afold = ta ? nullptr : ascale;  // op(A) = A: a_c folded into S
+  hipLaunchKernelGGL(skinny_colscale_kernel, dim3((unsigned)N), dim3(256), 0, c->stream, K32, S,
+                     lds, afold, 32768.0f, b.p);
+  RSVD_LAUNCH_CHECK();
+  {
+    const int64_t total = KB * NTP * 64;
+    int64_t blocks = (total + 255) / 256;
+    if (blocks > 16384) blocks = 16384;
+    hipLaunchKernelGGL(tile_skinny16_kernel, dim3((unsigned)blocks), dim3(256), 0, c->stream, K32,
+                       N, S, lds, KB, NTP, afold, (const float*)b.p, St.p);
+    RSVD_LAUNCH_CHECK();
+  }
+  const bool vec = ta && (reinterpret_cast<uintptr_t>(A) % 16 == 0) && (lda % 4 == 0);
+  DBuf<float> Wk;
+  if (ks > 1) RSVD_TRY(Wk.alloc((size_t)(ks * M * NTP * 16)));
+  const int64_t KBper = (KB + ks - 1) / ks;
+  const int64_t gxp = (gx2 + 7) / 8 * 8;
+  const unsigned blocks = (unsigned)(gxp * gy2 * ks);
+  {
+    KTimer kt(c, tag);
+#define RSVD_APASS16(TA_, V_)                                                                  \
+  hipLaunchKernelGGL((apass2_kernel<TA_, 2, V_, TA_ ? kMT2T : kMT2>), dim3(blocks), dim3(kT2), 0, \
+                     c->stream, M, N, KB, KBper, alpha, A, lda, (const bf16x8_t*)St.p, NTP, beta,  \
+                     C, ldc, Wk.p, (int)gxp, (int)gy2, ks, ascale, (const float*)b.p)
+    if (ta) {
+      if (vec) RSVD_APASS16(true, true);
+      else RSVD_APASS16(true, false);
+    } else {
+      RSVD_APASS16(false, false);
+    }
+#undef RSVD_APASS16
+    RSVD_LAUNCH_CHECK();
+    if (ks > 1) {
+      int64_t rb = (M * N + 255) / 256;
+      if (rb > 8192) rb = 8192;
+      hipLaunchKernelGGL(apass2_reduce_kernel, dim3((unsigned)rb), dim3(256), 0, c->stream, M, N,
+                         ks, (const float*)Wk.p, NTP * 16, alpha, beta, C, ldc);
+      RSVD_LAUNCH_CHECK();
+    }
+  }
+  if (K32 < K) {
+    // the last K % 32 columns of op(A) in split-bf16
     const float* Ar = ta ? A + K32 : A + K32 * lda;
     RSVD_TRY(gemm_bf16_general(c, tag, ta ? 'T' : 'N', 'N', M, N, K - K32, alpha, Ar, lda,
                                S + K32, lds, 1.0f, C, ldc, 3));

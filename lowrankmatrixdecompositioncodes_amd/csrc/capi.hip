@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "pipeline.h"
+#include "idcur.h"
 #include "pipeline_mp.h"
 #include "rsvdpack.h"
 #include "rsvdpack_ext.h"
@@ -636,6 +637,258 @@ void use_QB_decomp_for_approximation(mat* M, mat* Q, mat* B) {
 }
 
 // ======================================================================
+// interpolative and CUR decompositions (rra.c:1969-2332)
+// ======================================================================
+static vec* index_vec(const std::vector<int>& I) {
+  vec* v = vector_new((int)I.size());
+  for (size_t i = 0; i < I.size(); ++i) v->d[i] = (double)I[i];
+  return v;
+}
+
+// The QB inside draws RN of n x (kstep * nstep) columns after randQB_pb_new's
+// kstep clamp and TOL-mode nstep, with nstep = (k + p) / kstep (0 in TOL mode).
+static int idcur_setup(Api& api, mat* M, int k, int p, int kstep, DBuf<double>& dA,
+                       DBuf<double>& ob, OmegaSrc* om) {
+  Context* c = api.c;
+  const int64_t m = M->nrows, n = M->ncols;
+  if (kstep <= 0) return set_error("kstep must be positive (kstep=%d)", kstep);
+  int nstep = k > 0 ? (k + p) / kstep : 0, ks = kstep, tm = 0;
+  RSVD_TRY(qb_adjust(m, n, &ks, &nstep, 1, &tm));
+  RSVD_TRY(upload(c, M, dA));
+  return host_omega(c, n, (int64_t)ks * nstep, ob, om);
+}
+
+static mat* download_mat(Context* c, const double* src, int64_t rows, int64_t cols, int64_t ld) {
+  mat* H = matrix_new((int)rows, (int)cols);
+  if (download(c, src, rows, cols, ld, H->d) || stream_sync(c)) {
+    matrix_delete(H);
+    return nullptr;
+  }
+  return H;
+}
+
+// rra.c:924-975 pivotedQR_mkl: M(:, I) = Q R with column pivoting (every
+// column free), k = min(m, n); R is k x n (m <= n) or k x k, Q m x k.
+void pivotedQR_mkl(mat* M, mat** Q, mat** R, vec** I) {
+  if (Q) *Q = nullptr;
+  if (R) *R = nullptr;
+  if (I) *I = nullptr;
+  Api api;
+  if (api.rc) return;
+  if (!valid(M) || !Q || !R || !I) {
+    set_error("pivotedQR_mkl: bad arguments");
+    return;
+  }
+  Context* c = api.c;
+  const int64_t m = M->nrows, n = M->ncols, k = m < n ? m : n;
+  const int64_t Rc = m <= n ? n : k;
+  DBuf<double> dA, tau, Rd;
+  DBuf<int> perm;
+  if (upload(c, M, dA) || tau.alloc((size_t)(k > 0 ? k : 1)) || perm.alloc((size_t)(n > 0 ? n : 1)))
+    return;
+  if (pivoted_qr_f64(c, m, n, dA.p, m, perm.p, tau.p)) return;
+  // R: the upper trapezoid of the first k rows, before Q overwrites them
+  if (Rd.alloc((size_t)(k * Rc > 0 ? k * Rc : 1)) || copy2d_f64(c, k, Rc, dA.p, m, Rd.p, k) ||
+      zero_lower_f64(c, k, Rc, Rd.p, k) || org2r_f64(c, m, k, dA.p, m, tau.p))
+    return;
+  std::vector<int> ip((size_t)n);
+  mat* Qh = matrix_new((int)m, (int)k);
+  mat* Rh = matrix_new((int)k, (int)Rc);
+  if (download(c, dA.p, m, k, m, Qh->d) || download(c, Rd.p, k, Rc, k, Rh->d) ||
+      hipMemcpyAsync(ip.data(), perm.p, sizeof(int) * (size_t)n, hipMemcpyDeviceToHost,
+                     c->stream) != hipSuccess ||
+      stream_sync(c)) {
+    set_error("pivotedQR_mkl: copy back failed");
+    matrix_delete(Qh);
+    matrix_delete(Rh);
+    return;
+  }
+  *Q = Qh;
+  *R = Rh;
+  *I = index_vec(ip);
+}
+
+void id_blockrand_decomp_fixed_rank_or_prec(mat* M, int k, int p, double TOL, int kstep, int q,
+                                            int s, int* frank, vec** I, mat** T) {
+  if (I) *I = nullptr;
+  if (T) *T = nullptr;
+  Api api;
+  if (api.rc) return;
+  if (!valid(M) || !I || !T || !frank) {
+    set_error("id_blockrand_decomp_fixed_rank_or_prec: bad arguments");
+    return;
+  }
+  DBuf<double> dA, ob;
+  OmegaSrc om;
+  if (idcur_setup(api, M, k, p, kstep, dA, ob, &om)) return;
+  const int64_t m = M->nrows, n = M->ncols;
+  IdOut out;
+  if (id_blockrand_f64(api.c, dA.p, m, n, m, k, p, TOL, kstep, q, s, om, &out)) return;
+  mat* Th = download_mat(api.c, out.T.p, out.frank, n - out.frank, out.frank);
+  if (!Th) return;
+  *frank = out.frank;
+  *I = index_vec(out.I);
+  *T = Th;
+}
+
+void id_two_sided_blockrand_decomp_fixed_rank_or_prec(mat* M, int k, int p, double TOL, int kstep,
+                                                      int q, int s, int* frank, vec** Icol,
+                                                      vec** Irow, mat** T, mat** S) {
+  if (Icol) *Icol = nullptr;
+  if (Irow) *Irow = nullptr;
+  if (T) *T = nullptr;
+  if (S) *S = nullptr;
+  Api api;
+  if (api.rc) return;
+  if (!valid(M) || !Icol || !Irow || !T || !S || !frank) {
+    set_error("id_two_sided_blockrand_decomp_fixed_rank_or_prec: bad arguments");
+    return;
+  }
+  DBuf<double> dA, ob;
+  OmegaSrc om;
+  if (idcur_setup(api, M, k, p, kstep, dA, ob, &om)) return;
+  const int64_t m = M->nrows, n = M->ncols;
+  IdOut col, row;
+  if (id_two_sided_blockrand_f64(api.c, dA.p, m, n, m, k, p, TOL, kstep, q, s, om, &col, &row))
+    return;
+  const int f = col.frank;
+  mat* Th = download_mat(api.c, col.T.p, f, n - f, f);
+  mat* Sh = Th ? download_mat(api.c, row.T.p, f, m - f, f) : nullptr;
+  if (!Sh) {
+    matrix_delete(Th);
+    return;
+  }
+  *frank = f;
+  *Icol = index_vec(col.I);
+  *Irow = index_vec(row.I);
+  *T = Th;
+  *S = Sh;
+}
+
+void cur_blockrand_decomp_fixed_rank_or_prec(mat* M, int k, int p, double TOL, int kstep, int q,
+                                             int s, int* frank, mat** C, mat** U, mat** R) {
+  if (C) *C = nullptr;
+  if (U) *U = nullptr;
+  if (R) *R = nullptr;
+  Api api;
+  if (api.rc) return;
+  if (!valid(M) || !C || !U || !R || !frank) {
+    set_error("cur_blockrand_decomp_fixed_rank_or_prec: bad arguments");
+    return;
+  }
+  DBuf<double> dA, ob;
+  OmegaSrc om;
+  if (idcur_setup(api, M, k, p, kstep, dA, ob, &om)) return;
+  const int64_t m = M->nrows, n = M->ncols;
+  CurOut out;
+  if (cur_blockrand_f64(api.c, dA.p, m, n, m, k, p, TOL, kstep, q, s, om, &out)) return;
+  const int f = out.frank;
+  mat* Ch = download_mat(api.c, out.C.p, m, f, m);
+  mat* Uh = Ch ? download_mat(api.c, out.U.p, f, f, f) : nullptr;
+  mat* Rh = Uh ? download_mat(api.c, out.R.p, f, n, f) : nullptr;
+  if (!Rh) {
+    matrix_delete(Ch);
+    matrix_delete(Uh);
+    return;
+  }
+  *frank = f;
+  *C = Ch;
+  *U = Uh;
+  *R = Rh;
+}
+
+// MKL/mvf.c:1322-1336: P = (C U) R.
+void form_cur_product_matrix(mat* C, mat* U, mat* R, mat* P) {
+  clear_error();
+  if (!valid(C) || !valid(U) || !valid(R) || !valid(P) || C->ncols != U->nrows ||
+      U->ncols != R->nrows || P->nrows != C->nrows || P->ncols != R->ncols) {
+    set_error("form_cur_product_matrix: shape mismatch");
+    return;
+  }
+  mat* CU = matrix_new(C->nrows, U->ncols);
+  matrix_matrix_mult(C, U, CU);
+  matrix_matrix_mult(CU, R, P);
+  matrix_delete(CU);
+}
+
+// [I_k; X^T](inv, :) for an ID's coefficients X (k x (n - k)) and pivots I.
+static mat* interp_rows(mat* X, vec* I, int k) {
+  const int n = I->nrows;
+  std::vector<int> inv((size_t)n);
+  for (int i = 0; i < n; ++i) inv[(size_t)(int)I->d[i]] = i;          // mvf.c:1195-1201
+  mat* V = matrix_new(n, k);
+  for (int j = 0; j < k; ++j)
+    for (int i = 0; i < n; ++i) {
+      const int r = inv[(size_t)i];
+      V->d[(size_t)j * n + i] = r < k ? (r == j ? 1.0 : 0.0) : X->d[(size_t)(r - k) * k + j];
+    }
+  return V;
+}
+
+// rra.c:2402-2478: MA = M(:, I(1:k)) V^T with V = [I_k; T^T](I^-1, :).
+void use_id_decomp_for_approximation(mat* M, mat* T, vec* I, int k) {
+  clear_error();
+  if (!valid(M) || !valid(T) || !I || I->nrows != M->ncols || k <= 0 || k > M->ncols ||
+      T->nrows != k || T->ncols != M->ncols - k) {
+    set_error("use_id_decomp_for_approximation: shape mismatch");
+    return;
+  }
+  const int m = M->nrows;
+  mat* V = interp_rows(T, I, k);
+  mat* MI = matrix_new(m, k);
+  for (int j = 0; j < k; ++j)
+    memcpy(MI->d + (size_t)j * m, M->d + (size_t)(int)I->d[j] * m, sizeof(double) * m);
+  mat* MA = matrix_new(m, M->ncols);
+  matrix_matrix_transpose_mult(MI, V, MA);
+  printf("norm(M,fro) = %f\n", get_matrix_frobenius_norm(M));
+  printf("norm(MA,fro) = %f\n", get_matrix_frobenius_norm(MA));
+  printf("percent error = %f\n", get_percent_error_between_two_mats(M, MA));
+  matrix_delete(V);
+  matrix_delete(MI);
+  matrix_delete(MA);
+}
+
+// rra.c:2484-2556: MA = U M(Irow(1:k), Icol(1:k)) V^T with V from (Icol, T) and
+// U = [I_k; S^T](Irow^-1, :).
+void use_id_two_sided_decomp_for_approximation(mat* M, mat* T, mat* S, vec* Icol, vec* Irow,
+                                               int k) {
+  clear_error();
+  if (!valid(M) || !valid(T) || !valid(S) || !Icol || !Irow || Icol->nrows != M->ncols ||
+      Irow->nrows != M->nrows || k <= 0 || T->nrows != k || S->nrows != k ||
+      T->ncols != M->ncols - k || S->ncols != M->nrows - k) {
+    set_error("use_id_two_sided_decomp_for_approximation: shape mismatch");
+    return;
+  }
+  const int m = M->nrows, n = M->ncols;
+  mat* V = interp_rows(T, Icol, k);
+  mat* U = interp_rows(S, Irow, k);
+  mat* MIJ = matrix_new(k, k);
+  for (int j = 0; j < k; ++j)
+    for (int i = 0; i < k; ++i)
+      MIJ->d[(size_t)j * k + i] = M->d[(size_t)(int)Icol->d[j] * m + (size_t)(int)Irow->d[i]];
+  mat* UMIJ = matrix_new(m, k);
+  mat* MA = matrix_new(m, n);
+  matrix_matrix_mult(U, MIJ, UMIJ);
+  matrix_matrix_transpose_mult(UMIJ, V, MA);
+  printf("norm(M,fro) = %f\n", get_matrix_frobenius_norm(M));
+  printf("norm(MA,fro) = %f\n", get_matrix_frobenius_norm(MA));
+  printf("percent error = %f\n", get_percent_error_between_two_mats(M, MA));
+  matrix_delete(V);
+  matrix_delete(U);
+  matrix_delete(MIJ);
+  matrix_delete(UMIJ);
+  matrix_delete(MA);
+}
+
+// rra.c:2563-2573
+void use_cur_decomp_for_approximation(mat* M, mat* C, mat* U, mat* R) {
+  mat* P = matrix_new(M->nrows, M->ncols);
+  form_cur_product_matrix(C, U, R, P);
+  report(M, P);
+  matrix_delete(P);
+}
+
+// ======================================================================
 // extensions
 // ======================================================================
 const char* rsvd_last_error(void) { return last_error(); }
@@ -850,6 +1103,16 @@ static int dev_gemm(Context* c, int dtype, char ta, char tb, long long M, long l
   if (dtype == 5)
     return gemm_bf16_general(c, "gemm_bf16x6", ta, tb, M, N, K, (float)alpha, (const float*)A,
                              lda, (const float*)B, ldb, (float)beta, (float*)C, ldc, 6);
+  if (dtype == 6) {
+    if (tb != 'N' && tb != 'n') return set_error("rsvd_dev_gemm: fp16 needs tb = 'N'");
+    const bool t = (ta == 'T' || ta == 't');
+    const int64_t ar = t ? K : M, ac = t ? M : K;  // A is ar x ac
+    DBuf<float> sc;
+    RSVD_TRY(sc.alloc((size_t)((ac + 127) / 128 + 1)));
+    RSVD_TRY(fp16_column_scales(c, ar, ac, (const float*)A, lda, sc.p));
+    return apass_fp16(c, "gemm_fp16", t, M, N, K, (float)alpha, (const float*)A, lda, sc.p,
+                      (const float*)B, ldb, (float)beta, (float*)C, ldc);
+  }
   return set_error("rsvd_dev_gemm: dtype %d not supported", dtype);
 }
 
@@ -1046,10 +1309,36 @@ int rsvd_dev_learned_lowrank(const float* dTheta, const double* dA, const float*
     RSVD_TRY(read_small(c, &s2, nrm.p, sizeof s2));
   }
   RSVD_TRY(stream_sync(c));
-  *frank = fr;
-  ob.release(dU, dS, dV);
-  if (fell_back) *fell_back = 1;
-  if (resid_rel) *resid_rel = sqrt(a2 > s2 ? a2 - s2 : 0.0) / sqrt(a2);
+  const double res_fb = sqrt(a2 > s2 ? a2 - s2 : 0.0);
+  if (res_fb <= tol_rel * sqrt(a2) || m < n) {
+    *frank = fr;
+    ob.release(dU, dS, dV);
+    if (fell_back) *fell_back = 1;
+    if (resid_rel) *resid_rel = res_fb / sqrt(a2);
+    return 0;
+  }
+  // TOL is out of the adaptive method's reach below full rank: the exact
+  // full-rank decomposition (k = n, p = 0, Omega = I: a thin QR of A and the
+  // small eigenproblem), its residual formed explicitly
+  DBuf<double> I;
+  RSVD_TRY(I.alloc((size_t)(n * n)));
+  RSVD_TRY(fill2d_f64(c, n, n, 0.0, I.p, n));
+  RSVD_TRY(fill2d_f64(c, 1, n, 1.0, I.p, n + 1));
+  OmegaSrc oi;
+  oi.dev = I.p;
+  oi.ld = n;
+  OutBufs ox;
+  RSVD_TRY(ox.alloc(sizeof(double) * (size_t)(m * n), sizeof(double) * (size_t)n,
+                    sizeof(double) * (size_t)(n * n)));
+  RSVD_TRY(fixed_rank_f64(c, dA, m, n, lda, (int)n, 0, 2, 1, 1, oi, (double*)ox.p[0], m,
+                          (double*)ox.p[1], (double*)ox.p[2], n, nullptr));
+  double res_x = 0;
+  RSVD_TRY(lowrank_residual(c, dA, m, n, lda, n, ox.p[0], m, (const double*)ox.p[1], ox.p[2], n,
+                            0, &res_x));
+  *frank = (int)n;
+  ox.release(dU, dS, dV);
+  if (fell_back) *fell_back = 2;
+  if (resid_rel) *resid_rel = res_x / sqrt(a2);
   return 0;
 }
 

@@ -6,10 +6,14 @@
 namespace rsvd {
 
 // Precision of the sketch and power-iteration passes over A (the projection
-// B^T = A^T Q is always split-bf16): bf16, split-bf16, or fp8 e4m3 with
-// per-128-column scales, for every pass; or bf16 (MP_BF16_SPLIT) / fp8
-// (MP_FP8_SPLIT) for every pass but the last, which is split-bf16.
-enum MpPrecision { MP_BF16 = 1, MP_BF16X3 = 2, MP_FP8 = 3, MP_BF16_SPLIT = 4, MP_FP8_SPLIT = 5 };
+// B^T = A^T Q is split-bf16, except for MP_FP16): bf16, split-bf16, or fp8
+// e4m3 with per-128-column scales, for every pass; or bf16 (MP_BF16_SPLIT) /
+// fp8 (MP_FP8_SPLIT) for every pass but the last, which is split-bf16; or
+// MP_FP16: every pass over A, the projection included, with A in fp16 (per
+// 128-column scales) against the skinny operand in two fp16 planes.
+enum MpPrecision {
+  MP_BF16 = 1, MP_BF16X3 = 2, MP_FP8 = 3, MP_BF16_SPLIT = 4, MP_FP8_SPLIT = 5, MP_FP16 = 6
+};
 
 struct MpInfo {
   int qr_fallbacks = 0;
@@ -27,11 +31,12 @@ int fixed_rank_mp(Context* c, const float* A, int64_t m, int64_t n, int64_t lda,
 // A, the vnum branches of rra.c:133-225: B^T = A^T Q in split-bf16 (or the
 // caller's Bt_in, n x l, ld ldbt), the small problem in FP64, U and V
 // products in split-bf16 (exact = 0) or with fp32-exact products (exact = 1).
-// *sw = Jacobi sweeps, *fb += QR fallbacks.
+// fp16_ascale (optional): the fp16 chunk scales of A, to form B^T on the fp16
+// path instead.  *sw = Jacobi sweeps, *fb += QR fallbacks.
 int svd_from_Q_mp(Context* c, const float* A, int64_t m, int64_t n, int64_t lda, const float* Q,
                   int64_t ldq, int64_t l, int64_t k, int vnum, const float* Bt_in, int64_t ldbt,
                   int exact, float* U, int64_t ldu, double* S, float* V, int64_t ldv, int* sw,
-                  int* fb);
+                  int* fb, const float* fp16_ascale = nullptr);
 
 // randQB_pb_new (is_new = 1) / randQB_pb on a float32 A: a float32 working
 // copy R of A is deflated block by block (R -= Qp Bp with fp32-exact products,

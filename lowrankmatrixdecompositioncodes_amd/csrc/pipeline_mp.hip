@@ -136,7 +136,7 @@ static int orth_f32(Context* c, int64_t m, int64_t n, float* Y, int64_t ldy, int
 int svd_from_Q_mp(Context* c, const float* A, int64_t m, int64_t n, int64_t lda, const float* Q,
                   int64_t ldq, int64_t l, int64_t k, int vnum, const float* Bt_in, int64_t ldbt,
                   int exact, float* U, int64_t ldu, double* S, float* V, int64_t ldv, int* sw,
-                  int* fb) {
+                  int* fb, const float* fp16_ascale) {
   if (k <= 0 || k > l) return set_error("svd_from_Q_mp: k=%lld outside [1, l=%lld]",
                                         (long long)k, (long long)l);
   if ((vnum == 1 || vnum > 2) && n < l)
@@ -149,8 +149,12 @@ int svd_from_Q_mp(Context* c, const float* A, int64_t m, int64_t n, int64_t lda,
   int64_t ldb = ldbt;
   if (!Btp) {
     RSVD_TRY(Btb.alloc((size_t)(n * l)));
-    RSVD_TRY(gemm_bf16(c, "proj_bf16x3", 'T', 'N', n, l, m, 1.0f, A, lda, Q, ldq, 0.0f, Btb.p, n,
-                       3));
+    if (fp16_ascale)
+      RSVD_TRY(apass_fp16(c, "proj_fp16", true, n, l, m, 1.0f, A, lda, fp16_ascale, Q, ldq, 0.0f,
+                          Btb.p, n));
+    else
+      RSVD_TRY(gemm_bf16(c, "proj_bf16x3", 'T', 'N', n, l, m, 1.0f, A, lda, Q, ldq, 0.0f, Btb.p,
+                         n, 3));
     Btp = Btb.p;
     ldb = n;
   }
@@ -207,12 +211,14 @@ int svd_from_Q_mp(Context* c, const float* A, int64_t m, int64_t n, int64_t lda,
 }
 
 // One pass C = op(A) S (N columns) at the sketch precision; ascale: the fp8
-// chunk scales of A (prec = MP_FP8 only).
+// or fp16 chunk scales of A (prec = MP_FP8 / MP_FP16 only).
 static int mp_pass(Context* c, int prec, bool ta, int64_t M, int64_t N, int64_t K, const float* A,
                    int64_t lda, const float* ascale, const float* S, int64_t lds, float* C,
                    int64_t ldc) {
   if (prec == MP_FP8)
     return apass_fp8(c, "apass_fp8", ta, M, N, K, 1.0f, A, lda, ascale, S, lds, 0.0f, C, ldc);
+  if (prec == MP_FP16)
+    return apass_fp16(c, "apass_fp16", ta, M, N, K, 1.0f, A, lda, ascale, S, lds, 0.0f, C, ldc);
   const int split = (prec == MP_BF16X3) ? 3 : 1;
   return gemm_bf16(c, split == 3 ? "apass_bf16x3" : "apass_bf16", ta ? 'T' : 'N', 'N', M, N, K,
                    1.0f, A, lda, S, lds, 0.0f, C, ldc, split);
@@ -225,24 +231,25 @@ int fixed_rank_mp(Context* c, const float* A, int64_t m, int64_t n, int64_t lda,
   if (k <= 0 || p < 0) return set_error("fixed_rank_mp: need k > 0 and p >= 0");
   if (l > (m < n ? m : n)) return set_error("fixed_rank_mp: k+p exceeds min(m,n)");
   if (s <= 0) return set_error("fixed_rank_mp: s must be positive");
-  if (prec < MP_BF16 || prec > MP_FP8_SPLIT)
+  if (prec < MP_BF16 || prec > MP_FP16)
     return set_error("fixed_rank_mp: unknown precision %d", prec);
   // precision of pass i of npass (the sketch, then 2(q-1) power passes): the
   // two-level modes run every pass but the last in bf16 / fp8 and the last,
   // whose range becomes Q, in split-bf16
   const int npass = 1 + 2 * (q > 1 ? q - 1 : 0);
   auto prec_of = [&](int i) -> int {
-    if (prec <= MP_FP8) return prec;
+    if (prec <= MP_FP8 || prec == MP_FP16) return prec;
     if (i + 1 == npass) return MP_BF16X3;
     return prec == MP_BF16_SPLIT ? MP_BF16 : MP_FP8;
   };
-  // fp8: the per-128-column scales of A, once per call
+  // fp8 / fp16: the per-128-column scales of A, once per call
   DBuf<float> ascale;
   bool any_fp8 = false;
   for (int i = 0; i < npass; ++i) any_fp8 = any_fp8 || prec_of(i) == MP_FP8;
-  if (any_fp8) {
+  if (any_fp8 || prec == MP_FP16) {
     RSVD_TRY(ascale.alloc((size_t)((n + 127) / 128)));
-    RSVD_TRY(fp8_column_scales(c, m, n, A, lda, ascale.p));
+    if (prec == MP_FP16) RSVD_TRY(fp16_column_scales(c, m, n, A, lda, ascale.p));
+    else RSVD_TRY(fp8_column_scales(c, m, n, A, lda, ascale.p));
   }
   // one sketch / power pass: C = op(A) S
   int ip = 0;
@@ -270,7 +277,7 @@ int fixed_rank_mp(Context* c, const float* A, int64_t m, int64_t n, int64_t lda,
   RSVD_TRY(orth_f32(c, m, l, Y.p, m, 2, 1, &fb));                      // rra.c:129-130
   int sw = 0;
   RSVD_TRY(svd_from_Q_mp(c, A, m, n, lda, Y.p, m, l, k, vnum, nullptr, 0, 0, U, ldu, S, V, ldv,
-                         &sw, &fb));
+                         &sw, &fb, prec == MP_FP16 ? ascale.p : nullptr));
   if (info) {
     info->qr_fallbacks = fb;
     info->jacobi_sweeps = sw;

@@ -118,16 +118,21 @@ int householder_qr_f64(Context* c, int64_t m, int64_t n, double* Y, int64_t ldy,
                        (const double*)Y, ldy, R);
     RSVD_LAUNCH_CHECK();
   }
+  return org2r_f64(c, m, n, Y, ldy, tau.p);
+}
+
+int org2r_f64(Context* c, int64_t m, int64_t n, double* Y, int64_t ldy, const double* tau) {
+  if (n <= 0 || m <= 0) return 0;
   for (int64_t j = n - 1; j >= 0; --j) {
     if (j + 1 < n) {
       hipLaunchKernelGGL(hh_apply_kernel, dim3((unsigned)(n - j - 1)), dim3(256), 0, c->stream, m,
-                         Y, ldy, j, j + 1, (const double*)tau.p, (const double*)Y, ldy);
+                         Y, ldy, j, j + 1, tau, (const double*)Y, ldy);
       RSVD_LAUNCH_CHECK();
     }
     int blocks = (int)((m + 255) / 256);
     if (blocks > 1024) blocks = 1024;
     hipLaunchKernelGGL(hh_form_col_kernel, dim3(blocks), dim3(256), 0, c->stream, m, Y, ldy, j,
-                       (const double*)tau.p);
+                       tau);
     RSVD_LAUNCH_CHECK();
   }
   return 0;

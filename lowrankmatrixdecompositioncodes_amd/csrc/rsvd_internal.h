@@ -206,6 +206,16 @@ int fp8_column_scales(Context* c, int64_t M, int64_t N, const float* A, int64_t 
 int apass_fp8(Context* c, const char* tag, bool ta, int64_t M, int64_t N, int64_t K, float alpha,
               const float* A, int64_t lda, const float* ascale, const float* S, int64_t lds,
               float beta, float* C, int64_t ldc);
+// a_c = max |A| over the 128-column chunk c / 32768: the fp16 scales.
+int fp16_column_scales(Context* c, int64_t M, int64_t N, const float* A, int64_t lda,
+                       float* scale);
+// C = alpha op(A) S + beta C with A / a_c rounded to fp16 (ascale from
+// fp16_column_scales of A) and S in two fp16 planes (hi + lo, per-column
+// scales), two products on the fp16 matrix cores, fp32 accumulation; shapes
+// the streaming kernel does not fill, and a K % 32 remainder, in split-bf16.
+int apass_fp16(Context* c, const char* tag, bool ta, int64_t M, int64_t N, int64_t K, float alpha,
+               const float* A, int64_t lda, const float* ascale, const float* S, int64_t lds,
+               float beta, float* C, int64_t ldc);
 // true when apass_bf16 has enough row blocks to fill the GPU.
 bool apass_suitable(Context* c, int64_t M, int64_t N);
 // The tiled bf16 GEMM without the A-streaming dispatch (any shape).
@@ -275,6 +285,9 @@ int orth_f64(Context* c, int64_t m, int64_t n, double* Y, int64_t ldy, double* R
              int* used_fallback);
 // Householder QR (geqr2 + org2r) in place, R optional.
 int householder_qr_f64(Context* c, int64_t m, int64_t n, double* Y, int64_t ldy, double* R);
+// org2r: the first n columns of Q = H_0 ... H_{n-1} in place of the reflectors
+// stored below the diagonal of Y (m x n, v_j(j) = 1 implied) with tau (device).
+int org2r_f64(Context* c, int64_t m, int64_t n, double* Y, int64_t ldy, const double* tau);
 // R = chol(G) (upper) and Rinv = R^-1 in one blocked pass (n x n, ld n, zero
 // below the diagonal).  G's trailing part is overwritten.  *ok = 0 when a
 // pivot is not safely positive.

@@ -177,7 +177,7 @@ def learned_lowrank(theta: rp.DeviceMatrix, fm: FamilyMatrix, tol_rel: float = T
     k = fr.value
     sig = np.empty(k)
     out = {"frank": k, "sigma": sig, "q_pred": qp.value, "k_pred": kp.value,
-           "fell_back": bool(fb.value), "resid_rel": res.value, "seconds": dt}
+           "fell_back": int(fb.value), "resid_rel": res.value, "seconds": dt}
     if k:
         rp._check(lib.rsvd_memcpy_d2h(sig.ctypes.data, s.value, 8 * k), "d2h")
         if keep:

@@ -90,6 +90,21 @@ _SIGS = {
                              C.POINTER(C.c_int), C.POINTER(MatP), C.POINTER(MatP)]),
     "use_low_rank_svd_for_approximation": (None, [MatP, MatP, MatP, MatP]),
     "use_QB_decomp_for_approximation": (None, [MatP, MatP, MatP]),
+    "pivotedQR_mkl": (None, [MatP, C.POINTER(MatP), C.POINTER(MatP), C.POINTER(VecP)]),
+    "id_blockrand_decomp_fixed_rank_or_prec": (
+        None, [MatP, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int,
+               C.POINTER(C.c_int), C.POINTER(VecP), C.POINTER(MatP)]),
+    "id_two_sided_blockrand_decomp_fixed_rank_or_prec": (
+        None, [MatP, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int,
+               C.POINTER(C.c_int), C.POINTER(VecP), C.POINTER(VecP), C.POINTER(MatP),
+               C.POINTER(MatP)]),
+    "cur_blockrand_decomp_fixed_rank_or_prec": (
+        None, [MatP, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int,
+               C.POINTER(C.c_int), C.POINTER(MatP), C.POINTER(MatP), C.POINTER(MatP)]),
+    "form_cur_product_matrix": (None, [MatP, MatP, MatP, MatP]),
+    "use_id_decomp_for_approximation": (None, [MatP, MatP, VecP, C.c_int]),
+    "use_id_two_sided_decomp_for_approximation": (None, [MatP, MatP, MatP, VecP, VecP, C.c_int]),
+    "use_cur_decomp_for_approximation": (None, [MatP, MatP, MatP, MatP]),
     # extensions
     "rsvd_last_error": (C.c_char_p, []),
     "rsvd_init": (C.c_int, []),
@@ -457,6 +472,69 @@ def randQB_pb_new(M, kstep, nstep, TOL, q, s):
 
 
 # ------------------------------------------------------------------ device arrays
+def _from_vec(v) -> np.ndarray:
+    n = v.contents.nrows
+    out = np.array([v.contents.d[i] for i in range(n)], dtype=np.float64)
+    load().vector_delete(v)
+    return out
+
+
+def pivotedQR_mkl(M):
+    """Column-pivoted QR; returns (Q, R, I) with M[:, I] = Q R."""
+    lib = load()
+    Mm = to_mat(M)
+    Q, R, I = MatP(), MatP(), VecP()
+    lib.pivotedQR_mkl(Mm, C.byref(Q), C.byref(R), C.byref(I))
+    lib.matrix_delete(Mm)
+    if not Q:
+        raise RsvdError("pivotedQR_mkl: " + last_error())
+    return from_mat(Q), from_mat(R), _from_vec(I).astype(np.int64)
+
+
+def id_blockrand_decomp_fixed_rank_or_prec(M, k, p, TOL, kstep, q, s):
+    """Column ID; returns (I as int64 0-based indices, T, frank)."""
+    lib = load()
+    Mm = to_mat(M)
+    fr = C.c_int(0)
+    I, T = VecP(), MatP()
+    lib.id_blockrand_decomp_fixed_rank_or_prec(Mm, k, p, TOL, kstep, q, s, C.byref(fr),
+                                               C.byref(I), C.byref(T))
+    lib.matrix_delete(Mm)
+    if not I or not T:
+        raise RsvdError("id_blockrand_decomp_fixed_rank_or_prec: " + last_error())
+    return _from_vec(I).astype(np.int64), from_mat(T), fr.value
+
+
+def id_two_sided_blockrand_decomp_fixed_rank_or_prec(M, k, p, TOL, kstep, q, s):
+    """Two-sided ID; returns (Icol, Irow, T, S, frank)."""
+    lib = load()
+    Mm = to_mat(M)
+    fr = C.c_int(0)
+    Ic, Ir, T, S = VecP(), VecP(), MatP(), MatP()
+    lib.id_two_sided_blockrand_decomp_fixed_rank_or_prec(Mm, k, p, TOL, kstep, q, s, C.byref(fr),
+                                                         C.byref(Ic), C.byref(Ir), C.byref(T),
+                                                         C.byref(S))
+    lib.matrix_delete(Mm)
+    if not Ic or not T:
+        raise RsvdError("id_two_sided_blockrand_decomp_fixed_rank_or_prec: " + last_error())
+    return (_from_vec(Ic).astype(np.int64), _from_vec(Ir).astype(np.int64), from_mat(T),
+            from_mat(S), fr.value)
+
+
+def cur_blockrand_decomp_fixed_rank_or_prec(M, k, p, TOL, kstep, q, s):
+    """CUR; returns (C, U, R, frank)."""
+    lib = load()
+    Mm = to_mat(M)
+    fr = C.c_int(0)
+    Cm, Um, Rm = MatP(), MatP(), MatP()
+    lib.cur_blockrand_decomp_fixed_rank_or_prec(Mm, k, p, TOL, kstep, q, s, C.byref(fr),
+                                                C.byref(Cm), C.byref(Um), C.byref(Rm))
+    lib.matrix_delete(Mm)
+    if not Cm:
+        raise RsvdError("cur_blockrand_decomp_fixed_rank_or_prec: " + last_error())
+    return from_mat(Cm), from_mat(Um), from_mat(Rm), fr.value
+
+
 class DeviceMatrix:
     """A column-major matrix in HBM, allocated through rsvd_dev_alloc."""
 

@@ -66,6 +66,13 @@ typedef void (*dgesvd_t)(const char *, const char *, const int *, const int *, d
                          const int *, double *, double *, const int *, double *, const int *,
                          double *, const int *, int *, size_t, size_t);
 typedef void (*setthreads_t)(int);
+typedef void (*dgeqp3_t)(const int *, const int *, double *, const int *, int *, double *,
+                         double *, const int *, int *);
+typedef void (*dtrsm_t)(const char *, const char *, const char *, const char *, const int *,
+                        const int *, const double *, const double *, const int *, double *,
+                        const int *, size_t, size_t, size_t, size_t);
+typedef void (*dgesv_t)(const int *, const int *, double *, const int *, int *, double *,
+                        const int *, int *);
 /* single precision (the oneAPI variant, see the FP32 section below) */
 typedef void (*sgemm_t)(const char *, const char *, const int *, const int *, const int *,
                         const float *, const float *, const int *, const float *, const int *,
@@ -86,6 +93,9 @@ static dorgqr_t p_dorgqr;
 static dsyev_t p_dsyev;
 static dgesvd_t p_dgesvd;
 static setthreads_t p_setthreads;
+static dgeqp3_t p_dgeqp3;
+static dtrsm_t p_dtrsm;
+static dgesv_t p_dgesv;
 static sgemm_t p_sgemm;
 static sgeqrf_t p_sgeqrf;
 static sorgqr_t p_sorgqr;
@@ -107,6 +117,9 @@ static int bind_prefix(void *h, const char *pre) {
     BIND(p_dorgqr, dorgqr_t, "dorgqr_");
     BIND(p_dsyev, dsyev_t, "dsyev_");
     BIND(p_dgesvd, dgesvd_t, "dgesvd_");
+    BIND(p_dgeqp3, dgeqp3_t, "dgeqp3_");
+    BIND(p_dtrsm, dtrsm_t, "dtrsm_");
+    BIND(p_dgesv, dgesv_t, "dgesv_");
     BIND(p_sgemm, sgemm_t, "sgemm_");
     BIND(p_sgeqrf, sgeqrf_t, "sgeqrf_");
     BIND(p_sorgqr, sorgqr_t, "sorgqr_");
@@ -662,6 +675,167 @@ int oref_blockrand(mat *M, int k, int p, double TOL, int vnum, int kstep, int q,
     svd_from_Q(M, Q, k, vnum, U, S, V);                   /* rra.c:288-380, re-reads M */
     oref_matrix_delete(Q);
     oref_matrix_delete(B);
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Interpolative and CUR decompositions on top of the QB (rra.c)        */
+/* ------------------------------------------------------------------ */
+
+/* rra.c:924-975 pivotedQR_mkl: dgeqp3 on a copy of M (every column free: the
+ * reference hands dgeqp3 an uninitialised pivot array, whose intended and
+ * usual -- zero-filled -- content means exactly that), R = the upper
+ * trapezoid of the first min(m, n) rows, Q = dorgqr, I = 0-based pivots. */
+static void pivoted_qr(mat *M, mat **Q, mat **R, int *I) {
+    int m = M->nrows, n = M->ncols, k = OMIN(m, n);
+    mat *W = oref_matrix_new(m, n);
+    matrix_copy(W, M);
+    int *jpvt = (int *)calloc((size_t)n + 1, sizeof(int));
+    double *tau = (double *)calloc((size_t)k + 1, sizeof(double));
+    int lwork = -1, info = 0, lda = OMAX(1, m);
+    double wq;
+    p_dgeqp3(&m, &n, W->d, &lda, jpvt, tau, &wq, &lwork, &info);
+    lwork = (int)wq + 1;
+    double *work = (double *)malloc(sizeof(double) * (size_t)lwork);
+    p_dgeqp3(&m, &n, W->d, &lda, jpvt, tau, work, &lwork, &info);
+    free(work);
+    int Rr = k, Rc = (m <= n) ? n : k, Qr = (m <= n) ? k : m, Qc = k;
+    *R = oref_matrix_new(Rr, Rc);
+    for (int j = 0; j < Rc; ++j)
+        for (int i = 0; i <= OMIN(j, Rr - 1); ++i) MEL(*R, i, j) = MEL(W, i, j);
+    if (Q) {
+        orgqr(Qr, Qc, k, W->d, lda, tau);
+        *Q = oref_matrix_new(Qr, Qc);
+        for (int j = 0; j < Qc; ++j)
+            for (int i = 0; i < Qr; ++i) MEL(*Q, i, j) = MEL(W, i, j);
+    }
+    for (int i = 0; i < n; ++i) I[i] = jpvt[i] - 1;
+    free(jpvt); free(tau);
+    oref_matrix_delete(W);
+}
+
+/* mvf.c:1477-1494 upper_triangular_system_solve, solve_type 1: X = A^-1 B by
+ * dtrsm (left, upper, no transpose, non-unit). */
+static void trsm_upper(mat *A, mat *B, mat *X) {
+    const double one = 1.0;
+    int m = B->nrows, n = B->ncols, lda = OMAX(1, A->nrows), ldb = OMAX(1, B->nrows);
+    memcpy(X->d, B->d, (size_t)m * n * sizeof(double));
+    if (m && n) p_dtrsm("L", "U", "N", "N", &m, &n, &one, A->d, &lda, X->d, &ldb, 1, 1, 1, 1);
+}
+
+/* T = Rk1^-1 Rk2 from the first f rows of R: Rk1 = R(0:f, 0:f) made upper
+ * triangular, Rk2 = R(0:f, f:n) (rra.c:2005-2020, mvf.c:959-1010, 773-784). */
+static mat *id_coefficients(mat *R, int f) {
+    int n = R->ncols;
+    mat *Rk1 = oref_matrix_new(f, f), *Rk2 = oref_matrix_new(f, n - f);
+    for (int j = 0; j < f; ++j)
+        for (int i = 0; i <= j; ++i) MEL(Rk1, i, j) = MEL(R, i, j);
+    for (int j = f; j < n; ++j)
+        for (int i = 0; i < f; ++i) MEL(Rk2, i, j - f) = MEL(R, i, j);
+    mat *T = oref_matrix_new(f, n - f);
+    trsm_upper(Rk1, Rk2, T);
+    oref_matrix_delete(Rk1); oref_matrix_delete(Rk2);
+    return T;
+}
+
+/* rra.c:1969-2027 id_blockrand_decomp_fixed_rank_or_prec: the QB of
+ * randQB_pb_new with nstep = (k + p) / kstep (integer division, :1974; 0 in
+ * TOL mode), the full pivoted QR of B, frank = k (rank mode) or the shrunk
+ * round(frank / (frank + p + 1e-6) * frank) (TOL mode, :1998-2003), T from
+ * the first frank rows of R.  I: n 0-based pivots. */
+int oref_id_blockrand(mat *M, int k, int p, double TOL, int kstep, int q, int s, int *frank,
+                      int *I, mat **T) {
+    if (!g_backend) return -2;
+    if (kstep <= 0) return -1;
+    int nstep = (int)ceil((double)((k + p) / kstep)), rankMode = 1;
+    if (k <= 0) { rankMode = 0; nstep = 0; }
+    mat *Q = NULL, *B = NULL, *R = NULL;
+    int rc = oref_randQB_pb_new(M, kstep, nstep, TOL, q, s, frank, &Q, &B);
+    if (rc) return rc;
+    pivoted_qr(B, NULL, &R, I);
+    if (rankMode) *frank = k;
+    else *frank = (int)round((double)(*frank / (*frank + p + 1e-6)) * (*frank));
+    int f = *frank;
+    if (f <= 0 || f > R->nrows || f > R->ncols) {
+        oref_matrix_delete(Q); oref_matrix_delete(B); oref_matrix_delete(R);
+        return -4;
+    }
+    *T = id_coefficients(R, f);
+    oref_matrix_delete(Q); oref_matrix_delete(B); oref_matrix_delete(R);
+    return 0;
+}
+
+/* rra.c:1807-1853 id_decomp_fixed_rank_or_prec on its full-rank branch
+ * (k >= min(m, n), the case the two-sided ID uses): the full pivoted QR of M,
+ * T = Rk1^-1 Rk2 from the first k rows. */
+int oref_id_full(mat *M, int k, int *I, mat **T) {
+    if (!g_backend) return -2;
+    mat *R = NULL;
+    pivoted_qr(M, NULL, &R, I);
+    if (k <= 0 || k > R->nrows) { oref_matrix_delete(R); return -4; }
+    *T = id_coefficients(R, k);
+    oref_matrix_delete(R);
+    return 0;
+}
+
+/* rra.c:2086-2112 id_two_sided_blockrand_decomp_fixed_rank_or_prec: the
+ * column ID above, then a full ID of MI^T = M(:, Icol(1:frank))^T (frank x m),
+ * whose pivots are the row indices Irow (m of them) and coefficients S. */
+int oref_id_two_sided_blockrand(mat *M, int k, int p, double TOL, int kstep, int q, int s,
+                                int *frank, int *Icol, int *Irow, mat **T, mat **S) {
+    int rc = oref_id_blockrand(M, k, p, TOL, kstep, q, s, frank, Icol, T);
+    if (rc) return rc;
+    int m = M->nrows, f = *frank;
+    mat *MIt = oref_matrix_new(f, m);
+    for (int j = 0; j < f; ++j)
+        for (int i = 0; i < m; ++i) MEL(MIt, j, i) = MEL(M, i, Icol[j]);
+    rc = oref_id_full(MIt, f, Irow, S);
+    oref_matrix_delete(MIt);
+    if (rc) oref_matrix_delete(*T);
+    return rc;
+}
+
+/* rra.c:2262-2332 cur_blockrand_decomp_fixed_rank_or_prec: the two-sided ID;
+ * V = [I_k; T^T](Icol^-1, :) (n x k), R = M(Irow(1:k), :), C = M(:, Icol(1:k)),
+ * U^T the solution of (R R^T) U^T = R V by dgesv (mvf.c:1525-1531). */
+int oref_cur_blockrand(mat *M, int k, int p, double TOL, int kstep, int q, int s, int *frank,
+                       mat **Cm, mat **Um, mat **Rm) {
+    int m = M->nrows, n = M->ncols;
+    int *Icol = (int *)malloc(sizeof(int) * (size_t)n), *Irow = (int *)malloc(sizeof(int) * (size_t)m);
+    mat *T = NULL, *S = NULL;
+    int rc = oref_id_two_sided_blockrand(M, k, p, TOL, kstep, q, s, frank, Icol, Irow, &T, &S);
+    if (rc) { free(Icol); free(Irow); return rc; }
+    k = *frank;
+    int *Icolinv = (int *)malloc(sizeof(int) * (size_t)n);
+    for (int i = 0; i < n; ++i) Icolinv[Icol[i]] = i;           /* mvf.c:1195-1201 */
+    mat *V = oref_matrix_new(n, k);                              /* V1(Icolinv, :) */
+    for (int i = 0; i < n; ++i) {
+        int r = Icolinv[i];
+        for (int j = 0; j < k; ++j) MEL(V, i, j) = (r < k) ? (r == j ? 1.0 : 0.0) : MEL(T, j, r - k);
+    }
+    *Rm = oref_matrix_new(k, n);
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < k; ++i) MEL(*Rm, i, j) = MEL(M, Irow[i], j);
+    *Cm = oref_matrix_new(m, k);
+    for (int j = 0; j < k; ++j)
+        memcpy(&MEL(*Cm, 0, j), &MEL(M, 0, Icol[j]), sizeof(double) * (size_t)m);
+    mat *RRt = oref_matrix_new(k, k), *RV = oref_matrix_new(k, k);
+    matrix_matrix_transpose_mult(*Rm, *Rm, RRt);
+    matrix_matrix_mult(*Rm, V, RV);
+    int *ipiv = (int *)malloc(sizeof(int) * (size_t)(k + 1)), info = 0, kk = k;
+    p_dgesv(&kk, &kk, RRt->d, &kk, ipiv, RV->d, &kk, &info);    /* RV <- Ut */
+    *Um = oref_matrix_new(k, k);
+    matrix_build_transpose(*Um, RV);
+    free(ipiv); free(Icol); free(Irow); free(Icolinv);
+    oref_matrix_delete(V); oref_matrix_delete(RRt); oref_matrix_delete(RV);
+    oref_matrix_delete(T); oref_matrix_delete(S);
+    return 0;
+}
+
+/* rra.c:924-975 exposed for the tests: R (min(m,n) x n for m <= n) and pivots. */
+int oref_pivoted_qr(mat *M, mat **Q, mat **R, int *I) {
+    if (!g_backend) return -2;
+    pivoted_qr(M, Q, R, I);
     return 0;
 }
 

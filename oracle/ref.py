@@ -93,6 +93,22 @@ def load(nthreads: int | None = None) -> C.CDLL:
         lib.oref_svd_residual.restype = C.c_double
         lib.oref_svd_residual.argtypes = [MatP, MatP, MatP, MatP]
         lib.oref_wall_seconds.restype = C.c_double
+        IP = C.POINTER(C.c_int)
+        lib.oref_pivoted_qr.restype = C.c_int
+        lib.oref_pivoted_qr.argtypes = [MatP, C.POINTER(MatP), C.POINTER(MatP), IP]
+        lib.oref_id_blockrand.restype = C.c_int
+        lib.oref_id_blockrand.argtypes = [MatP, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int,
+                                          C.c_int, IP, IP, C.POINTER(MatP)]
+        lib.oref_id_full.restype = C.c_int
+        lib.oref_id_full.argtypes = [MatP, C.c_int, IP, C.POINTER(MatP)]
+        lib.oref_id_two_sided_blockrand.restype = C.c_int
+        lib.oref_id_two_sided_blockrand.argtypes = [MatP, C.c_int, C.c_int, C.c_double, C.c_int,
+                                                    C.c_int, C.c_int, IP, IP, IP,
+                                                    C.POINTER(MatP), C.POINTER(MatP)]
+        lib.oref_cur_blockrand.restype = C.c_int
+        lib.oref_cur_blockrand.argtypes = [MatP, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int,
+                                           C.c_int, IP, C.POINTER(MatP), C.POINTER(MatP),
+                                           C.POINTER(MatP)]
         lib.oref32_matrix_new.restype = Mat32P
         lib.oref32_matrix_new.argtypes = [C.c_int64, C.c_int64]
         lib.oref32_matrix_delete.argtypes = [Mat32P]
@@ -234,6 +250,88 @@ def blockrand(M, k, p, TOL, vnum, kstep, q, s, seed=777, omega=None):
     if rc:
         raise ValueError(f"oref_blockrand rc={rc}")
     return _from_mat(U), _from_mat(S), _from_mat(V), fr.value, l.value
+
+
+def _ints(n):
+    return np.zeros(max(n, 1), dtype=np.int32)
+
+
+def _iptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def pivoted_qr(M):
+    """Oracle of pivotedQR_mkl: (Q, R, I) with M[:, I] = Q R, I 0-based."""
+    lib = load()
+    Mm = _to_mat(M)
+    Q, R = MatP(), MatP()
+    I = _ints(M.shape[1])
+    rc = lib.oref_pivoted_qr(Mm, C.byref(Q), C.byref(R), _iptr(I))
+    lib.oref_matrix_delete(Mm)
+    if rc:
+        raise ValueError(f"oref_pivoted_qr rc={rc}")
+    return _from_mat(Q), _from_mat(R), I[:M.shape[1]].astype(np.int64)
+
+
+def id_blockrand(M, k, p, TOL, kstep, q, s, seed=777, omega=None):
+    """Oracle of id_blockrand_decomp_fixed_rank_or_prec: (I, T, frank)."""
+    lib = load()
+    Mm = _to_mat(M)
+    T = MatP()
+    fr = C.c_int(0)
+    I = _ints(M.shape[1])
+    with _Omega(seed, omega):
+        rc = lib.oref_id_blockrand(Mm, k, p, TOL, kstep, q, s, C.byref(fr), _iptr(I), C.byref(T))
+    lib.oref_matrix_delete(Mm)
+    if rc:
+        raise ValueError(f"oref_id_blockrand rc={rc}")
+    return I[:M.shape[1]].astype(np.int64), _from_mat(T), fr.value
+
+
+def id_full(M, k):
+    """Oracle of id_decomp_fixed_rank_or_prec for k >= min(m, n): (I, T)."""
+    lib = load()
+    Mm = _to_mat(M)
+    T = MatP()
+    I = _ints(M.shape[1])
+    rc = lib.oref_id_full(Mm, k, _iptr(I), C.byref(T))
+    lib.oref_matrix_delete(Mm)
+    if rc:
+        raise ValueError(f"oref_id_full rc={rc}")
+    return I[:M.shape[1]].astype(np.int64), _from_mat(T)
+
+
+def id_two_sided_blockrand(M, k, p, TOL, kstep, q, s, seed=777, omega=None):
+    """Oracle of id_two_sided_blockrand_decomp_fixed_rank_or_prec:
+    (Icol, Irow, T, S, frank)."""
+    lib = load()
+    Mm = _to_mat(M)
+    T, S = MatP(), MatP()
+    fr = C.c_int(0)
+    Ic, Ir = _ints(M.shape[1]), _ints(M.shape[0])
+    with _Omega(seed, omega):
+        rc = lib.oref_id_two_sided_blockrand(Mm, k, p, TOL, kstep, q, s, C.byref(fr), _iptr(Ic),
+                                             _iptr(Ir), C.byref(T), C.byref(S))
+    lib.oref_matrix_delete(Mm)
+    if rc:
+        raise ValueError(f"oref_id_two_sided_blockrand rc={rc}")
+    return (Ic[:M.shape[1]].astype(np.int64), Ir[:M.shape[0]].astype(np.int64), _from_mat(T),
+            _from_mat(S), fr.value)
+
+
+def cur_blockrand(M, k, p, TOL, kstep, q, s, seed=777, omega=None):
+    """Oracle of cur_blockrand_decomp_fixed_rank_or_prec: (C, U, R, frank)."""
+    lib = load()
+    Mm = _to_mat(M)
+    Cm, Um, Rm = MatP(), MatP(), MatP()
+    fr = C.c_int(0)
+    with _Omega(seed, omega):
+        rc = lib.oref_cur_blockrand(Mm, k, p, TOL, kstep, q, s, C.byref(fr), C.byref(Cm),
+                                    C.byref(Um), C.byref(Rm))
+    lib.oref_matrix_delete(Mm)
+    if rc:
+        raise ValueError(f"oref_cur_blockrand rc={rc}")
+    return _from_mat(Cm), _from_mat(Um), _from_mat(Rm), fr.value
 
 
 def wall() -> float:

@@ -2,8 +2,8 @@
 
 * Fixed rank, 65536 x 16384 float32 A = U diag(sigma) V^T + N(0, 1e-4^2)
   noise, rank-2048 Haar factors, sigma_j = 1 / (j + 1); k = 512, p = 32, q = 2,
-  s = 1, vnum = 2; split-bf16 passes (rsvd_dev_fixed_rank_mp precision 2)
-  against the FP64 plain path (rsvd_dev_fixed_rank) on the same A and Omega
+  s = 1, vnum = 2; split-bf16 passes (rsvd_dev_fixed_rank_mp precision 2) and
+  fp16 passes (precision 6) against the FP64 plain path (rsvd_dev_fixed_rank) on the same A and Omega
   (reference algorithm: MKL/rank_revealing_algorithms_intel_mkl.c:73-234).
   Checks: every returned sigma within 1e-5 relative of the FP64 path's; the
   explicit FP64 residual ||A - U S V^T||_F within 1e-5 relative of the FP64
@@ -43,7 +43,9 @@ def _gram_dev(A_host64):
     return float(np.abs(G - np.eye(G.shape[0])).max())
 
 
-def test_fixed_rank_65536x16384_split_bf16_vs_fp64(gpu_lib):
+@pytest.mark.parametrize("prec", [2, 6])
+def test_fixed_rank_65536x16384_vs_fp64(gpu_lib, prec):
+    """prec 2: split-bf16 passes; 6: fp16 passes (two products)."""
     rp = gpu_lib
     lib = rp.load()
     m, n, r, k, p, q = 65536, 16384, 2048, 512, 32, 2
@@ -58,11 +60,12 @@ def test_fixed_rank_65536x16384_split_bf16_vs_fp64(gpu_lib):
                                    S64.ptr, V64.ptr, n) == 0, rp.last_error()
     Um, Sm, Vm = (rp.DeviceMatrix(m, k, np.float32), rp.DeviceMatrix(k, 1),
                   rp.DeviceMatrix(n, k, np.float32))
-    assert lib.rsvd_dev_fixed_rank_mp(A32.ptr, m, n, m, k, p, 2, q, 1, seed, None, 2, Um.ptr, m,
-                                      Sm.ptr, Vm.ptr, n) == 0, rp.last_error()
+    assert lib.rsvd_dev_fixed_rank_mp(A32.ptr, m, n, m, k, p, 2, q, 1, seed, None, prec, Um.ptr,
+                                      m, Sm.ptr, Vm.ptr, n) == 0, rp.last_error()
     s64, smp = S64.to_host()[:, 0], Sm.to_host()[:, 0]
     assert np.all(np.diff(s64) >= 0) and np.all(np.diff(smp) >= 0)  # vnum = 2: ascending
     rel = np.abs(smp - s64) / s64
+    print(f"precision {prec}: sigma rel err max {rel.max():.2e}")
     assert rel.max() <= 1e-5, rel.max()
 
     res64 = _resid(lib, rp, A64, m, n, k, U64.ptr, m, S64.ptr, V64.ptr, n, 0)

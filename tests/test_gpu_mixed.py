@@ -113,13 +113,84 @@ def test_gemm_fp8_matches_emulation(gpu_lib, ta, M, N, K):
     assert np.linalg.norm(got - exact) / np.linalg.norm(exact) < 0.1
 
 
+def emulate_fp16(A: np.ndarray, B: np.ndarray, ta: bool, K32: int):
+    """op(A)(:, :K32) @ B(:K32) through the library's fp16 scheme: a_c =
+    max |A chunk| / 32768 per 128 columns of A, A * (1 / a_c) rounded to fp16;
+    S' = B (times a_c along k for op(A) = A), b_n = max_k |S'(k, n)| / 32768,
+    hi = fp16(S' / b_n), lo = fp16(S' / b_n - hi); exact products hi A + lo A,
+    then b_n and (op(A) = A^T) the row's a_c.  float32 arithmetic for the
+    scales and conversions, as on the device."""
+    f = np.float32
+    nc = (A.shape[1] + 127) // 128
+    a = np.ones(nc, dtype=f)
+    for c in range(nc):
+        mx = f(np.abs(A[:, 128 * c:128 * (c + 1)]).max())
+        if mx > 0:
+            a[c] = mx / f(32768.0)
+    inv = (f(1.0) / a).astype(f)
+    opA = (A.T if ta else A)[:, :K32]
+    M = opA.shape[0]
+    if ta:   # chunks along the rows of op(A)
+        Aq = (opA * inv[np.arange(M) // 128][:, None]).astype(f).astype(np.float16)
+        Sp = B[:K32].astype(f)
+    else:    # chunks along K, folded into the skinny operand
+        Aq = (opA * inv[np.arange(K32) // 128][None, :]).astype(f).astype(np.float16)
+        Sp = (B[:K32].astype(f) * a[np.arange(K32) // 128][:, None]).astype(f)
+    b = np.ones(Sp.shape[1], dtype=f)
+    mx = np.abs(Sp).max(axis=0).astype(f)
+    b[mx > 0] = mx[mx > 0] / f(32768.0)
+    v = (Sp * (f(1.0) / b)[None, :]).astype(f)
+    hi = v.astype(np.float16)
+    lo = (v - hi.astype(f)).astype(np.float16)
+    P = Aq.astype(np.float64) @ (hi.astype(np.float64) + lo.astype(np.float64))
+    P *= b[None, :].astype(np.float64)
+    if ta:
+        P *= a[np.arange(M) // 128][:, None].astype(np.float64)
+    mag = np.abs(Aq.astype(np.float64)) @ np.abs(v.astype(np.float64)) * b[None, :]
+    return P, mag
+
+
 @pytest.mark.gpu
-@pytest.mark.parametrize("prec,bound", [(2, 1e-5), (1, 2e-3), (3, 0.1)])
+@pytest.mark.parametrize("ta,M,N,K", [("N", 65536, 288, 4096), ("T", 32768, 288, 8200),
+                                      ("N", 1000, 50, 300)])
+def test_gemm_fp16_matches_emulation(gpu_lib, ta, M, N, K):
+    """rsvd_dev_gemm dtype 6 on the streaming kernel's fp16 path (the first two
+    shapes fill the GPU; K = 8200 leaves a remainder of 8 for split-bf16) and
+    on the split-bf16 fallback for a small shape."""
+    rng = np.random.default_rng(M + N + K)
+    A = rng.standard_normal((K, M) if ta == "T" else (M, K)).astype(np.float32)
+    A *= np.exp(rng.uniform(-3, 3, A.shape[1])).astype(np.float32)[None, :]
+    B = rng.standard_normal((K, N)).astype(np.float32)
+    dA, dB = gpu_lib.DeviceMatrix.from_host(A), gpu_lib.DeviceMatrix.from_host(B)
+    dC = gpu_lib.DeviceMatrix(M, N, np.float32)
+    lib = gpu_lib.load()
+    rc = lib.rsvd_dev_gemm(6, ta.encode(), b"N", M, N, K, 1.0, dA.ptr, dA.rows, dB.ptr, dB.rows,
+                           0.0, dC.ptr, M)
+    assert rc == 0, gpu_lib.last_error()
+    got = dC.to_host().astype(np.float64)
+    opA = (A.T if ta == "T" else A).astype(np.float64)
+    exact = opA @ B.astype(np.float64)
+    if M >= 32768:
+        K32 = K - K % 32
+        want, mag = emulate_fp16(A, B, ta == "T", K32)
+        want += opA[:, K32:] @ B[K32:].astype(np.float64)
+        # the same quantisation; fp32 accumulation of exact products
+        assert np.abs(got - want).max() <= 1e-5 * mag.max()
+        # and the format's own error: A's fp16 rounding, 2^-12 relative
+        assert np.linalg.norm(got - exact) / np.linalg.norm(exact) < 1e-3
+    else:
+        assert np.linalg.norm(got - exact) / np.linalg.norm(exact) < 1e-5
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec,bound", [(2, 1e-5), (1, 2e-3), (3, 0.1), (6, 5e-5)])
 def test_fixed_rank_mp_against_fp64(gpu_lib, prec, bound):
     """8192 x 2048, sigma_j = 1/(j+1) for j < 512 plus 1e-6 noise, k = 128, p
     = 16, q = 2: the k singular values against the FP64 plain path on the same
     Omega (relative, value by value), orthonormality of the fp32 factors, and
-    the residual ||A - U S V^T||_F against the FP64 one."""
+    the residual ||A - U S V^T||_F against the FP64 one.  fp16 passes (6) keep
+    A's fp16 rounding (2^-12): ~3e-5 here, where the spectrum has almost no
+    noise floor, ~3e-6 on the 65536 x 16384 workload."""
     rp = gpu_lib
     lib = rp.load()
     m, n, r, k, p, q, seed = 8192, 2048, 512, 128, 16, 2, 4242
@@ -162,9 +233,11 @@ def test_fixed_rank_mp_near_degenerate_cluster(gpu_lib):
     """A cluster of 24 nearly equal singular values (sigma = 1 + 1e-6 j, relative
     gaps 1e-6) above a decaying tail, 4000 x 1500, k = 64, p = 16, q = 2.  The
     eigensolver's early stop for the mixed path must not leave rotations
-    inside the cluster unfinished: the returned sigma agree with the FP64 path
-    to 1e-6 and the cluster's columns of V (V = B^T Uhat Sigma^-1, sensitive to
-    an unconverged Uhat) and U are orthonormal to fp32 level."""
+    inside the cluster unfinished (they would show as a loss of orthogonality
+    of V = B^T Uhat Sigma^-1 up to the cluster width, ~5e-5): the returned
+    sigma agree with the FP64 path to 1e-6 and the cluster's columns of V and U
+    are orthonormal to 1e-5, the level the split-bf16 U and V products leave
+    (16-bit operand splits, ~4e-6 measured)."""
     rp = gpu_lib
     lib = rp.load()
     m, n, k, p, q, seed = 4000, 1500, 64, 16, 2, 99
@@ -192,7 +265,7 @@ def test_fixed_rank_mp_near_degenerate_cluster(gpu_lib):
     ou = np.abs(U.T @ U - np.eye(24)).max()
     print(f"cluster: sigma diff {np.abs(sm - s64).max():.2e}, V^T V - I {ov:.2e}, "
           f"U^T U - I {ou:.2e}")
-    assert ov <= 2e-6 and ou <= 2e-6, (ov, ou)
+    assert ov <= 1e-5 and ou <= 1e-5, (ov, ou)
 
 
 def _blockrand_case(rp, m, n, seed):
