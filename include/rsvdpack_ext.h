@@ -242,27 +242,33 @@ int rsvd_dev_mlp_grad(const float *dTheta, const float *dX, const int *dq, const
 int rsvd_dev_mlp_train(float *dTheta, const float *dX, const int *dq, const float *dr, int n,
                        float lam, int epochs, int batch, float lr, unsigned long long seed,
                        float *loss_hist);
-/* The 64 features of a float matrix (m, n >= 64): singular values s_j of a
- * 64-column bf16 sketch Y = A Omega (seeded stream), x_0 = log(||Y||_F / (8 s_0)),
- * x_j = log(s_j / s_0). */
-int rsvd_dev_sketch_features(const float *dA, long long m, long long n, long long lda,
-                             unsigned long long seed, float *dFeat);
+/* The 64 features of a matrix (m, n >= 64; dA FP64, dA32 its float copy,
+ * same lda): Y = A32 Omega (64 seeded Gaussian columns, split-bf16 products),
+ * Q its orthonormal basis, lam_0 >= ... >= lam_63 the squared singular values
+ * of B = Q^T A in FP64, a2 = ||A||_F^2; x_j = log10(t_j / a2) / 8 with
+ * t_j = a2 - (lam_0 + ... + lam_j) (floored at 1e-16 a2): the energy the
+ * best rank-(j + 1) approximation inside the sketch leaves. */
+int rsvd_dev_sketch_features(const double *dA, const float *dA32, long long m, long long n,
+                             long long lda, unsigned long long seed, float *dFeat);
 /* TOL-driven low-rank approximation with the learned front end: predicts
- * (q, rank) from the sketch features of dA32 (the float copy of dA), runs one
+ * (q, rank) from the sketch features of dA (dA32 its float copy), runs one
  * FP64 fixed-rank rSVD of dA (vnum = 2, oversampling p, q_pred + 1, i.e. q_pred
- * power iterations, k = ceil(rank (1 + margin)) + 2), and accepts it when its
+ * power iterations, k = ceil(rank (1 + margin)) + 2, rank = min(m, n)^o_5 from
+ * the network's output o_5 clamped to [0, 1]), and accepts it when its
  * FP64 residual ||A - U S V^T||_F = sqrt(||A||_F^2 - sum sigma^2) is at most
  * tol_rel ||A||_F.  A predicted k at the sketch limit (k + p >= min(m, n)) is a
  * prediction of full rank; for m >= n it is solved as one: k = n, p = 0,
  * Omega = I (an exact thin QR of A and the small eigenproblem), its residual
- * formed explicitly.  A result above TOL falls back to
+ * formed explicitly.  A result above TOL is retried once with
+ * k = ceil(1.5 k) + 8 (same rules); a second miss falls back to
  * low_rank_svd_blockrand_decomp_fixed_rank_or_prec in TOL mode (TOL = tol_rel
  * ||A||_F, kstep, q_fallback) and returns that method's result; when that too
  * misses TOL (TOL not reachable below full rank) and m >= n, the exact
  * full-rank decomposition (k = n, Omega = I) is returned instead, so no result
  * exceeds TOL.  Outputs as rsvd_dev_blockrand; *q_pred, *k_pred the
- * prediction, *fell_back 1 when the adaptive fallback's result is returned, 2
- * when the exact one is, *resid_rel the relative residual of the result. */
+ * prediction, *fell_back 0 when the first attempt is returned, 3 when the
+ * retry is, 1 when the adaptive fallback's result is, 2 when the exact one
+ * is; *resid_rel the relative residual of the result. */
 int rsvd_dev_learned_lowrank(const float *dTheta, const double *dA, const float *dA32,
                              long long m, long long n, long long lda, double tol_rel, int p,
                              double margin, int kstep, int q_fallback, unsigned long long seed,

@@ -1193,11 +1193,12 @@ int rsvd_dev_mlp_train(float* dTheta, const float* dX, const int* dq, const floa
   return stream_sync(api.c);
 }
 
-int rsvd_dev_sketch_features(const float* dA, long long m, long long n, long long lda,
-                             unsigned long long seed, float* dFeat) {
+int rsvd_dev_sketch_features(const double* dA, const float* dA32, long long m, long long n,
+                             long long lda, unsigned long long seed, float* dFeat) {
   DevApi api;
   if (api.rc) return api.rc;
-  RSVD_TRY(sketch_features(api.c, dA, m, n, lda, seed, dFeat));
+  if (!dA || !dA32 || !dFeat) return set_error("rsvd_dev_sketch_features: NULL argument");
+  RSVD_TRY(sketch_features(api.c, dA, dA32, m, n, lda, seed, dFeat));
   return stream_sync(api.c);
 }
 
@@ -1220,7 +1221,7 @@ int rsvd_dev_learned_lowrank(const float* dTheta, const double* dA, const float*
   DBuf<float> feat, out;
   RSVD_TRY(feat.alloc(64));
   RSVD_TRY(out.alloc(6));
-  RSVD_TRY(sketch_features(c, dA32, m, n, lda, seed ^ 0x5eed5eedULL, feat.p));
+  RSVD_TRY(sketch_features(c, dA, dA32, m, n, lda, seed ^ 0x5eed5eedULL, feat.p));
   RSVD_TRY(mlp_forward(c, dTheta, feat.p, 1, out.p));
   float o[6];
   RSVD_HIP(hipMemcpyAsync(o, out.p, sizeof o, hipMemcpyDeviceToHost, c->stream));
@@ -1233,40 +1234,44 @@ int rsvd_dev_learned_lowrank(const float* dTheta, const double* dA, const float*
   int q = 0;
   for (int i = 1; i < 5; ++i)
     if (o[i] > o[q]) q = i;
+  // output 5 predicts log(rank) / log(min(m, n))
   double rk = std::isfinite(o[5]) ? o[5] : 1.0;
-  long long k = (long long)ceil(rk * (double)mn * (1.0 + margin)) + 2;
-  if (k < 1) k = 1;
-  // A predicted rank at the sketch limit (k + p >= min(m, n)) is a prediction
-  // of full rank: for m >= n it is solved as such -- k = n, no oversampling,
-  // Omega = I, so Y = A and the pipeline is an exact thin QR of A followed by
-  // the small eigenproblem (q is moot: Y already spans range(A)).  Below the
-  // limit: one fixed-rank rSVD with the predicted q and k.
-  const bool full = k >= mn - p && m >= n;
-  if (full) k = mn;
-  else if (k > mn - p) k = mn - p;
-  const int pp = full ? 0 : p, qq = full ? 1 : q + 1;
+  rk = rk < 0.0 ? 0.0 : (rk > 1.0 ? 1.0 : rk);
+  long long k0 = (long long)ceil(exp(rk * log((double)mn)) * (1.0 + margin)) + 2;
+  if (k0 < 1) k0 = 1;
   if (q_pred) *q_pred = q;
-  if (k_pred) *k_pred = (int)k;
-  const bool try_fixed = full || k < mn - p;
-  // one fixed-rank rSVD, judged by its FP64 residual
-  double *u = nullptr, *sv = nullptr, *v = nullptr;
   OmegaSrc om;
   om.seed = seed;
   DBuf<double> eye;
-  if (full) {
-    RSVD_TRY(eye.alloc((size_t)(n * n)));
-    RSVD_TRY(fill2d_f64(c, n, n, 0.0, eye.p, n));
-    RSVD_TRY(fill2d_f64(c, 1, n, 1.0, eye.p, n + 1));  // the diagonal
-    om.dev = eye.p;
-    om.ld = n;
-  }
-  if (try_fixed) {
+  // Up to two fixed-rank attempts, each judged by its FP64 residual: the
+  // predicted k, then (if that misses TOL) ceil(1.5 k) + 8 -- an under-
+  // predicted rank costs one more fixed-rank solve instead of the whole
+  // adaptive method.  A k at the sketch limit (k + p >= min(m, n)) is a
+  // prediction of full rank: for m >= n it is solved as such -- k = n, no
+  // oversampling, Omega = I, so Y = A and the pipeline is an exact thin QR of
+  // A followed by the small eigenproblem (q is moot: Y already spans
+  // range(A)).
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    long long k = attempt == 0 ? k0 : (long long)ceil(1.5 * (double)k0) + 8;
+    const bool full = k >= mn - p && m >= n;
+    if (full) k = mn;
+    else if (k > mn - p) k = mn - p;
+    const int pp = full ? 0 : p, qq = full ? 1 : q + 1;
+    if (k_pred && attempt == 0) *k_pred = (int)k;
+    if (!(full || k < mn - p)) break;  // m < n at the sketch limit
+    if (full && !eye.p) {
+      RSVD_TRY(eye.alloc((size_t)(n * n)));
+      RSVD_TRY(fill2d_f64(c, n, n, 0.0, eye.p, n));
+      RSVD_TRY(fill2d_f64(c, 1, n, 1.0, eye.p, n + 1));  // the diagonal
+    }
+    om.dev = full ? eye.p : nullptr;
+    om.ld = full ? n : 0;
     OutBufs ob;
     RSVD_TRY(ob.alloc(sizeof(double) * (size_t)(m * k), sizeof(double) * (size_t)k,
                       sizeof(double) * (size_t)(n * k)));
-    u = (double*)ob.p[0];
-    sv = (double*)ob.p[1];
-    v = (double*)ob.p[2];
+    double* u = (double*)ob.p[0];
+    double* sv = (double*)ob.p[1];
+    double* v = (double*)ob.p[2];
     RSVD_TRY(fixed_rank_f64(c, dA, m, n, lda, (int)k, pp, 2, qq, 1, om, u, m, sv, v, n, nullptr));
     double s2 = 0, res = 0;
     if (full) {
@@ -1281,10 +1286,11 @@ int rsvd_dev_learned_lowrank(const float* dTheta, const double* dA, const float*
     if (res <= tol_rel * sqrt(a2)) {
       *frank = (int)k;
       ob.release(dU, dS, dV);
-      if (fell_back) *fell_back = 0;
+      if (fell_back) *fell_back = attempt == 0 ? 0 : 3;
       if (resid_rel) *resid_rel = res / sqrt(a2);
       return 0;
     }
+    if (full) break;  // an exact decomposition above TOL: rounding; the fallback decides
   }
   // fallback: the adaptive blocked method in TOL mode
   om.dev = nullptr;

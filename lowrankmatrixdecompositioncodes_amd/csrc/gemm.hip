@@ -50,7 +50,7 @@ constexpr int kThreads = 256;
 
 // C[z] = alpha * op(A)[:, kslice z] * op(B)[kslice z, :] + beta * C
 // When sumsq != nullptr, each workgroup writes the sum of squares of the values
-// it stored into sumsq[blockIdx.x + gridDim.x*blockIdx.y] (double).
+// it stored into sumsq[tm + gridDim.x*tn] for its tile (tm, tn) (double).
 template <typename T, bool TA, bool TB, int BM, int BN, int BK>
 __global__ void __launch_bounds__(kThreads)
 gemm_kernel(int64_t M, int64_t N, int64_t K, int64_t kchunk, T alpha, const T* __restrict__ A,
@@ -71,8 +71,28 @@ gemm_kernel(int64_t M, int64_t N, int64_t K, int64_t kchunk, T alpha, const T* _
   const int wave = tid >> 6;
   const int wm0 = (wave & 1) * WM;
   const int wn0 = (wave >> 1) * WN;
-  const int64_t m0 = (int64_t)blockIdx.x * BM;
-  const int64_t n0 = (int64_t)blockIdx.y * BN;
+  // Tile order.  Workgroups are handed to the 8 XCDs round-robin by
+  // dispatch order (b % 8), each XCD with its own L2: XCD x takes a
+  // contiguous run of tiles, walked with the dimension that has fewer tiles
+  // fastest, so the workgroups that run together on one XCD share the large
+  // operand's panel (the n / l column tiles of one row block of A run side by
+  // side and A leaves HBM once instead of once per column tile).
+  int64_t tm, tn;
+  {
+    const int64_t gx = gridDim.x, gy = gridDim.y, nt = gx * gy;
+    const int64_t b = blockIdx.x + gx * (int64_t)blockIdx.y;
+    const int64_t xcd = b & 7, j = b >> 3, q = nt >> 3, r = nt & 7;
+    const int64_t t = xcd * q + (xcd < r ? xcd : r) + j;
+    if (gy <= gx) {
+      tn = t % gy;
+      tm = t / gy;
+    } else {
+      tm = t % gx;
+      tn = t / gx;
+    }
+  }
+  const int64_t m0 = tm * BM;
+  const int64_t n0 = tn * BN;
   // a Gram (sym): tiles wholly below the diagonal are mirrored afterwards
   if (sym && m0 >= n0 + BN) return;
   const int64_t kbeg = (int64_t)blockIdx.z * kchunk;
@@ -179,7 +199,7 @@ gemm_kernel(int64_t M, int64_t N, int64_t K, int64_t kchunk, T alpha, const T* _
     if (tid == 0) {
       double t = 0;
       for (int w = 0; w < kThreads / 64; ++w) t += red[w];
-      sumsq[blockIdx.x + (int64_t)gridDim.x * blockIdx.y] = t;
+      sumsq[tm + (int64_t)gridDim.x * tn] = t;
     }
   }
 }

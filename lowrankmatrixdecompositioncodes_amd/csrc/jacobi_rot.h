@@ -60,7 +60,7 @@ static __device__ __forceinline__ void rotation(double a, double b, double g, do
   const double sa = ldexp(a, -e), sb = ldexp(b, -e), sg = ldexp(g, -e);
   const double ab = fabs(sa * sb), g2 = sg * sg;
   const bool big = ag > flo;
-  const bool rot = big && g2 > kRotTol * kRotTol * ab;
+  const bool rot = big & (g2 > kRotTol * kRotTol * ab);
   const double d = sb - sa;
   const double r2 = rot ? fma(d, d, 4.0 * g2) : 1.0;
   const double r = r2 * rsq_nr(r2);
@@ -69,8 +69,8 @@ static __device__ __forceinline__ void rotation(double a, double b, double g, do
   const double t2g = (d < 0.0) ? -2.0 * sg : 2.0 * sg;
   *cs = rot ? den * q : 1.0;
   *sn = rot ? t2g * q : 0.0;
-  *sig = ((big && g2 > kStopTol * kStopTol * ab) ? 1 : 0) |
-         ((rot && fabs(*sn) > kLooseStopTol) ? 2 : 0);
+  *sig = ((big & (g2 > kStopTol * kStopTol * ab)) ? 1 : 0) |
+         ((rot & (fabs(*sn) > kLooseStopTol)) ? 2 : 0);
 }
 
 }  // namespace rsvd

@@ -1,7 +1,7 @@
 // learned.hip -- the learned (q, rank) front end of the TOL-driven low-rank
 // approximation: a small multilayer perceptron, trained and served by the
 // kernels below, predicts the power-iteration count q and the rank from the
-// spectrum of a cheap 64-column bf16 sketch, so that one fixed-rank rSVD
+// energy profile of a cheap 64-column bf16 sketch, so that one fixed-rank rSVD
 // (rra.c:73-234) can replace the adaptive blocked loop
 // (low_rank_svd_blockrand_decomp_fixed_rank_or_prec, rra.c:239-381).  The
 // adaptive loop stays as the fallback and the judge: a prediction whose FP64
@@ -9,7 +9,7 @@
 //
 // Network (fp32; weights [out][in] row-major, packed W1 b1 W2 b2 W3 b3 W4 b4):
 //   64 -> 128 -> 128 -> 128 (GELU, erf form) -> 6
-// outputs 0..4: logits of q in {0..4}; output 5: rank / min(m, n).
+// outputs 0..4: logits of q in {0..4}; output 5: log(rank) / log(min(m, n)).
 // Loss per sample: logsumexp(o[0:5]) - o[q] + lam (o[5] - r)^2, batch mean.
 //
 // Training: one launch computes forward and backward for chunks of 16
@@ -180,22 +180,21 @@ __global__ void mlp_init_kernel(uint64_t seed, float* __restrict__ theta) {
   theta[p] = scale == 0.0f ? 0.0f : scale * (float)gaussian_at(seed, (uint64_t)p);
 }
 
-// 64 features from the singular values s_0 >= ... >= s_63 of Y = A Omega_64
-// (lambda = eigenvalues of Y^T Y, ascending): x_0 = log(||Y||_F / (8 s_0)),
-// x_j = log(s_j / s_0), floored at log(1e-12).
+// 64 features from the energy a 64-column sketch leaves behind: with lam_0 >=
+// ... >= lam_63 the squared singular values of B = Q^T A (Q an orthonormal
+// basis of the sketch) and a2 = ||A||_F^2, x_j = log10(t_j / a2) / 8 with
+// t_j = a2 - (lam_0 + ... + lam_j) the squared residual of the best rank
+// j + 1 approximation inside range(Q), floored at 1e-16 a2 (x_j in [-2, 0]).
 __global__ void sketch_features_kernel(const double* __restrict__ lam, int k,
+                                       const double* __restrict__ a2p,
                                        float* __restrict__ feat) {
   const int j = threadIdx.x;
   if (j >= k) return;
-  double tot = 0.0;
-  for (int i = 0; i < k; ++i) tot += fmax(lam[i], 0.0);
-  const double s0 = sqrt(fmax(lam[k - 1], 1e-300));
-  if (j == 0) {
-    feat[0] = (float)log(sqrt(tot) / (sqrt((double)k) * s0));
-  } else {
-    const double sj = sqrt(fmax(lam[k - 1 - j], 0.0));
-    feat[j] = (float)log(fmax(sj / s0, 1e-12));
-  }
+  const double a2 = *a2p;
+  double cap = 0.0;
+  for (int i = 0; i <= j; ++i) cap += fmax(lam[i], 0.0);
+  const double t = fmax(a2 - cap, 1e-16 * a2);
+  feat[j] = (a2 > 0.0) ? (float)(log10(t / a2) / 8.0) : -2.0f;
 }
 
 }  // namespace
@@ -288,27 +287,37 @@ int mlp_train(Context* c, float* theta, const float* X, const int* q, const floa
 // Features of one fp32 matrix: Y = A Omega (n x 64 Gaussian, seeded stream)
 // on the bf16 matrix cores, Y^T Y in FP64 from fp32 Y, its eigenvalues, 64
 // features into feat (device).
-int sketch_features(Context* c, const float* A, int64_t m, int64_t n, int64_t lda, uint64_t seed,
-                    float* feat) {
+int sketch_features(Context* c, const double* A, const float* A32, int64_t m, int64_t n,
+                    int64_t lda, uint64_t seed, float* feat) {
   const int k = kIn;
   if (m < k || n < k) return set_error("sketch features: need min(m, n) >= %d", k);
   DBuf<float> Om, Y;
-  DBuf<double> Y64, G, lam;
+  DBuf<double> Y64, B, G, lam, a2;
   RSVD_TRY(Om.alloc((size_t)(n * k)));
   RSVD_TRY(Y.alloc((size_t)(m * k)));
   RSVD_TRY(Y64.alloc((size_t)(m * k)));
+  RSVD_TRY(B.alloc((size_t)(k * n)));
   RSVD_TRY(G.alloc((size_t)(k * k)));
   RSVD_TRY(lam.alloc((size_t)k));
+  RSVD_TRY(a2.alloc(1));
+  // the sketch on the bf16 matrix cores (split-bf16 products: the basis is
+  // that of the float matrix to ~1e-7), its orthonormal basis in FP64, and
+  // the captured energy of the FP64 matrix (B = Q^T A, G = B B^T)
   RSVD_TRY(gaussian_f32(c, seed, n, k, 0, n, Om.p, n));
-  RSVD_TRY(gemm_bf16(c, "sketch_bf16", 'N', 'N', m, k, n, 1.0f, A, lda, Om.p, n, 0.0f, Y.p, m, 1));
+  RSVD_TRY(gemm_bf16(c, "sketch_bf16", 'N', 'N', m, k, n, 1.0f, A32, lda, Om.p, n, 0.0f, Y.p, m,
+                     3));
   RSVD_TRY(f32_to_f64(c, m, k, Y.p, m, Y64.p, m));
-  RSVD_TRY(gemm_f64(c, 'T', 'N', k, k, m, 1.0, Y64.p, m, Y64.p, m, 0.0, G.p, k));
+  int fb = 0;
+  RSVD_TRY(orth_f64(c, m, k, Y64.p, m, nullptr, &fb));
+  RSVD_TRY(gemm_f64(c, 'T', 'N', k, n, m, 1.0, Y64.p, m, A, lda, 0.0, B.p, k));
+  RSVD_TRY(gemm_f64(c, 'N', 'T', k, k, n, 1.0, B.p, k, B.p, k, 0.0, G.p, k));
   RSVD_TRY(symmetrize_upper_f64(c, k, G.p));
   int sw = 0;
-  RSVD_TRY(symeig_jacobi_f64(c, k, G.p, k, nullptr, k, lam.p, 60, &sw));
-  RSVD_TRY(sort_pairs_f64(c, k, lam.p, 0, nullptr, 0, 0, nullptr, 0, 0));
+  RSVD_TRY(symeig_jacobi_f64(c, k, G.p, k, nullptr, k, lam.p, 60, &sw, 1));
+  RSVD_TRY(sort_pairs_f64(c, k, lam.p, 1, nullptr, 0, 0, nullptr, 0, 0));
+  RSVD_TRY(sumsq_f64(c, m, n, A, lda, a2.p));
   hipLaunchKernelGGL(sketch_features_kernel, dim3(1), dim3(64), 0, c->stream,
-                     (const double*)lam.p, k, feat);
+                     (const double*)lam.p, k, (const double*)a2.p, feat);
   RSVD_LAUNCH_CHECK();
   return 0;
 }

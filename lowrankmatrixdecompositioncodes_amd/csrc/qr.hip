@@ -143,9 +143,9 @@ int org2r_f64(Context* c, int64_t m, int64_t n, double* Y, int64_t ldy, const do
 // per 32-row panel k (p = 32 k).  The serial chain is the panel count; every
 // workgroup does O(1) tile work, so no launch has a loop over earlier panels.
 // Each workgroup first factors the panel's diagonal block D = G_kk (already
-// updated by panels < k) in LDS -- redundantly, which is cheaper than another
-// launch -- then takes one role:
-//   trailing (ti <= tj > k): solve R_kk^T R_k,t = G_k,t for t = ti, tj and
+// updated by panels < k) in LDS and inverts it, X_kk = R_kk^-1 --
+// redundantly, which is cheaper than another launch -- then takes one role:
+//   trailing (ti <= tj > k): R_k,t = X_kk^T G_k,t for t = ti, tj and
 //       update G_ti,tj -= R_k,ti^T R_k,tj in place; the ti == tj tile writes
 //       R_k,ti (and, in panel 0, the original diagonal for the pivot test);
 //   accumulate (i <= k, tj > k): X_ik = -Acc_ik X_kk (X_kk when i == k) and
@@ -176,38 +176,90 @@ __device__ __forceinline__ double ci_rcp(double d) {
   return r;
 }
 
-// Unnormalised right-looking Cholesky of Dm (upper triangle valid, kb x kb):
-// step j subtracts U_j,a U_j,c / U_j,j from the rows below, one barrier per
-// step.  On exit Dm holds R_kk (zero below the diagonal and past kb) and
-// sqd / invd its diagonal and reciprocals.  A pivot that is not above
-// kCholRelTol * ref (or is not finite) sets *bad and is replaced (by 1 when
-// it is not positive) so the factorisation runs to the end.
+// Unnormalised right-looking Cholesky of Dm (upper triangle valid, kb x kb),
+// four pivots per barrier.  A step takes the pivot rows P = j .. j + 3: every
+// thread eliminates the 4 x 4 pivot block itself (the same instructions on
+// the same LDS entries in every thread, so all copies agree) and the pivot
+// rows at the columns it needs (its row a and its four columns), then
+// subtracts sum_{w in P} U_wa U_wc / U_ww from its entries of the rows below
+// P.  A pivot that is not above kCholRelTol * ref (or is not finite) sets
+// *bad and is replaced (by 1 when it is not positive) so the factorisation
+// runs to the end.  Rows stay in registers; the rows of the next step are
+// stored before the barrier that ends a step, all rows at the end.  On exit Dm
+// holds R_kk (zero below the diagonal and past kb) and sqd / invd its diagonal
+// and reciprocals.
 __device__ void ci_factor(double (*Dm)[CHP], int kb, const double* ref, double* piv,
                           double* sqd, double* invd, int* bad) {
+  constexpr int PB = 4;
   const int tid = threadIdx.x, a = tid & (CB - 1), c0 = (tid >> 5) * 4;
   double own[4];
 #pragma unroll
   for (int q = 0; q < 4; ++q) own[q] = Dm[a][c0 + q];
-  for (int j = 0; j < kb; ++j) {
-    double d = Dm[j][j];
-    const bool ok = (d > kCholRelTol * ref[j]) && isfinite(d);
-    if (!ok) d = (d > 0.0 && isfinite(d)) ? d : 1.0;
-    if (tid == 0) {
-      if (!ok) *bad = 1;
-      piv[j] = d;
-    }
-    if (a > j && a < kb) {
-      const double u = Dm[j][a] * ci_rcp(d);
+  // All eight steps run on full 4 x 4 blocks: the rows and columns past kb
+  // are zero, their pivots are taken as 1 and change nothing, so the step
+  // body has no bounds on the block.
+  for (int j = 0; j < CB; j += PB) {
+    // S of the pivot block (upper part) and of the pivot rows at column a
+    // (x = 0) and at columns c0 .. c0 + 3 (x = 1 .. 4)
+    double U[PB][PB], X[PB][5], rinv[PB];
 #pragma unroll
-      for (int q = 0; q < 4; ++q)
-        if (c0 + q >= a && c0 + q < kb) own[q] -= u * Dm[j][c0 + q];
-      if (a == j + 1) {
+    for (int u = 0; u < PB; ++u) {
+#pragma unroll
+      for (int v = u; v < PB; ++v) U[u][v] = Dm[j + u][j + v];
+      X[u][0] = Dm[j + u][a];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) X[u][1 + q] = Dm[j + u][c0 + q];
+    }
+    // eliminate inside the pivot block, pivot by pivot
+#pragma unroll
+    for (int u = 0; u < PB; ++u) {
+#pragma unroll
+      for (int w = 0; w < u; ++w) {
+        const double l = U[w][u] * rinv[w];
+#pragma unroll
+        for (int v = u; v < PB; ++v) U[u][v] -= l * U[w][v];
+#pragma unroll
+        for (int x = 0; x < 5; ++x) X[u][x] -= l * X[w][x];
+      }
+      double d = U[u][u];
+      const bool pad = j + u >= kb;
+      const bool ok = pad | ((d > kCholRelTol * ref[pad ? 0 : j + u]) & isfinite(d));
+      d = pad ? 1.0 : (ok ? d : ((d > 0.0 && isfinite(d)) ? d : 1.0));
+      rinv[u] = ci_rcp(d);
+      if (tid == 0) {
+        if (!ok) *bad = 1;
+        piv[j + u] = d;
+      }
+    }
+    if (a >= j + PB && a < kb) {
+      // rows below P: the Schur complement update
+#pragma unroll
+      for (int u = 0; u < PB; ++u) {
+        const double l = X[u][0] * rinv[u];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          if (c0 + q >= a && c0 + q < kb) own[q] -= l * X[u][1 + q];
+      }
+      if (a < j + 2 * PB) {
 #pragma unroll
         for (int q = 0; q < 4; ++q)
           if (c0 + q >= a && c0 + q < kb) Dm[a][c0 + q] = own[q];
       }
+    } else if (a >= j && a < j + PB) {
+      // a pivot row: its eliminated values are final
+#pragma unroll
+      for (int u = 0; u < PB; ++u)
+        if (a == j + u) {
+#pragma unroll
+          for (int q = 0; q < 4; ++q) own[q] = X[u][1 + q];
+        }
     }
     __syncthreads();
+  }
+  if (a < kb) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (c0 + q >= a && c0 + q < kb) Dm[a][c0 + q] = own[q];
   }
   if (tid < kb) {
     const double sj = sqrt(piv[tid]);
@@ -225,42 +277,65 @@ __device__ void ci_factor(double (*Dm)[CHP], int kb, const double* ref, double* 
   __syncthreads();
 }
 
-// Column c of R_kk^-1 into Xk by back substitution, one thread per column,
-// the column in registers: x_r = -(sum_{u > r} R_ru x_u) / R_rr with x_c =
-// 1 / R_cc and x_r = 0 for r > c (R_kk is zero past kb, so the padding
-// columns come out zero).  Fully unrolled, so the R_kk reads (the same
-// address in every lane) are independent of the recurrence and the serial
-// path is one multiply-add per row.
-__device__ void ci_inv_col(const double (*Rk)[CHP], const double* invd, int kb, int c,
-                           double (*Xk)[CHP]) {
-  double x[CB];
+// Xk = R_kk^-1 (upper, zero past kb) by blocked inversion on all threads:
+// the four 8 x 8 diagonal blocks by back substitution (a thread per column),
+// then X_12 = -X_11 R_12 X_22 for the two 16 x 16 diagonal blocks and for the
+// whole tile, each as two small products with a barrier between.  Tw is
+// scratch.  Ends with a barrier.
+__device__ void ci_tri_inv(const double (*Rk)[CHP], const double* invd, int kb,
+                           double (*Xk)[CHP], double (*Tw)[CHP]) {
+  const int tid = threadIdx.x;
+  for (int e = tid; e < CB * CB; e += 256) Xk[e & (CB - 1)][e >> 5] = 0.0;
+  __syncthreads();
+  if (tid < CB) {
+    const int c = tid, b0 = c & ~7;
+    double x[8];
 #pragma unroll
-  for (int r = CB - 1; r >= 0; --r) {
+    for (int r = 7; r >= 0; --r) {
+      double s = 0.0;
+#pragma unroll
+      for (int u = 7; u > r; --u) s = fma(Rk[b0 + r][b0 + u], x[u], s);
+      const double ir = (b0 + r < kb) ? invd[b0 + r] : 0.0;
+      x[r] = (b0 + r > c) ? 0.0 : ((b0 + r == c) ? ir : -s * ir);
+    }
+#pragma unroll
+    for (int r = 0; r < 8; ++r) Xk[b0 + r][c] = (c < kb) ? x[r] : 0.0;
+  }
+  __syncthreads();
+  // 8 -> 16: blocks at b0 = 0 and 16, sub-blocks of 8
+  if (tid < 128) {
+    const int b0 = (tid >> 6) * 16, r = (tid >> 3) & 7, c = tid & 7;
+    double s = 0.0;  // T = R_12 X_22
+#pragma unroll
+    for (int u = 0; u < 8; ++u) s = fma(Rk[b0 + r][b0 + 8 + u], Xk[b0 + 8 + u][b0 + 8 + c], s);
+    Tw[b0 + r][c] = s;
+  }
+  __syncthreads();
+  if (tid < 128) {
+    const int b0 = (tid >> 6) * 16, r = (tid >> 3) & 7, c = tid & 7;
+    double s = 0.0;  // X_12 = -X_11 T
+#pragma unroll
+    for (int u = 0; u < 8; ++u) s = fma(Xk[b0 + r][b0 + u], Tw[b0 + u][c], s);
+    Xk[b0 + r][b0 + 8 + c] = -s;
+  }
+  __syncthreads();
+  // 16 -> 32
+  {
+    const int r = tid >> 4, c = tid & 15;
     double s = 0.0;
 #pragma unroll
-    for (int u = CB - 1; u > r; --u) s = fma(Rk[r][u], x[u], s);
-    const double ir = (r < kb) ? invd[r] : 0.0;
-    x[r] = (r > c) ? 0.0 : ((r == c) ? ir : -s * ir);
+    for (int u = 0; u < 16; ++u) s = fma(Rk[r][16 + u], Xk[16 + u][16 + c], s);
+    Tw[r][c] = s;
   }
+  __syncthreads();
+  {
+    const int r = tid >> 4, c = tid & 15;
+    double s = 0.0;
 #pragma unroll
-  for (int r = 0; r < CB; ++r) Xk[r][c] = (c < kb) ? x[r] : 0.0;
-}
-
-// Column c of P <- R_kk^-T P by forward substitution, one thread per column,
-// fully unrolled (the column in registers; the last term of each row's sum is
-// the one just computed).
-__device__ void ci_fwd_col(const double (*Rk)[CHP], const double* invd, int kb, int c,
-                           double (*P)[CHP]) {
-  double y[CB];
-#pragma unroll
-  for (int j = 0; j < CB; ++j) {
-    double s = P[j][c];
-#pragma unroll
-    for (int i = 0; i < j; ++i) s = fma(-Rk[i][j], y[i], s);
-    y[j] = (j < kb) ? s * invd[j] : 0.0;
+    for (int u = 0; u < 16; ++u) s = fma(Xk[r][u], Tw[u][c], s);
+    Xk[r][16 + c] = -s;
   }
-#pragma unroll
-  for (int j = 0; j < CB; ++j) P[j][c] = y[j];
+  __syncthreads();
 }
 
 // acc[q] += sum_j op(A)[a][j] B[j][c0 + q] for thread (a, c0) over the padded
@@ -334,14 +409,23 @@ cholinv_panel_kernel(int64_t n, double* __restrict__ G, double* __restrict__ R,
   if (role == 0 && k == 0 && ti == tj && tid < bw(ti)) diag0[pti + tid] = T3[tid][tid];
   ci_factor(Dm, kb, refd, piv, sqd, invd, &bad);
 
+  ci_tri_inv(Dm, invd, kb, Xk, T4);
+
   if (role == 0) {
-    // R_k,ti and R_k,tj, then the trailing tile
-    if (tid < CB) ci_fwd_col(Dm, invd, kb, tid, T1);
-    else if (tj != ti && tid >= 64 && tid < 64 + CB) ci_fwd_col(Dm, invd, kb, tid - 64, T2);
+    // R_k,ti = X_kk^T G_k,ti into T4 and R_k,tj into Dm (R_kk is not needed
+    // here any more), then the trailing tile
+    double pa[4] = {0.0, 0.0, 0.0, 0.0}, pb[4] = {0.0, 0.0, 0.0, 0.0};
+    ci_mma<true>(Xk, T1, pa);
+    if (tj != ti) ci_mma<true>(Xk, T2, pb);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      T4[a][c0 + q] = pa[q];
+      if (tj != ti) Dm[a][c0 + q] = pb[q];
+    }
     __syncthreads();
-    const double(*Pj)[CHP] = (tj != ti) ? T2 : T1;
+    const double(*Pj)[CHP] = (tj != ti) ? Dm : T4;
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    ci_mma<true>(T1, Pj, acc);
+    ci_mma<true>(T4, Pj, acc);
     const int wi = bw(ti), wj = bw(tj);
     if (a < wi) {
 #pragma unroll
@@ -351,31 +435,32 @@ cholinv_panel_kernel(int64_t n, double* __restrict__ G, double* __restrict__ R,
     if (ti == tj) {
       for (int e = tid; e < CB * CB; e += 256) {
         const int r = e & (CB - 1), cc = e >> 5;
-        if (r < kb && cc < wi) R[(p + r) + (pti + cc) * n] = T1[r][cc];
+        if (r < kb && cc < wi) R[(p + r) + (pti + cc) * n] = T4[r][cc];
       }
     }
     return;
   }
 
-  // roles 1 and 2 need X_kk; role 1 also R_k,tj
-  if (tid < CB) ci_inv_col(Dm, invd, kb, tid, Xk);
-  else if (role == 1 && tid >= 64 && tid < 64 + CB) ci_fwd_col(Dm, invd, kb, tid - 64, T1);
-  __syncthreads();
-  // X_ik = -Acc_ik X_kk (rows of block i), or X_kk
+  // X_ik = -Acc_ik X_kk (rows of block i) into T4, or X_kk itself; role 1
+  // also R_k,tj = X_kk^T G_k,tj into Dm
   const double(*Xi)[CHP] = Xk;
-  if (i < k) {
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    ci_mma<false>(T2, Xk, acc);
+  {
+    double pb[4] = {0.0, 0.0, 0.0, 0.0}, xa[4] = {0.0, 0.0, 0.0, 0.0};
+    if (role == 1) ci_mma<true>(Xk, T1, pb);
+    if (i < k) ci_mma<false>(T2, Xk, xa);
 #pragma unroll
-    for (int q = 0; q < 4; ++q) T4[a][c0 + q] = -acc[q];
+    for (int q = 0; q < 4; ++q) {
+      if (role == 1) Dm[a][c0 + q] = pb[q];
+      if (i < k) T4[a][c0 + q] = -xa[q];
+    }
     __syncthreads();
-    Xi = T4;
+    if (i < k) Xi = T4;
   }
   if (role == 1) {
     double acc[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) acc[q] = (i < k) ? T3[a][c0 + q] : 0.0;
-    ci_mma<false>(Xi, T1, acc);
+    ci_mma<false>(Xi, Dm, acc);
     const int rows = (i == k) ? kb : CB, wj = bw(tj);
     if (a < rows) {
 #pragma unroll

@@ -315,8 +315,9 @@ int mlp_grad(Context* c, const float* theta, const float* X, const int* idx, con
              const float* r, float lam, int B, float* grad, float* loss);
 int mlp_train(Context* c, float* theta, const float* X, const int* q, const float* r, int n,
               float lam, int epochs, int batch, float lr, uint64_t seed, float* loss_hist);
-int sketch_features(Context* c, const float* A, int64_t m, int64_t n, int64_t lda, uint64_t seed,
-                    float* feat);
+// The 64 learned-path features of A (FP64) from a sketch of its float copy.
+int sketch_features(Context* c, const double* A, const float* A32, int64_t m, int64_t n,
+                    int64_t lda, uint64_t seed, float* feat);
 // Symmetric eigensolver (two-sided block Jacobi).  G (n x n, ld ldg, both
 // triangles, symmetric) is diagonalised in place: on return G ~ diag(lambda)
 // and V (optional, n x n) holds the eigenvectors (V^T G_in V = diag);

@@ -72,43 +72,40 @@ __device__ __forceinline__ void local_pair(int k, int t, bool cross, int* p, int
 
 // The inner sweep of one block pair on its S x S tile (S = 2B), in LDS: H in
 // Ha (double-buffered with Hb), J accumulated in Jl (row-major, padded HP).
-// A step: threads 0..B-1 compute the rotation of local pair tid once and
-// publish it; after a barrier thread t = (bi, bj) (bi = t / B, bj = t % B)
-// applies R_bi^T . R_bj to the 2 x 2 block of H at (pair bi rows, pair bj
-// columns) -- the blocks partition H -- and R_bi to rows 2 bj, 2 bj + 1 of J's
-// columns of pair bi.  Returns the buffer holding the final H.
-template <int B>
-__device__ double* jac_inner(double* Ha, double* Hb, double* Jl, double* rc, double* rs,
-                             bool cross, double flo, int* any_sig) {
+// A step: thread t = (bi, bj) (bi = t / B, bj = t % B) computes the rotation
+// of local pair bj (the 16 lanes of each row of a wave compute the 16
+// rotations), takes that of pair bi from the lane of its row that computed it
+// (a cross-lane read, no LDS round trip and no extra barrier), applies
+// R_bi^T . R_bj to the 2 x 2 block of H at (pair bi rows, pair bj columns) --
+// the blocks partition H -- and R_bi to rows 2 bj, 2 bj + 1 of J's columns of
+// pair bi.  One barrier per step.  Returns the buffer holding the final H.
+template <int B, bool CROSS>
+__device__ double* jac_inner_t(double* Ha, double* Hb, double* Jl, double flo, int* any_sig) {
   constexpr int S = 2 * B, HP = S + 1;
-  static_assert(B * B == 256, "one 2x2 block per thread");
+  static_assert(B == 16, "a row of 16 lanes holds the B rotations of a step");
   const int tid = threadIdx.x;
   const int bi = tid / B, bj = tid % B;
+  const int src = (tid & 48) | bi;  // lane of this row that owns pair bi
   double* Hc = Ha;
   double* Hn = Hb;
   int sig = 0;
-  const int steps = cross ? B : S - 1;
+  constexpr int steps = CROSS ? B : S - 1;
   for (int t = 0; t < steps; ++t) {
-    if (tid < B) {
-      int p, q;
-      local_pair<B>(tid, t, cross, &p, &q);
-      double c, sn;
-      int sg;
-      rotation(Hc[p * HP + p], Hc[q * HP + q], Hc[p * HP + q], flo, &c, &sn, &sg);
-      rc[tid] = c;
-      rs[tid] = sn;
-      sig |= sg;
-    }
-    __syncthreads();
     int pi, qi, pj, qj;
-    local_pair<B>(bi, t, cross, &pi, &qi);
-    local_pair<B>(bj, t, cross, &pj, &qj);
-    const double ci = rc[bi], si = rs[bi], cj = rc[bj], sj = rs[bj];
+    local_pair<B>(bi, t, CROSS, &pi, &qi);
+    local_pair<B>(bj, t, CROSS, &pj, &qj);
+    const int x0 = 2 * bj;
+    // every LDS read of the step first, so that one latency covers them
+    const double aj = Hc[pj * HP + pj], bbj = Hc[qj * HP + qj], gj = Hc[pj * HP + qj];
     const double h00 = Hc[pi * HP + pj], h01 = Hc[pi * HP + qj];
     const double h10 = Hc[qi * HP + pj], h11 = Hc[qi * HP + qj];
-    const int x0 = 2 * bj;
     const double j00 = Jl[x0 * HP + pi], j01 = Jl[x0 * HP + qi];
     const double j10 = Jl[(x0 + 1) * HP + pi], j11 = Jl[(x0 + 1) * HP + qi];
+    double cj, sj;
+    int sgj;
+    rotation(aj, bbj, gj, flo, &cj, &sj, &sgj);
+    sig |= sgj;
+    const double ci = __shfl(cj, src, 64), si = __shfl(sj, src, 64);
     // X = H_blk R_j, then R_i^T X
     const double x00 = cj * h00 - sj * h01, x01 = sj * h00 + cj * h01;
     const double x10 = cj * h10 - sj * h11, x11 = sj * h10 + cj * h11;
@@ -130,6 +127,13 @@ __device__ double* jac_inner(double* Ha, double* Hb, double* Jl, double* rc, dou
   }
   *any_sig = sig;
   return Hc;
+}
+
+template <int B>
+__device__ double* jac_inner(double* Ha, double* Hb, double* Jl, bool cross, double flo,
+                             int* any_sig) {
+  return cross ? jac_inner_t<B, true>(Ha, Hb, Jl, flo, any_sig)
+               : jac_inner_t<B, false>(Ha, Hb, Jl, flo, any_sig);
 }
 
 // J and the final tile H of pair w out (column-major S x S each); the flag
@@ -161,7 +165,6 @@ symj_solve_kernel(int64_t n, const double* __restrict__ G, int64_t ldg, int nblo
                   double* __restrict__ Hout, unsigned long long* __restrict__ flag) {
   constexpr int S = 2 * B, HP = S + 1;
   __shared__ double Ha[S * HP], Hb[S * HP], Jl[S * HP];
-  __shared__ double rc[B], rs[B];
   __shared__ int64_t cols[S];
   const int w = blockIdx.x;
   const int tid = threadIdx.x;
@@ -176,7 +179,7 @@ symj_solve_kernel(int64_t n, const double* __restrict__ G, int64_t ldg, int nblo
   const double flo = 4.0 * 2.220446049250313e-16 * sqrt(*fro2);
   __syncthreads();
   int sig = 0;
-  const double* Hf = jac_inner<B>(Ha, Hb, Jl, rc, rs, round > 0, flo, &sig);
+  const double* Hf = jac_inner<B>(Ha, Hb, Jl, round > 0, flo, &sig);
   jac_out<B>(Hf, Jl, w, Jout, Hout, sig, flag);
 }
 
@@ -345,7 +348,6 @@ symj_step_kernel(int64_t n, const double* __restrict__ Gin, double* __restrict__
   // workgroups share a CU.
   static_assert(2 * B * HP <= S * HP && S * BP <= S * HP, "solve role fits the apply regions");
   __shared__ double sm[4 * S * HP];
-  __shared__ double rc[B], rs[B];
   __shared__ int64_t c0[S], c1[S];
   const int tid = threadIdx.x;
   if ((int)blockIdx.x >= nsolve) {
@@ -434,7 +436,7 @@ symj_step_kernel(int64_t n, const double* __restrict__ Gin, double* __restrict__
     Jl[i * HP + j] = (i == j) ? 1.0 : 0.0;
   }
   int sig = 0;
-  const double* Hf = jac_inner<B>(Ha, Hb, Jl, rc, rs, round1 > 0, flo, &sig);
+  const double* Hf = jac_inner<B>(Ha, Hb, Jl, round1 > 0, flo, &sig);
   jac_out<B>(Hf, Jl, w, Jout, Hout, sig, flag);
 }
 

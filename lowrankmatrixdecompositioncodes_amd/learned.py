@@ -91,12 +91,13 @@ def plain_adaptive(A64: rp.DeviceMatrix, tol_rel: float, q: int, kstep: int = KS
 
 
 def label(fm: FamilyMatrix, tol_rel: float = TOL_DEF, kstep: int = KSTEP_DEF, p: int = P_DEF):
-    """(q*, rank fraction, frank per q) from plain adaptive runs for q = 0..4.
+    """(q*, rank target, frank per q) from plain adaptive runs for q = 0..4.
 
-    Runs that miss TOL (TOL out of reach below full rank: the noise floor's
-    energy alone exceeds it) cost as if they had gone to full rank; when every
-    q misses, the rank label is 1 (the network then learns to send such
-    matrices straight to the adaptive method)."""
+    The rank target is log(frank) / log(min(m, n)) of the cheapest q.  Runs
+    that miss TOL (TOL out of reach below full rank: the noise floor's energy
+    alone exceeds it) cost as if they had gone to full rank; when every q
+    misses, the target is 1 (full rank: the served path then solves such a
+    matrix exactly)."""
     m, n = fm.A64.rows, fm.A64.cols
     mn = min(m, n)
     anorm = frob(fm.A64)
@@ -108,14 +109,15 @@ def label(fm: FamilyMatrix, tol_rel: float = TOL_DEF, kstep: int = KSTEP_DEF, p:
         ok.append(res <= tol_rel * (1 + 1e-9))
     cost = [(2 * q + 2) * ((franks[q] if ok[q] else mn) + kstep) for q in range(5)]
     qs = int(np.argmin(cost))
-    return qs, (franks[qs] / mn if ok[qs] else 1.0), franks
+    # the rank target on a log scale, log(frank) / log(min(m, n)): 1 = full rank
+    return qs, (np.log(max(franks[qs], 1)) / np.log(mn) if ok[qs] else 1.0), franks
 
 
 def features(fm: FamilyMatrix) -> np.ndarray:
     lib = rp.load()
     f = rp.DeviceMatrix(64, 1, np.float32)
-    rp._check(lib.rsvd_dev_sketch_features(fm.A32.ptr, fm.A32.rows, fm.A32.cols, fm.A32.rows,
-                                           fm.seed ^ 0x5EED5EED, f.ptr), "features")
+    rp._check(lib.rsvd_dev_sketch_features(fm.A64.ptr, fm.A32.ptr, fm.A32.rows, fm.A32.cols,
+                                           fm.A32.rows, fm.seed ^ 0x5EED5EED, f.ptr), "features")
     return f.to_host()[:, 0]
 
 

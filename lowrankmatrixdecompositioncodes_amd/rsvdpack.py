@@ -161,8 +161,8 @@ _SIGS = {
     "rsvd_dev_mlp_train": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
                                      C.c_float, C.c_int, C.c_int, C.c_float, C.c_ulonglong,
                                      C.c_void_p]),
-    "rsvd_dev_sketch_features": (C.c_int, [C.c_void_p, C.c_longlong, C.c_longlong, C.c_longlong,
-                                           C.c_ulonglong, C.c_void_p]),
+    "rsvd_dev_sketch_features": (C.c_int, [C.c_void_p, C.c_void_p, C.c_longlong, C.c_longlong,
+                                           C.c_longlong, C.c_ulonglong, C.c_void_p]),
     "rsvd_dev_learned_lowrank": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_longlong,
                                            C.c_longlong, C.c_longlong, C.c_double, C.c_int,
                                            C.c_double, C.c_int, C.c_int, C.c_ulonglong,

@@ -92,6 +92,38 @@ def test_mlp_training_learns_a_rule(gpu_lib):
     assert np.sqrt(np.mean((out[:, 5] - r) ** 2)) < 0.1
 
 
+def test_sketch_features_energy_profile(gpu_lib):
+    """x_j = log10(t_j / ||A||^2) / 8 with t_j the energy the best rank-(j+1)
+    approximation inside the 64-column sketch leaves.  On an exactly rank-20
+    matrix the sketch spans range(A) (to the float copy's rounding), so t_j is
+    the optimal SVD tail sum_{i > j} sigma_i^2 while that tail is far above
+    the rounding floor, and from j = 19 on x_j sits at that floor (the float
+    copy's rounding seen through the sketch basis: ~1e-11 ||A||^2 here, far
+    below the TOL^2 = 1e-8 the features have to resolve); the profile never
+    increases."""
+    rp = gpu_lib
+    lib = rp.load()
+    rng = np.random.default_rng(5)
+    m, n, r = 300, 200, 20
+    U, _ = np.linalg.qr(rng.standard_normal((m, r)))
+    V, _ = np.linalg.qr(rng.standard_normal((n, r)))
+    sig = 2.0 ** -np.arange(r)
+    A = (U * sig) @ V.T
+    A64 = rp.DeviceMatrix.from_host(A)
+    A32 = rp.DeviceMatrix.from_host(A, np.float32)
+    f = rp.DeviceMatrix(64, 1, np.float32)
+    assert lib.rsvd_dev_sketch_features(A64.ptr, A32.ptr, m, n, m, 99, f.ptr) == 0, \
+        rp.last_error()
+    x = f.to_host()[:, 0].astype(np.float64)
+    a2 = np.sum(sig ** 2)
+    tail = a2 - np.cumsum(sig ** 2)
+    want = np.log10(np.maximum(tail[:12], 1e-16 * a2) / a2) / 8
+    assert np.abs(x[:12] - want).max() <= 2e-4, (x[:12], want)
+    assert np.all(x[19:] <= -1.25), x[19:]
+    assert np.all(np.diff(x) <= 1e-6)
+    assert x.min() >= -2.0 - 1e-6 and x.max() <= 0.0
+
+
 def test_learned_lowrank_meets_tol(gpu_lib):
     from lowrankmatrixdecompositioncodes_amd import learned
     ds = learned.build_dataset(24, seed=77)
