@@ -16,12 +16,20 @@ constexpr double kStopTol = 1e-10;
 // pair with relative eigenvalue gap d and ratio r turns by |s| ~ r / d, and
 // the next sweep leaves ratios ~ r^2 / d = s^2 d.  A sweep whose rotations
 // all had |s| <= 1e-4 therefore leaves off-diagonals below ~1e-8 of
-// sqrt(g_pp g_qq) for every gap, clusters of nearly equal eigenvalues included
-// (there the ratio test alone stops while rotations inside the cluster are
-// still large and refill each other); the eigenvalues are then exact to
-// ~1e-16 relative (second order) and the eigenvectors, and the orthogonality
-// of B^T V Sigma^-1, to ~1e-8, below the fp32 resolution of the outputs
+// sqrt(g_pp g_qq) for every gap (a ratio test of 1e-4 alone could stop with
+// ratios of 1e-4 inside a cluster of nearly equal eigenvalues, i.e.
+// first-order eigenvalue errors of 1e-4 there).  A rotation of ratio at most
+// kLooseRatioTol does not count, however large its angle: inside a cluster
+// (gap below 1e-4) the angles stay large until the splitting the cluster's
+// members carry -- for a plateau of equal singular values, the rounding of
+// the preceding stages -- is resolved to rounding, while such a pair already
+// perturbs its eigenvalues by at most 1e-8 relative and leaves the columns
+// of B^T V Sigma^-1 orthogonal to 1e-8.  Either way the eigenvalues are good
+// to ~1e-8 relative or better and the eigenvectors of separated eigenvalues,
+// and the orthogonality of B^T V Sigma^-1, to ~1e-8, below the fp32
+// resolution of the outputs.
 constexpr double kLooseStopTol = 1e-4;
+constexpr double kLooseRatioTol = 1e-8;
 
 // 1/x and 1/sqrt(x) from the hardware estimates plus two Newton steps
 // (error a few ulp; a rotation only needs c^2 + s^2 = 1 to rounding).
@@ -51,7 +59,8 @@ static __device__ __forceinline__ double rsq_nr(double x) {
 // Evaluated on the entries scaled by a power of two so that the squares stay
 // in range.  Branch-free.  No rotation when |g| <= flo or |g| <= kRotTol
 // sqrt|ab|; *sig bit 0 reports |g| > kStopTol sqrt|ab|, bit 1 a rotation
-// with |s| > kLooseStopTol (|s| <= |g| / sqrt|ab|, so bit 1 implies bit 0).
+// with |s| > kLooseStopTol and |g| > kLooseRatioTol sqrt|ab| (bit 1 implies
+// bit 0).
 static __device__ __forceinline__ void rotation(double a, double b, double g, double flo, double* cs,
                                          double* sn, int* sig) {
   const double ag = fabs(g);
@@ -70,7 +79,8 @@ static __device__ __forceinline__ void rotation(double a, double b, double g, do
   *cs = rot ? den * q : 1.0;
   *sn = rot ? t2g * q : 0.0;
   *sig = ((big & (g2 > kStopTol * kStopTol * ab)) ? 1 : 0) |
-         ((rot & (fabs(*sn) > kLooseStopTol)) ? 2 : 0);
+         ((rot & (fabs(*sn) > kLooseStopTol) & (g2 > kLooseRatioTol * kLooseRatioTol * ab)) ? 2
+                                                                                           : 0);
 }
 
 }  // namespace rsvd

@@ -138,7 +138,7 @@ __device__ double* jac_inner(double* Ha, double* Hb, double* Jl, bool cross, dou
 
 // J and the final tile H of pair w out (column-major S x S each); the flag
 // collects the rotation-size bits of rotation() (bit 0: above kStopTol, bit
-// 1: above kLooseStopTol).
+// 1: a rotation the loose stop counts).
 template <int B>
 __device__ void jac_out(const double* Hf, const double* Jl, int w, double* __restrict__ Jout,
                         double* __restrict__ Hout, int any_sig,
@@ -504,7 +504,7 @@ int run_sweeps(Context* c, int64_t n, double* G, int64_t ldg, double* V, int64_t
       }
       unsigned long long h = 0;
       RSVD_TRY(read_small(c, &h, flags.p + sweep, sizeof(h)));
-      // no pair met in this sweep was above kStopTol (kLooseStopTol)
+      // no pair met in this sweep was above kStopTol (or counted by the loose stop)
       if ((h & (loose ? 2ULL : 1ULL)) == 0) {
         ++sweep;
         break;
