@@ -401,6 +401,14 @@ apass2_kernel(int64_t M, int64_t N, int64_t KBtot, int64_t KBper, float alpha,
       }
     }
   };
+  // one row tile of A for k block kb (op(A) = A: eight 4-byte loads along k)
+  auto loadA_row = [&](int64_t kb, int i) {
+    kb = kb < klast ? kb : klast;
+    kb_raw = kb;
+    const float* p = Ar[i] + kb * 32 * lda;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) raw[i][e] = p[e * lda];
+  };
   bf16x8_t ah[MT], al[MT];
   // fp16: 1 / a_c of the chunk of A's columns the values in raw come from --
   // the k block's chunk (op(A) = A) or the workgroup's 128 rows (op(A) = A^T)
@@ -432,11 +440,21 @@ apass2_kernel(int64_t M, int64_t N, int64_t KBtot, int64_t KBper, float alpha,
 #pragma unroll
     for (int i = 0; i < MT; ++i) acc[j][i] = f4v_t{0.f, 0.f, 0.f, 0.f};
 
-  // one k block on stage `cur`: A(kb + 1) and S(kb + 2) loads first, then the
-  // MFMAs with the S fragments read one tile ahead, then the A split for kb + 1
+  // one k block on stage `cur`: the MFMAs with the S fragments read one tile
+  // ahead, the A loads of kb + 1 and the A split for kb + 1, and the S(kb + 2)
+  // loads.  op(A) = A issues A(kb + 1) one row tile per column pair, spread
+  // over the MFMAs (its 32 four-byte loads per block, issued together, kept
+  // the load path busy in bursts), and S(kb + 2) after the split: the split's
+  // wait for A then covers no S load that is not due anyway, so S keeps a full
+  // block of lead (measured standalone at m = 65536, n = 16384, l = 544:
+  // 3.55 -> 3.22 ms, bitwise the same output).  op(A) = A^T, whose A loads
+  // are 16-byte vectors, measured 3% slower that way and keeps both at the
+  // start of the block.
   auto step = [&](int64_t kb, bf16x8_t (*cur)[kBNT2][64], bf16x8_t (*nx2)[kBNT2][64]) {
-    loadA(kb + 1);
-    loadS(kb + 2, nx2);
+    if constexpr (TA) {
+      loadA(kb + 1);
+      loadS(kb + 2, nx2);
+    }
     __builtin_amdgcn_sched_barrier(0);
     if constexpr (!TA) {
       // column tiles in pairs, so that consecutive MFMAs on one accumulator are
@@ -448,9 +466,11 @@ apass2_kernel(int64_t M, int64_t N, int64_t KBtot, int64_t KBper, float alpha,
         sh[0][q] = cur[0][wn * kNTW2 + q][lane];
         if (SPLIT >= 2) sl[0][q] = cur[NP - 1][wn * kNTW2 + q][lane];
       }
+      static_assert(NPR >= MT, "one row tile of A per column pair");
 #pragma unroll
       for (int jp = 0; jp < NPR; ++jp) {
         const int b = jp & 1;
+        if (jp < MT) loadA_row(kb + 1, jp);
         if (jp + 1 < NPR) {
 #pragma unroll
           for (int q = 0; q < 2; ++q) {
@@ -521,6 +541,7 @@ apass2_kernel(int64_t M, int64_t N, int64_t KBtot, int64_t KBper, float alpha,
       }
     }
     split();
+    if constexpr (!TA) loadS(kb + 2, nx2);
     __syncthreads();
   };
 

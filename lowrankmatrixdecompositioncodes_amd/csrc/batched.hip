@@ -426,9 +426,13 @@ __device__ __forceinline__ int circle(int pos, int r, int N) {
 // two-sided cyclic Jacobi on H (np x np, np even, positive semidefinite) with
 // J accumulated.  A step: threads 0..np/2-1 compute the rotation of their pair
 // once (round-robin pairing) and publish it with the pair's two indices;
-// after a barrier the 2 x 2 blocks of H (pairs bi x bj) and the (row, pair)
-// entries of J are updated in parallel into the second H buffer.  Sweeps
-// until no pair met in a sweep had a relative off-diagonal above kStopTol.
+// after a barrier the 2 x 2 blocks of H (pairs bi x bj: thread t takes bi in
+// {t / 16, t / 16 + 16}, bj in {t % 16, t % 16 + 16}) and the (row, pair)
+// entries of J (row t % 64, pairs t / 64 + 4 i) are updated in parallel into
+// the second H buffer -- fixed strides, no division by the runtime size.
+// Sweeps until a sweep's rotations pass the loose stop of rotation() (the
+// outputs are fp32: eigenvalues good to ~1e-8 relative, eigenvectors of
+// separated eigenvalues to ~1e-8).
 __device__ void lds_jacobi(double (*H)[kLP], double (*H2)[kLP], double (*J)[kLP], int np,
                            int* sig_flag) {
   __shared__ double rc[kL / 2], rs[kL / 2];
@@ -460,8 +464,11 @@ __device__ void lds_jacobi(double (*H)[kLP], double (*H2)[kLP], double (*J)[kLP]
         sig |= sg;
       }
       __syncthreads();
-      for (int e = tid; e < half * half; e += 256) {
-        const int bi = e / half, bj = e - bi * half;
+      static_assert(kL / 2 <= 32, "pair indices below 32: two strides of 16");
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int bi = (tid >> 4) + 16 * (e >> 1), bj = (tid & 15) + 16 * (e & 1);
+        if (bi >= half || bj >= half) continue;
         const int pi = pp[bi], qi = qq[bi], pj = pp[bj], qj = qq[bj];
         const double ci = rc[bi], si = rs[bi], cj = rc[bj], sj = rs[bj];
         const double h00 = Hc[pi][pj], h01 = Hc[pi][qj], h10 = Hc[qi][pj], h11 = Hc[qi][qj];
@@ -475,8 +482,10 @@ __device__ void lds_jacobi(double (*H)[kLP], double (*H2)[kLP], double (*J)[kLP]
         Hn[qi][pj] = y10;
         Hn[qi][qj] = y11;
       }
-      for (int e = tid; e < np * half; e += 256) {
-        const int k = e / np, x = e - k * np;
+      static_assert(kL <= 64, "a J row per lane of a wave");
+      for (int k = tid >> 6; k < half; k += 4) {
+        const int x = tid & 63;
+        if (x >= np) continue;
         const int pk = pp[k], qk = qq[k];
         const double ck = rc[k], sk = rs[k];
         const double jp = J[x][pk], jq = J[x][qk];
@@ -488,7 +497,7 @@ __device__ void lds_jacobi(double (*H)[kLP], double (*H2)[kLP], double (*J)[kLP]
       Hc = Hn;
       Hn = tmp;
     }
-    if (sig) atomicOr(sig_flag, 1);
+    if (sig & 2) atomicOr(sig_flag, 1);
     __syncthreads();
     const int any = *sig_flag;
     __syncthreads();
@@ -736,8 +745,14 @@ int autotune(Context* c, Plan& P) {
     else P.rt_nn = best_rt;
   }
   // the other stages, timed once each for the report (the work buffers hold
-  // real operands from the eager call; the graph recomputes everything)
-  for (int st = 0; st < 4 && rc == 0; ++st) {
+  // real operands from the eager call; the graph recomputes everything).  The
+  // final stage runs before the orthonormalisation of the Z region, which at
+  // this point holds the eager call's B^T: orthonormalised first, it would
+  // give the final stage B B^T = I to diagonalise.  stage_ms: orth Y, orth Z,
+  // final, tile.
+  static const int kStageOrder[4] = {0, 2, 1, 3};
+  for (int i = 0; i < 4 && rc == 0; ++i) {
+    const int st = kStageOrder[i];
     (void)hipEventRecord(e0, c->stream);
     if (st == 0) rc = launch_orth(c, P, false);
     else if (st == 1) rc = launch_orth(c, P, true);
