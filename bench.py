@@ -379,7 +379,7 @@ def bench_learned(args, rp, lib) -> int:
     log = lambda s: print("[bench] " + s, file=sys.stderr, flush=True)  # noqa: E731
     seed = 2027
     t0 = time.perf_counter()
-    ds = learned.build_dataset(args.train, seed)
+    ds = learned.build_dataset(args.train, seed, progress=128)
     log(f"dataset: {args.train} matrices ({5 * args.train} plain runs) in "
         f"{time.perf_counter() - t0:.1f} s, q* histogram {np.bincount(ds.q, minlength=5).tolist()}")
     t0 = time.perf_counter()

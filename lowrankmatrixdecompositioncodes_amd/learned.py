@@ -19,6 +19,7 @@ this module drives them through the C ABI:
 from __future__ import annotations
 
 import ctypes as C
+import sys
 import time
 from dataclasses import dataclass, field
 
@@ -129,9 +130,14 @@ class Dataset:
     franks: list = field(default_factory=list)
 
 
-def build_dataset(count: int, seed: int, tol_rel: float = TOL_DEF) -> Dataset:
+def build_dataset(count: int, seed: int, tol_rel: float = TOL_DEF,
+                  progress: int = 0) -> Dataset:
+    """Features and labels of family matrices 0 .. count - 1; with progress > 0
+    a line on stderr every `progress` matrices."""
     X, qs, rs, fr = [], [], [], []
     for i in range(count):
+        if progress and i and i % progress == 0:
+            print(f"[learned] dataset: {i} of {count} matrices", file=sys.stderr, flush=True)
         fm = family(i, seed)
         X.append(features(fm))
         q, r, franks = label(fm, tol_rel)
