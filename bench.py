@@ -342,6 +342,52 @@ def bench_blocked(args, rp, lib) -> int:
         kk = min(len(s64), len(sm))
         err = {"max_abs_sigma_diff_over_sigma1": float(np.abs(sm[:kk] - s64[:kk]).max() / sigma[0]),
                "frank_fp64": out64[0]}
+    cpu = None
+    if not args.no_cpu:
+        # the oracle (blockrand in TOL mode, FP64) on rows 0 .. rows - 1 of
+        # the same A with the same relative TOL, and the FP64 device path on
+        # that same sample (a view of A: leading dimension m) beside it; the
+        # oracle's cost is not linear in the rows (its Gaussian test matrix,
+        # the SVD of B and the final re-read depend on n and the rank), so
+        # the sample's seconds are reported as they are, not scaled
+        from oracle import ref
+        threads, host_cores = physical_cores()
+        threads = args.cpu_threads or threads
+        ref.load(threads)
+        rows = min(args.cpu_rows_blocked, m)
+        log(f"cpu baseline: oracle blockrand on {threads} threads ({ref.backend()}), "
+            f"rows 0..{rows - 1}")
+        A_host = np.empty((rows, n), order="F")
+        for c in range(n):
+            if lib.rsvd_memcpy_d2h(A_host[:, c].ctypes.data, A64.ptr + 8 * c * m, 8 * rows):
+                raise SystemExit("d2h: " + rp.last_error())
+        tol_s = 1e-6 * float(np.linalg.norm(A_host))
+        t0 = time.perf_counter()
+        _, _, _, fr_s, _ = ref.blockrand(A_host, -1, p, tol_s, 2, kstep, q, 1, seed=99)
+        cpu_s = time.perf_counter() - t0
+        del A_host
+        g_s = []
+        for it in range(3):
+            fr = C.c_int(0)
+            u, sv, v = C.c_void_p(), C.c_void_p(), C.c_void_p()
+            lib.rsvd_sync()
+            t0 = time.perf_counter()
+            if lib.rsvd_dev_blockrand(A64.ptr, rows, n, m, -1, p, tol_s, 2, kstep, q, 1, 99, None,
+                                      C.byref(fr), C.byref(u), C.byref(sv), C.byref(v)):
+                raise SystemExit("blockrand (sample): " + rp.last_error())
+            g_s.append(time.perf_counter() - t0)
+            for ptr in (u, sv, v):
+                lib.rsvd_dev_free(ptr.value)
+        gpu_s = float(np.median(g_s[1:]))
+        cpu = {"value": round(cpu_s, 3), "unit": "s per solve of the sample",
+               "cores": threads, "host_physical_cores": host_cores, "kind": "port",
+               "sample_frank": fr_s, "gpu_fp64_same_sample_s": round(gpu_s, 4),
+               "gpu_fp64_frank": fr.value,
+               "sample": f"one call of the oracle (FP64 blockrand, {ref.backend().split(':')[0]})"
+                         f" on rows 0..{rows - 1} of the same A ({rows}x{n}), TOL = 1e-6 of the "
+                         f"sample's norm, kstep={kstep} q={q} p={p}; the FP64 device path on the "
+                         f"same sample beside it (median of 2 after 1 warm-up)"}
+
     line = {
         "metric": "blocked fixed-precision QB + rSVD (blockrand TOL mode), seconds per solve",
         "value": round(ms / 1e3, 4), "unit": "s", "n_gpus": 1, "steps": args.steps,
@@ -360,6 +406,7 @@ def bench_blocked(args, rp, lib) -> int:
         "speedup_vs_fp64_plain": None if (ms64 is None or not prec_id) else round(ms64 / ms, 3),
         "error_vs_fp64": err,
         "kernel_ms_per_call": breakdown,
+        "cpu_baseline": cpu,
         "note": "frank is the reference's reported rank (shrunk by its rra.c:278 rule); "
                 "residual_rel = ||A - U S V^T||_F / ||A||_F in FP64 against the double A",
     }
@@ -396,6 +443,30 @@ def bench_learned(args, rp, lib) -> int:
     tf_learn = sum(r["seconds"] for r in feas)
     viol = [r for r in res if r["resid_rel"] > tol]
     sp_feas = tf_plain / max(tf_learn, 1e-12)
+    cpu = None
+    if not args.no_cpu:
+        # the oracle's plain method (blockrand in TOL mode, FP64, same q,
+        # kstep, p and TOL) on the first held-out matrices
+        from oracle import ref
+        threads, host_cores = physical_cores()
+        threads = args.cpu_threads or threads
+        ref.load(threads)
+        cnt = min(args.cpu_learned, args.eval)
+        cpu_s = 0.0
+        for j in range(cnt):
+            A = learned.family(10**6 + j, seed).A64.to_host()
+            tol_j = learned.TOL_DEF * float(np.linalg.norm(A))
+            t0 = time.perf_counter()
+            ref.blockrand(A, -1, learned.P_DEF, tol_j, 2, learned.KSTEP_DEF, learned.Q_PLAIN, 1,
+                          seed=j + 1)
+            cpu_s += time.perf_counter() - t0
+        cpu = {"value": round(1e3 * cpu_s / max(cnt, 1), 3), "unit": "ms per plain solve",
+               "cores": threads, "host_physical_cores": host_cores, "kind": "port",
+               "seconds": round(cpu_s, 3),
+               "sample": f"the oracle's blockrand (FP64, {ref.backend().split(':')[0]}) in TOL "
+                         f"mode on the first {cnt} held-out matrices, q={learned.Q_PLAIN} "
+                         f"kstep={learned.KSTEP_DEF} p={learned.P_DEF}"}
+        log(f"cpu baseline: {cpu['value']} ms per plain solve over {cnt} matrices")
     line = {
         "metric": "learned q/rank path vs plain adaptive blockrand at TOL=1e-4 (speedup where "
                   "the plain method meets TOL)",
@@ -420,6 +491,7 @@ def bench_learned(args, rp, lib) -> int:
         "learned_meets_tol": sum(r["resid_rel"] <= tol for r in res),
         "plain_meets_tol": len(feas),
         "train_loss": [float(hist[0]), float(hist[-1])],
+        "cpu_baseline": cpu,
         "feasible_detail": [{"i": r["i"], "plain_ms": round(1e3 * r["plain_seconds"], 2),
                              "learned_ms": round(1e3 * r["seconds"], 2),
                              "plain_frank": r["plain_frank"], "k_pred": r["k_pred"],
@@ -544,6 +616,10 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-rows", type=int, default=16384,
                     help="rows of A in the CPU baseline's sample (bounds it to ~10-30 s)")
+    ap.add_argument("--cpu-learned", type=int, default=48,
+                    help="learned: held-out matrices the CPU baseline solves")
+    ap.add_argument("--cpu-rows-blocked", type=int, default=3000,
+                    help="blocked: rows of A in the CPU baseline's sample")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="run one plain device copy of this size first (HBM counter calibration)")
     ap.add_argument("--workload", default="fixed_rank",

@@ -183,30 +183,39 @@ symj_solve_kernel(int64_t n, const double* __restrict__ G, int64_t ldg, int nblo
   jac_out<B>(Hf, Jl, w, Jout, Hout, sig, flag);
 }
 
-// out (S x S) = op(A) * Bm with A, Bm in LDS (row-major, padded HP); thread
-// owns the R x R output block (rows R (tid / 16) .., columns R (tid % 16) ..),
-// R = S / 16.
-template <int S, bool TA>
-__device__ __forceinline__ void tile_mm(const double* __restrict__ A, const double* __restrict__ Bm,
-                                        double (&o)[S / 16][S / 16]) {
-  constexpr int R = S / 16, HP = S + 1;
-  const int i0 = (threadIdx.x >> 4) * R, j0 = (threadIdx.x & 15) * R;
+typedef double d4_t __attribute__((ext_vector_type(4)));
+
+// The wave's 16 x 16 quadrant (ti, tj) = (wave / 2, wave % 2) of the 32 x 32
+// product op(P) Q, P and Q in LDS (row-major, padded HP), on the FP64 matrix
+// cores: eight v_mfma_f64_16x16x4f64 steps along k.  srcA carries Q[k][n] and
+// srcB op(P)[m][k] (lane l: n or m = l % 16, k = 4 s + l / 16), so lane l's
+// accumulator register r holds C[16 ti + l % 16][16 tj + 4 r + l / 16].  Two
+// LDS reads per 16 multiply-adds a lane, where the register-blocked scalar
+// product read one per two.
+template <bool TA>
+__device__ __forceinline__ d4_t tile_mfma(const double* __restrict__ P,
+                                          const double* __restrict__ Q) {
+  constexpr int HP = 33;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int ti = wave >> 1, tj = wave & 1, li = lane & 15, lk = lane >> 4;
+  d4_t acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-  for (int a = 0; a < R; ++a)
-#pragma unroll
-    for (int b = 0; b < R; ++b) o[a][b] = 0.0;
-#pragma unroll 4
-  for (int u = 0; u < S; ++u) {
-    double av[R], bv[R];
-#pragma unroll
-    for (int a = 0; a < R; ++a) av[a] = TA ? A[u * HP + i0 + a] : A[(i0 + a) * HP + u];
-#pragma unroll
-    for (int b = 0; b < R; ++b) bv[b] = Bm[u * HP + j0 + b];
-#pragma unroll
-    for (int a = 0; a < R; ++a)
-#pragma unroll
-      for (int b = 0; b < R; ++b) o[a][b] += av[a] * bv[b];
+  for (int s = 0; s < 8; ++s) {
+    const int k = 4 * s + lk;
+    const double q = Q[k * HP + 16 * tj + li];
+    const double p = TA ? P[k * HP + 16 * ti + li] : P[(16 * ti + li) * HP + k];
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(q, p, acc, 0, 0, 0);
   }
+  return acc;
+}
+
+// Stores a wave's quadrant (the layout of tile_mfma) into the 32 x 32 tile C.
+__device__ __forceinline__ void tile_store(const d4_t& acc, double* __restrict__ C) {
+  constexpr int HP = 33;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int ti = wave >> 1, tj = wave & 1, li = lane & 15, lk = lane >> 4;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) C[(16 * ti + li) * HP + 16 * tj + 4 * r + lk] = acc[r];
 }
 
 // Apply round r's rotations (the apply role of symj_step_kernel): blocks
@@ -220,11 +229,10 @@ __device__ void jac_apply(int bid, int64_t n, const double* __restrict__ Gin,
                           int64_t ldv, int nblocks, int N, int round,
                           const double* __restrict__ Jall, int ntg, double* T, double* Ja,
                           double* Jb, double* X, int64_t* ra, int64_t* cb) {
-  constexpr int S = 2 * B, HP = S + 1, R = S / 16;
+  static_assert(B == 16, "the tile products are 32 x 32, one 16 x 16 quadrant a wave");
+  constexpr int S = 2 * B, HP = S + 1;
   const int pairs = N / 2;
   const int tid = threadIdx.x;
-  const int i0 = (tid >> 4) * R, j0 = (tid & 15) * R;
-  double o[R][R];
   if (bid < ntg) {
     int w1 = 0;
     while (bid >= pairs - w1) {
@@ -247,18 +255,11 @@ __device__ void jac_apply(int bid, int64_t n, const double* __restrict__ Gin,
       Jb[r * HP + c] = J2[r + c * S];
     }
     __syncthreads();
-    tile_mm<S, false>(T, Jb, o);  // X = T J_w2
-#pragma unroll
-    for (int a = 0; a < R; ++a)
-#pragma unroll
-      for (int b = 0; b < R; ++b) X[(i0 + a) * HP + j0 + b] = o[a][b];
+    tile_store(tile_mfma<false>(T, Jb), X);  // X = T J_w2
     __syncthreads();
-    tile_mm<S, true>(Ja, X, o);   // Y = J_w1^T X
-    __syncthreads();
-#pragma unroll
-    for (int a = 0; a < R; ++a)
-#pragma unroll
-      for (int b = 0; b < R; ++b) T[(i0 + a) * HP + j0 + b] = o[a][b];
+    // Y = J_w1^T X into T (T is not read again: its last reads were before
+    // the barrier above)
+    tile_store(tile_mfma<true>(Ja, X), T);
     __syncthreads();
     if (w1 == w2) {
       // diagonal pair tile: the upper triangle, mirrored
@@ -303,17 +304,10 @@ __device__ void jac_apply(int bid, int64_t n, const double* __restrict__ Gin,
     Jb[r * HP + c] = Jw[r + c * S];
   }
   __syncthreads();
-  double o2[R][R];
-  tile_mm<S, false>(T, Jb, o);
-  tile_mm<S, false>(X, Jb, o2);
+  const d4_t o1 = tile_mfma<false>(T, Jb), o2 = tile_mfma<false>(X, Jb);
   __syncthreads();
-#pragma unroll
-  for (int a = 0; a < R; ++a)
-#pragma unroll
-    for (int b = 0; b < R; ++b) {
-      T[(i0 + a) * HP + j0 + b] = o[a][b];
-      X[(i0 + a) * HP + j0 + b] = o2[a][b];
-    }
+  tile_store(o1, T);
+  tile_store(o2, X);
   __syncthreads();
   for (int e = tid; e < 2 * S * S; e += 256) {
     const int r = e % (2 * S), c = e / (2 * S);
