@@ -295,6 +295,15 @@ int rsvd_dev_lowrank_test_matrix(long long m, long long n, int r, const double *
 int rsvd_dev_sym_eig_psd(long long n, const double *dG, long long ldg, double *dEvals,
                          double *dEvecs, long long ldv, int *sweeps);
 
+/* QR_factorization_getQ / compact_QR_factorization (MKL/mvf.c:1214-1263) with
+ * the reference's own arithmetic, LAPACK dgeqrf + dorgqr: a blocked
+ * Householder QR of the device matrix Y (m x n, ldy, m >= n), Y overwritten
+ * by Q (LAPACK's reflectors and sign convention: R(i, i) = -sign(alpha_i)
+ * times the norm, so Q and R equal LAPACK's to rounding), R (n x n, ld n,
+ * upper; optional).  The default orthonormalisation of the library is
+ * CholeskyQR2 (positive diagonal of R) and falls back to this routine. */
+int rsvd_dev_householder_qr(double *dY, long long m, long long n, long long ldy, double *dR);
+
 /* ---- timing of the library's stream ---- */
 int rsvd_timer_start(void);
 /* Milliseconds since rsvd_timer_start, measured with HIP events on the

@@ -1429,6 +1429,17 @@ int rsvd_dev_lowrank_residual(const double* dA, long long m, long long n, long l
   return stream_sync(api.c);
 }
 
+int rsvd_dev_householder_qr(double* dY, long long m, long long n, long long ldy, double* dR) {
+  DevApi api;
+  if (api.rc) return api.rc;
+  if (!dY) return set_error("rsvd_dev_householder_qr: NULL Y");
+  if (m < 0 || n < 0 || m < n || ldy < (m > 1 ? m : 1))
+    return set_error("rsvd_dev_householder_qr: need m >= n >= 0 and ldy >= m (m=%lld n=%lld "
+                     "ldy=%lld)", m, n, ldy);
+  RSVD_TRY(householder_qr_f64(api.c, m, n, dY, ldy, dR));
+  return stream_sync(api.c);
+}
+
 int rsvd_dev_sym_eig_psd(long long n, const double* dG, long long ldg, double* dEvals,
                          double* dEvecs, long long ldv, int* sweeps) {
   DevApi api;

@@ -4,10 +4,10 @@
 //
 // Main path: CholeskyQR2.  G = Y^T Y on the matrix cores (split-K GEMM), FP64
 // blocked Cholesky G = R^T R fused with the triangular inverse (one launch per
-// 32-row panel), Q1 = Y R^-1 (GEMM), and the same once more on Q1; R = R2 R1.  Breakdown (a pivot below
-// tol * original diagonal, or non-finite) is reported back to the host and the
-// column-by-column Householder QR (geqr2 + org2r, LAPACK sign convention)
-// runs instead.
+// 32-row panel), Q1 = Y R^-1 (GEMM), and the same once more on Q1; R = R2 R1.
+// Breakdown (a pivot below tol * original diagonal, or non-finite) is
+// reported back to the host and the blocked Householder QR (geqrf + orgqr,
+// LAPACK sign convention, hhqr.hip) runs instead.
 #include <cmath>
 
 #include "rsvd_internal.h"
@@ -17,75 +17,7 @@ namespace rsvd {
 constexpr double kCholRelTol = 1e-13;   // pivot / original diagonal
 
 // ---------------------------------------------------------------- Householder
-// Reflector for column j (rows j..m-1) of Y, LAPACK dlarfg convention:
-// beta = -sign(alpha)*||x||, tau = (beta-alpha)/beta, v = x/(alpha-beta), v0=1.
-__global__ void __launch_bounds__(1024)
-hh_reflector_kernel(int64_t m, double* __restrict__ Y, int64_t ldy, int64_t j,
-                    double* __restrict__ tau) {
-  __shared__ double red[16];
-  __shared__ double sh_scale;
-  double* x = Y + j + j * ldy;
-  const int64_t len = m - j;
-  double s = 0;
-  for (int64_t i = 1 + threadIdx.x; i < len; i += 1024) s += x[i] * x[i];
-  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double sig = 0;
-    for (int w = 0; w < 16; ++w) sig += red[w];
-    double alpha = x[0];
-    if (sig == 0.0) {
-      tau[j] = 0.0;
-      sh_scale = 0.0;
-    } else {
-      double nrm = sqrt(alpha * alpha + sig);
-      double beta = (alpha >= 0) ? -nrm : nrm;
-      tau[j] = (beta - alpha) / beta;
-      sh_scale = 1.0 / (alpha - beta);
-      x[0] = beta;
-    }
-  }
-  __syncthreads();
-  const double sc = sh_scale;
-  if (sc != 0.0)
-    for (int64_t i = 1 + threadIdx.x; i < len; i += 1024) x[i] *= sc;
-}
-
-// Apply H_j = I - tau v v^T (v stored below the diagonal of column j, v0 = 1)
-// to columns c0.. of Y restricted to rows j..m-1.  One workgroup per column.
-__global__ void __launch_bounds__(256)
-hh_apply_kernel(int64_t m, double* __restrict__ Y, int64_t ldy, int64_t j, int64_t c0,
-                const double* __restrict__ tau, const double* __restrict__ V, int64_t ldv) {
-  __shared__ double red[4];
-  const int64_t col = c0 + blockIdx.x;
-  const double t = tau[j];
-  if (t == 0.0) return;
-  const double* v = V + j + j * ldv;
-  double* y = Y + j + col * ldy;
-  const int64_t len = m - j;
-  double s = 0;
-  for (int64_t i = threadIdx.x; i < len; i += 256) s += (i == 0 ? 1.0 : v[i]) * y[i];
-  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
-  __syncthreads();
-  const double w = t * (red[0] + red[1] + red[2] + red[3]);
-  for (int64_t i = threadIdx.x; i < len; i += 256) y[i] -= (i == 0 ? 1.0 : v[i]) * w;
-}
-
-// org2r: column j of Q = H_j e_j after the later reflectors were applied.
-__global__ void hh_form_col_kernel(int64_t m, double* __restrict__ Y, int64_t ldy, int64_t j,
-                                   const double* __restrict__ tau) {
-  const double t = tau[j];
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    double* q = Y + j * ldy;
-    if (i < j) q[i] = 0.0;
-    else if (i == j) q[i] = 1.0 - t;
-    else q[i] = -t * q[i];
-  }
-}
-
+// (blocked geqrf / orgqr: hhqr.hip)
 __global__ void copy_upper_kernel(int64_t n, const double* __restrict__ Y, int64_t ldy,
                                   double* __restrict__ R) {
   int64_t total = n * n;
@@ -96,21 +28,13 @@ __global__ void copy_upper_kernel(int64_t n, const double* __restrict__ Y, int64
   }
 }
 
-int householder_qr_f64(Context* c, int64_t m, int64_t n, double* Y, int64_t ldy, double* R) {
+int householder_qr_f64(Context* c, int64_t m, int64_t n, double* Y, int64_t ldy, double* R,
+                       double* vwork) {
   if (n == 0 || m == 0) return 0;
   if (m < n) return set_error("householder_qr: m=%lld < n=%lld", (long long)m, (long long)n);
   DBuf<double> tau;
   RSVD_TRY(tau.alloc((size_t)n));
-  for (int64_t j = 0; j < n; ++j) {
-    hipLaunchKernelGGL(hh_reflector_kernel, dim3(1), dim3(1024), 0, c->stream, m, Y, ldy, j,
-                       tau.p);
-    RSVD_LAUNCH_CHECK();
-    if (j + 1 < n) {
-      hipLaunchKernelGGL(hh_apply_kernel, dim3((unsigned)(n - j - 1)), dim3(256), 0, c->stream, m,
-                         Y, ldy, j, j + 1, (const double*)tau.p, (const double*)Y, ldy);
-      RSVD_LAUNCH_CHECK();
-    }
-  }
+  RSVD_TRY(geqrf_f64(c, m, n, Y, ldy, tau.p, vwork));
   if (R) {
     int blocks = (int)((n * n + 255) / 256);
     if (blocks > 4096) blocks = 4096;
@@ -118,24 +42,11 @@ int householder_qr_f64(Context* c, int64_t m, int64_t n, double* Y, int64_t ldy,
                        (const double*)Y, ldy, R);
     RSVD_LAUNCH_CHECK();
   }
-  return org2r_f64(c, m, n, Y, ldy, tau.p);
+  return orgqr_f64(c, m, n, Y, ldy, tau.p, vwork);
 }
 
 int org2r_f64(Context* c, int64_t m, int64_t n, double* Y, int64_t ldy, const double* tau) {
-  if (n <= 0 || m <= 0) return 0;
-  for (int64_t j = n - 1; j >= 0; --j) {
-    if (j + 1 < n) {
-      hipLaunchKernelGGL(hh_apply_kernel, dim3((unsigned)(n - j - 1)), dim3(256), 0, c->stream, m,
-                         Y, ldy, j, j + 1, tau, (const double*)Y, ldy);
-      RSVD_LAUNCH_CHECK();
-    }
-    int blocks = (int)((m + 255) / 256);
-    if (blocks > 1024) blocks = 1024;
-    hipLaunchKernelGGL(hh_form_col_kernel, dim3(blocks), dim3(256), 0, c->stream, m, Y, ldy, j,
-                       tau);
-    RSVD_LAUNCH_CHECK();
-  }
-  return 0;
+  return orgqr_f64(c, m, n, Y, ldy, tau, nullptr);
 }
 
 // ---------------------------------------------------------------- Cholesky + inverse
@@ -563,7 +474,7 @@ int orth_f64(Context* c, int64_t m, int64_t n, double* Y, int64_t ldy, double* R
   RSVD_TRY(read_flag(c, fail.p, &h));
   if (h) {
     if (used_fallback) *used_fallback = 1;
-    return householder_qr_f64(c, m, n, Y, ldy, R);
+    return householder_qr_f64(c, m, n, Y, ldy, R, W.p);
   }
   // pass 2: Q = Q1 R2^-1 into Y
   RSVD_TRY(cholqr_pass(c, m, n, W.p, m, Y, ldy, G.p, G2.p, Rinv.p, diag0.p, acc.p, fail.p));
@@ -572,7 +483,7 @@ int orth_f64(Context* c, int64_t m, int64_t n, double* Y, int64_t ldy, double* R
     // range(Q1) = range(Y): Householder on Q1, R = R_hh * R1
     if (used_fallback) *used_fallback = 1;
     RSVD_TRY(copy2d_f64(c, m, n, W.p, m, Y, ldy));
-    RSVD_TRY(householder_qr_f64(c, m, n, Y, ldy, R ? G2.p : nullptr));
+    RSVD_TRY(householder_qr_f64(c, m, n, Y, ldy, R ? G2.p : nullptr, W.p));
     if (R) RSVD_TRY(gemm_f64(c, 'N', 'N', n, n, n, 1.0, G2.p, n, G1.p, n, 0.0, R, n));
     return 0;
   }

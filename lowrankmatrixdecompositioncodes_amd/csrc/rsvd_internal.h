@@ -283,10 +283,22 @@ int coldot_f64(Context* c, int64_t L, int64_t n, const double* X, int64_t ldx, c
 // (optional) reports which path ran.
 int orth_f64(Context* c, int64_t m, int64_t n, double* Y, int64_t ldy, double* R,
              int* used_fallback);
-// Householder QR (geqr2 + org2r) in place, R optional.
-int householder_qr_f64(Context* c, int64_t m, int64_t n, double* Y, int64_t ldy, double* R);
-// org2r: the first n columns of Q = H_0 ... H_{n-1} in place of the reflectors
-// stored below the diagonal of Y (m x n, v_j(j) = 1 implied) with tau (device).
+// Householder QR in place (blocked geqrf + orgqr, LAPACK's reflectors and
+// sign convention), R (n x n, ld n) optional.  vwork (optional, m * min(n, 32)
+// doubles, device) holds the panel reflectors; allocated when null.
+int householder_qr_f64(Context* c, int64_t m, int64_t n, double* Y, int64_t ldy, double* R,
+                       double* vwork = nullptr);
+// Blocked Householder QR of Y (m x n, m >= n) in place: R in the upper
+// triangle, reflector j below the diagonal of column j (v_j(j) = 1 implied),
+// tau (n, device).  vwork as above.
+int geqrf_f64(Context* c, int64_t m, int64_t n, double* Y, int64_t ldy, double* tau,
+              double* vwork = nullptr);
+// The first n columns of Q = H_0 ... H_{n-1} in place of the reflectors
+// stored below the diagonal of Y (m x n, v_j(j) = 1 implied) with tau
+// (device): blocked, 32 columns a panel.
+int orgqr_f64(Context* c, int64_t m, int64_t n, double* Y, int64_t ldy, const double* tau,
+              double* vwork = nullptr);
+// orgqr_f64 with its own work buffer.
 int org2r_f64(Context* c, int64_t m, int64_t n, double* Y, int64_t ldy, const double* tau);
 // R = chol(G) (upper) and Rinv = R^-1 in one blocked pass (n x n, ld n, zero
 // below the diagonal).  G's trailing part is overwritten.  *ok = 0 when a

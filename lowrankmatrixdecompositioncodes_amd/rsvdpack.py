@@ -177,6 +177,8 @@ _SIGS = {
                                 C.c_longlong, C.c_double, C.c_void_p, C.c_longlong]),
     "rsvd_dev_sym_eig_psd": (C.c_int, [C.c_longlong, C.c_void_p, C.c_longlong, C.c_void_p,
                                        C.c_void_p, C.c_longlong, C.POINTER(C.c_int)]),
+    "rsvd_dev_householder_qr": (C.c_int, [C.c_void_p, C.c_longlong, C.c_longlong, C.c_longlong,
+                                          C.c_void_p]),
     "rsvd_dev_lowrank_test_matrix": (C.c_int, [C.c_longlong, C.c_longlong, C.c_int, C.c_void_p,
                                                C.c_double, C.c_ulonglong, C.c_void_p,
                                                C.c_longlong, C.c_void_p, C.c_longlong]),

@@ -101,6 +101,71 @@ def test_qr_householder_fallback_on_rank_deficient(gpu_lib):
     assert np.linalg.norm(Q @ (Q.T @ M) - M) < 1e-12 * np.linalg.norm(M)
 
 
+def _householder(rp, M, want_r=True):
+    m, n = M.shape
+    Y = rp.DeviceMatrix.from_host(M)
+    R = rp.DeviceMatrix(max(n, 1), max(n, 1)) if want_r else None
+    rc = rp.load().rsvd_dev_householder_qr(Y.ptr, m, n, m, R.ptr if R else None)
+    assert rc == 0, rp.last_error()
+    return Y.to_host(), (R.to_host()[:n, :n] if R else None)
+
+
+@pytest.mark.parametrize("shape", [(400, 30), (300, 100), (1000, 150), (97, 97), (64, 33),
+                                   (5, 1), (3000, 64), (70000, 40)])
+def test_householder_qr_matches_lapack(gpu_lib, shape):
+    """rsvd_dev_householder_qr is dgeqrf + dorgqr: Q and R equal LAPACK's
+    (numpy.linalg.qr) to rounding, signs included, for one panel, several
+    panels with a ragged last one, square, a single column, and a tall case
+    spread over hundreds of workgroups."""
+    m, n = shape
+    rng = np.random.default_rng(m + 7 * n)
+    M = rng.standard_normal((m, n))
+    Q, R = _householder(gpu_lib, M)
+    Ql, Rl = np.linalg.qr(M)
+    assert np.abs(R - Rl).max() <= 1e-13 * np.abs(Rl).max()
+    assert np.abs(Q - Ql).max() <= 1e-13 * np.sqrt(n)
+    assert np.abs(Q.T @ Q - np.eye(n)).max() <= 1e-14 * np.sqrt(m)
+
+
+def test_householder_qr_rank_deficient_and_zero_columns(gpu_lib):
+    """Exactly dependent and all-zero columns across panel boundaries: Q stays
+    orthonormal and spans M's columns (tau = 0 for a zero column, as dlarfg)."""
+    rng = np.random.default_rng(12)
+    M = rng.standard_normal((2000, 120))
+    M[:, 40] = M[:, 3]
+    M[:, 77] = 2.0 * M[:, 70] - M[:, 5]
+    M[:, 31] = 0.0
+    M[:, 100:] = M[:, :20] @ rng.standard_normal((20, 20))
+    Q, R = _householder(gpu_lib, M)
+    assert np.abs(Q.T @ Q - np.eye(120)).max() < 1e-13
+    assert np.linalg.norm(Q @ R - M) < 1e-13 * np.linalg.norm(M)
+    assert np.linalg.norm(Q @ (Q.T @ M) - M) < 1e-13 * np.linalg.norm(M)
+
+
+def test_householder_qr_c2_basis_time(gpu_lib):
+    """The fallback at the size the headline's basis has (65536 x 544):
+    orthonormal Q, Q R = M, and its time (printed)."""
+    import time
+    rp = gpu_lib
+    lib = rp.load()
+    m, n = 65536, 544
+    Y = rp.DeviceMatrix(m, n)
+    assert lib.rsvd_dev_gaussian(5, m, n, Y.ptr, m) == 0, rp.last_error()
+    M = Y.to_host()
+    R = rp.DeviceMatrix(n, n)
+    times = []
+    for _ in range(2):
+        Y2 = rp.DeviceMatrix.from_host(M)
+        lib.rsvd_sync()
+        t0 = time.perf_counter()
+        assert lib.rsvd_dev_householder_qr(Y2.ptr, m, n, m, R.ptr) == 0, rp.last_error()
+        times.append(time.perf_counter() - t0)
+    Q, Rh = Y2.to_host(), R.to_host()
+    print(f"householder QR {m}x{n}: {1e3 * min(times):.1f} ms")
+    assert np.abs(Q.T @ Q - np.eye(n)).max() < 1e-13
+    assert np.linalg.norm(Q @ Rh - M) < 1e-13 * np.linalg.norm(M)
+
+
 def test_qr_ill_conditioned(gpu_lib):
     rng = np.random.default_rng(6)
     U, _ = np.linalg.qr(rng.standard_normal((1000, 40)))
