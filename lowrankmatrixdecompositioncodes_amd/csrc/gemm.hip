@@ -213,6 +213,8 @@ __global__ void splitk_reduce_kernel(int64_t M, int64_t N, int splits, const T* 
        idx += (int64_t)gridDim.x * blockDim.x) {
     int64_t i = idx % M, j = idx / M;
     T s = T(0);
+    // loads issued ahead of the (in-order) adds
+#pragma unroll 8
     for (int z = 0; z < splits; ++z) s += W[(int64_t)z * total + idx];
     T* cp = C + i + j * ldc;
     if (beta != T(0)) s += beta * (*cp);
@@ -268,7 +270,8 @@ static int launch_gemm(Context* c, char ta, char tb, int64_t M, int64_t N, int64
   const int64_t target = 2LL * (c->num_cus > 0 ? c->num_cus : 256);
   if (!sumsq_partials && tiles < target && K >= 4 * 256) {
     int64_t maxs = K / 256;
-    if (maxs > 64) maxs = 64;
+    const int64_t cap = BN <= 32 ? 128 : 64;  // skinny tiles: more, shorter slices
+    if (maxs > cap) maxs = cap;
     splits = pick_k_slices(tiles, target, maxs);
   }
   int64_t kchunk = K;
@@ -357,6 +360,11 @@ static int gemm_any(Context* c, char ta, char tb, int64_t M, int64_t N, int64_t 
     RSVD_LAUNCH_CHECK();
     return 0;
   }
+  // N <= 32 (a product with a panel of 32 or fewer columns): a 128 x 32 tile,
+  // no half-empty 128 x 64 tile
+  if (N <= 32)
+    return launch_gemm<T, 128, 32, 16>(c, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc,
+                                       nullptr, tag);
   return launch_gemm<T, 128, 64, 16>(c, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc,
                                      nullptr, tag);
 }

@@ -32,9 +32,10 @@ int householder_qr_f64(Context* c, int64_t m, int64_t n, double* Y, int64_t ldy,
                        double* vwork) {
   if (n == 0 || m == 0) return 0;
   if (m < n) return set_error("householder_qr: m=%lld < n=%lld", (long long)m, (long long)n);
-  DBuf<double> tau;
+  DBuf<double> tau, Tall;
   RSVD_TRY(tau.alloc((size_t)n));
-  RSVD_TRY(geqrf_f64(c, m, n, Y, ldy, tau.p, vwork));
+  RSVD_TRY(Tall.alloc((size_t)((n + 31) / 32) * 32 * 32));
+  RSVD_TRY(geqrf_f64(c, m, n, Y, ldy, tau.p, vwork, Tall.p));
   if (R) {
     int blocks = (int)((n * n + 255) / 256);
     if (blocks > 4096) blocks = 4096;
@@ -42,7 +43,7 @@ int householder_qr_f64(Context* c, int64_t m, int64_t n, double* Y, int64_t ldy,
                        (const double*)Y, ldy, R);
     RSVD_LAUNCH_CHECK();
   }
-  return orgqr_f64(c, m, n, Y, ldy, tau.p, vwork);
+  return orgqr_f64(c, m, n, Y, ldy, tau.p, vwork, Tall.p);
 }
 
 int org2r_f64(Context* c, int64_t m, int64_t n, double* Y, int64_t ldy, const double* tau) {

@@ -290,14 +290,17 @@ int householder_qr_f64(Context* c, int64_t m, int64_t n, double* Y, int64_t ldy,
                        double* vwork = nullptr);
 // Blocked Householder QR of Y (m x n, m >= n) in place: R in the upper
 // triangle, reflector j below the diagonal of column j (v_j(j) = 1 implied),
-// tau (n, device).  vwork as above.
+// tau (n, device).  vwork as above.  Tall (optional, ceil(n / 32) * 32 * 32
+// doubles, device) receives each 32-column panel's block-reflector factor T
+// (H_j0 ... H_j0+31 = I - V T V^T), for orgqr_f64.
 int geqrf_f64(Context* c, int64_t m, int64_t n, double* Y, int64_t ldy, double* tau,
-              double* vwork = nullptr);
+              double* vwork = nullptr, double* Tall = nullptr);
 // The first n columns of Q = H_0 ... H_{n-1} in place of the reflectors
 // stored below the diagonal of Y (m x n, v_j(j) = 1 implied) with tau
-// (device): blocked, 32 columns a panel.
+// (device): blocked, 32 columns a panel; the panels' T from geqrf_f64 when
+// Tall is given, formed from V and tau otherwise.
 int orgqr_f64(Context* c, int64_t m, int64_t n, double* Y, int64_t ldy, const double* tau,
-              double* vwork = nullptr);
+              double* vwork = nullptr, const double* Tall = nullptr);
 // orgqr_f64 with its own work buffer.
 int org2r_f64(Context* c, int64_t m, int64_t n, double* Y, int64_t ldy, const double* tau);
 // R = chol(G) (upper) and Rinv = R^-1 in one blocked pass (n x n, ld n, zero
