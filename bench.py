@@ -311,6 +311,17 @@ def bench_blocked(args, rp, lib) -> int:
     n64 = args.steps if not prec_id else args.fp64_steps
     if n64 > 0:
         ms64, out64 = run(False, n64, args.warmup if not prec_id else 0)
+    # the FP64 plain method with its passes over A on rocBLAS's dgemm (a vendor
+    # reference for the FP64 side of the ratio)
+    ms64v = None
+    if n64 > 0 and not args.no_vendor:
+        lib.rsvd_set_fp64_vendor_gemm(1)
+        try:
+            ms64v, outv = run(False, 1, 1)
+        finally:
+            lib.rsvd_set_fp64_vendor_gemm(0)
+        for ptr in outv[2:]:
+            lib.rsvd_dev_free(ptr.value)
     breakdown = None
     if prec_id:
         ms, out = run(True, args.steps, args.warmup)
@@ -404,6 +415,9 @@ def bench_blocked(args, rp, lib) -> int:
         "fp64_plain": None if ms64 is None else {"seconds": round(ms64 / 1e3, 4),
                                                  "residual_rel": res64},
         "speedup_vs_fp64_plain": None if (ms64 is None or not prec_id) else round(ms64 / ms, 3),
+        "fp64_plain_rocblas_passes": None if ms64v is None else {"seconds": round(ms64v / 1e3, 4)},
+        "speedup_vs_fp64_rocblas_passes": None if (ms64v is None or not prec_id)
+        else round(ms64v / ms, 3),
         "error_vs_fp64": err,
         "kernel_ms_per_call": breakdown,
         "cpu_baseline": cpu,
@@ -613,6 +627,8 @@ def main() -> int:
     ap.add_argument("--q", type=int, default=2)
     ap.add_argument("--fp64-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-vendor", action="store_true",
+                    help="skip the FP64 plain run with its passes on rocBLAS dgemm")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-rows", type=int, default=16384,
                     help="rows of A in the CPU baseline's sample (bounds it to ~10-30 s)")
@@ -688,6 +704,16 @@ def main() -> int:
         ms64, ks64 = _timed(lib, rp, step64, n64, 1, ["apass_f64"])
         log(f"fp64 plain: {ms64:.2f} ms/step")
         s64 = S64.to_host()[:, 0]
+    # the same plain method with its passes over A on rocBLAS's dgemm: a
+    # vendor reference for the FP64 side of the ratio
+    ms64v = None
+    if n64 > 0 and not args.no_vendor:
+        lib.rsvd_set_fp64_vendor_gemm(1)
+        try:
+            ms64v, _ = _timed(lib, rp, step64, n64, 1, ["apass_f64"])
+        finally:
+            lib.rsvd_set_fp64_vendor_gemm(0)
+        log(f"fp64 plain, passes on rocBLAS dgemm: {ms64v:.2f} ms/step")
 
     # ---- mixed precision ----
     prec_id = {"bf16": 1, "bf16x3": 2, "fp8": 3, "bf16+x3": 4, "fp8+x3": 5,
@@ -807,6 +833,9 @@ def main() -> int:
             "ms_per_step": round(ms64, 3), "tflops": round(F / (ms64 * 1e-3) / 1e12, 3),
             "solves_per_s": round(1000.0 / ms64, 3)},
         "speedup_vs_fp64_plain": None if ms64 is None else round(ms64 / ms, 3),
+        "fp64_plain_rocblas_passes": None if ms64v is None else {
+            "ms_per_step": round(ms64v, 3), "tflops": round(F / (ms64v * 1e-3) / 1e12, 3)},
+        "speedup_vs_fp64_rocblas_passes": None if ms64v is None else round(ms64v / ms, 3),
         "error_vs_fp64": err,
         "falg_flops": F,
         "kernel_ms_per_step": breakdown,

@@ -304,6 +304,12 @@ int rsvd_dev_sym_eig_psd(long long n, const double *dG, long long ldg, double *d
  * CholeskyQR2 (positive diagonal of R) and falls back to this routine. */
 int rsvd_dev_householder_qr(double *dY, long long m, long long n, long long ldy, double *dR);
 
+/* Routes the large FP64 products of the plain path (the passes over A: 10+
+ * GFLOP, K >= 4096) through rocBLAS's dgemm (on = 1) instead of the library's
+ * FP64 matrix-core GEMM (on = 0, the default): a vendor reference to time the
+ * plain method against.  Results agree to rounding. */
+void rsvd_set_fp64_vendor_gemm(int on);
+
 /* ---- timing of the library's stream ---- */
 int rsvd_timer_start(void);
 /* Milliseconds since rsvd_timer_start, measured with HIP events on the

@@ -66,7 +66,7 @@ def build(verbose: bool = False) -> Path:
             print(log)
     tmp = LIB.with_suffix(".so.tmp")
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp),
-           *[str(o) for o, _ in results], "-lpthread"]
+           *[str(o) for o, _ in results], "-lrocblas", "-lpthread"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}{r.stderr}")

@@ -1495,6 +1495,12 @@ static void kstats_resolve(Context* c) {
   c->pending.clear();
 }
 
+void rsvd_set_fp64_vendor_gemm(int on) {
+  Api api;
+  if (api.rc) return;
+  api.c->fp64_vendor = on != 0;
+}
+
 void rsvd_kstats_enable(int on) {
   Api api;
   if (api.rc) return;

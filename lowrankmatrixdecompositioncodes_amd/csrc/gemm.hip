@@ -19,6 +19,10 @@
 //         i = 4*r + l/16     for v_mfma_f64_16x16x4f64
 // Long-K products with few output tiles are split along K into a workspace
 // and summed in a fixed order by a second kernel (deterministic).
+#include <climits>
+
+#include <rocblas/rocblas.h>
+
 #include "rsvd_internal.h"
 
 namespace rsvd {
@@ -346,12 +350,57 @@ __global__ void scale_kernel(int64_t M, int64_t N, T beta, T* C, int64_t ldc) {
   }
 }
 
+// With rsvd_set_fp64_vendor_gemm(1), the A-streaming FP64 products of the
+// plain path (Y = A X, Z = A^T Y, B^T = A^T Q: K in the thousands, 10+ GFLOP)
+// run on rocBLAS's dgemm instead of gemm_kernel: a vendor reference the
+// plain method can be timed against (off by default).  Never inside an
+// allocation-free call (rocBLAS takes its own workspace), nor for dimensions
+// beyond rocBLAS's 32-bit integers.
+static int vendor_f64(Context* c, int64_t M, int64_t N, int64_t K, int64_t lda, int64_t ldb,
+                      int64_t ldc, bool* use) {
+  *use = false;
+  if (!c->fp64_vendor || c->scratch_mode) return 0;
+  if (2.0 * (double)M * (double)N * (double)K < 1e10 || M * N < (int64_t(1) << 22) || K < 4096)
+    return 0;
+  for (int64_t v : {M, N, K, lda, ldb, ldc})
+    if (v > INT_MAX) return 0;
+  if (!c->blas) {
+    rocblas_handle h = nullptr;
+    if (rocblas_create_handle(&h) != rocblas_status_success)
+      return set_error("rocblas_create_handle failed");
+    if (rocblas_set_stream(h, c->stream) != rocblas_status_success) {
+      rocblas_destroy_handle(h);
+      return set_error("rocblas_set_stream failed");
+    }
+    c->blas = h;
+  }
+  *use = true;
+  return 0;
+}
+
 template <typename T>
 static int gemm_any(Context* c, char ta, char tb, int64_t M, int64_t N, int64_t K, T alpha,
                     const T* A, int64_t lda, const T* B, int64_t ldb, T beta, T* C, int64_t ldc,
                     const char* tag) {
   RSVD_TRY(check_dims(M, N, K, lda, ldb, ldc, ta, tb));
   if (M == 0 || N == 0) return 0;
+  if constexpr (sizeof(T) == 8) {
+    bool vendor = false;
+    if (K > 0) RSVD_TRY(vendor_f64(c, M, N, K, lda, ldb, ldc, &vendor));
+    if (vendor) {
+      KTimer kt(c, tag);
+      const bool tA = (ta == 'T' || ta == 't'), tB = (tb == 'T' || tb == 't');
+      const double a = alpha, b = beta;
+      const rocblas_status st = rocblas_dgemm(
+          (rocblas_handle)c->blas, tA ? rocblas_operation_transpose : rocblas_operation_none,
+          tB ? rocblas_operation_transpose : rocblas_operation_none, (rocblas_int)M,
+          (rocblas_int)N, (rocblas_int)K, &a, A, (rocblas_int)lda, B, (rocblas_int)ldb, &b, C,
+          (rocblas_int)ldc);
+      if (st != rocblas_status_success)
+        return set_error("rocblas_dgemm failed: %s", rocblas_status_to_string(st));
+      return 0;
+    }
+  }
   if (K == 0) {
     int blocks = (int)((M * N + 255) / 256);
     if (blocks > 4096) blocks = 4096;

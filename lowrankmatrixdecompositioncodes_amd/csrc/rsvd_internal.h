@@ -71,6 +71,10 @@ struct Context {
   char* scratch = nullptr;
   bool scratch_mode = false;
   std::vector<std::pair<size_t, size_t>> scratch_live;
+  // rsvd_set_fp64_vendor_gemm: the large FP64 products through rocBLAS
+  // (handle on `stream`, rocblas_handle, created on first use)
+  bool fp64_vendor = false;
+  void* blas = nullptr;
 };
 
 // Size of the static device scratch array behind the allocation-free entry

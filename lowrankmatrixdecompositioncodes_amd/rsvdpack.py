@@ -203,6 +203,7 @@ _SIGS = {
     "rsvd_timer_start": (C.c_int, []),
     "rsvd_timer_stop_ms": (C.c_double, []),
     "rsvd_kstats_enable": (None, [C.c_int]),
+    "rsvd_set_fp64_vendor_gemm": (None, [C.c_int]),
     "rsvd_kstats_reset": (None, []),
     "rsvd_kstats_select": (None, [C.c_char_p]),
     "rsvd_kstats_get": (C.c_int, [C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]),

@@ -61,6 +61,34 @@ def test_gemm_nt(gpu_lib, m, n, k):
     assert np.max(np.abs(C - ref)) <= 4 * k * 2.3e-16 * np.max(np.abs(A) @ np.abs(B.T))
 
 
+@pytest.mark.parametrize("ta", ["N", "T"])
+def test_fp64_vendor_gemm_switch(gpu_lib, ta):
+    """rsvd_set_fp64_vendor_gemm(1) sends a pass-sized FP64 product (8192 x 544
+    x 4096) to rocBLAS; both backends meet the FP64 error bound against
+    NumPy's product and agree with each other within it."""
+    rp = gpu_lib
+    lib = rp.load()
+    rng = np.random.default_rng(3 if ta == "N" else 4)
+    M, N, K = 8192, 544, 4096
+    A = rng.standard_normal((M, K) if ta == "N" else (K, M))
+    B = rng.standard_normal((K, N))
+    dA, dB = rp.DeviceMatrix.from_host(A), rp.DeviceMatrix.from_host(B)
+    outs = []
+    for vendor in (0, 1):
+        dC = rp.DeviceMatrix(M, N)
+        lib.rsvd_set_fp64_vendor_gemm(vendor)
+        try:
+            rp.dev_gemm(ta, "N", dA, dB, dC)
+        finally:
+            lib.rsvd_set_fp64_vendor_gemm(0)
+        outs.append(dC.to_host())
+    ref = (A if ta == "N" else A.T) @ B
+    bound = 4 * K * 2.3e-16 * np.max(np.abs(A if ta == "N" else A.T) @ np.abs(B))
+    for C in outs:
+        assert np.max(np.abs(C - ref)) <= bound
+    assert np.max(np.abs(outs[0] - outs[1])) <= bound
+
+
 def test_gemm_shape_mismatch_reports_error(gpu_lib):
     lib = gpu_lib.load()
     A, B, Cm = gpu_lib.to_mat(np.ones((3, 4))), gpu_lib.to_mat(np.ones((4, 5))), gpu_lib.empty_mat(3, 4)
