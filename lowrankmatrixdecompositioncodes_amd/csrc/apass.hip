@@ -628,15 +628,34 @@ __device__ __forceinline__ fp8x8_t pack_fp8x8(const float (&v)[8], float scale) 
 
 // scale_bits[c] = bits of max |A| over columns 128 c .. 128 c + 127 of A
 // (M x N, ld lda); non-negative floats order like their bit patterns.
+// max |A| over the 128-column chunk blockIdx.x and the row slice blockIdx.y
+// into scale_bits (float bits, atomicMax: the max of non-negative floats is
+// the max of their bit patterns, and max is order-independent).  16-byte
+// loads along the columns when A allows them, four columns in flight.
 __global__ void chunk_absmax_kernel(int64_t M, int64_t N, const float* __restrict__ A,
                                     int64_t lda, unsigned* __restrict__ scale_bits) {
   const int64_t c = blockIdx.x;
   const int64_t j0 = c * 128, j1 = (j0 + 128 < N) ? j0 + 128 : N;
-  const int64_t rs = (M + gridDim.y - 1) / gridDim.y;
+  const int64_t rs = (((M + gridDim.y - 1) / gridDim.y) + 3) & ~int64_t(3);
   const int64_t r0 = (int64_t)blockIdx.y * rs, r1 = (r0 + rs < M) ? r0 + rs : M;
   float mx = 0.0f;
-  for (int64_t j = j0; j < j1; ++j)
-    for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) mx = fmaxf(mx, fabsf(A[r + j * lda]));
+  if (r0 < r1) {
+    const bool vec = (lda & 3) == 0 && (reinterpret_cast<uintptr_t>(A) & 15) == 0;
+    const int64_t r4 = vec ? (r1 - r0) >> 2 : 0;  // float4 per column
+    for (int64_t j = j0; j < j1; j += 4) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (j + u >= j1) break;
+        const float* col = A + (j + u) * lda + r0;
+        for (int64_t q = threadIdx.x; q < r4; q += blockDim.x) {
+          const float4 v = *reinterpret_cast<const float4*>(col + 4 * q);
+          mx = fmaxf(mx, fmaxf(fmaxf(fabsf(v.x), fabsf(v.y)), fmaxf(fabsf(v.z), fabsf(v.w))));
+        }
+        for (int64_t r = r0 + 4 * r4 + threadIdx.x; r < r1; r += blockDim.x)
+          mx = fmaxf(mx, fabsf(A[r + (j + u) * lda]));
+      }
+    }
+  }
   for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
   if ((threadIdx.x & 63) == 0) atomicMax(scale_bits + c, __float_as_uint(mx));
 }
@@ -989,8 +1008,10 @@ static int column_scales(Context* c, int64_t M, int64_t N, const float* A, int64
   DBuf<unsigned> bits;
   RSVD_TRY(bits.alloc((size_t)nc));
   RSVD_HIP(hipMemsetAsync(bits.p, 0, sizeof(unsigned) * nc, c->stream));
-  int64_t gy = (M + 8191) / 8192;
-  if (gy > 64) gy = 64;
+  // slices of ~2048 rows: enough workgroups (and loads in flight) to stream A
+  // at HBM rate
+  int64_t gy = (M + 2047) / 2048;
+  if (gy > 256) gy = 256;
   if (gy < 1) gy = 1;
   hipLaunchKernelGGL(chunk_absmax_kernel, dim3((unsigned)nc, (unsigned)gy), dim3(256), 0,
                      c->stream, M, N, A, lda, bits.p);

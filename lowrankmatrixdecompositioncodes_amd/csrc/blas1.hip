@@ -40,6 +40,23 @@ static inline int grid_for(int64_t total) {
   return (int)b;
 }
 
+// Grid of the elementwise 2-D kernels: y over columns, x over rows (about
+// four elements a thread per column), so no thread divides by the row count.
+static inline dim3 grid2d(int64_t rows, int64_t cols) {
+  int64_t gx = (rows + 1023) / 1024;
+  if (gx > 64) gx = 64;
+  if (gx < 1) gx = 1;
+  int64_t gy = cols;
+  if (gy > 4096) gy = 4096;
+  if (gy < 1) gy = 1;
+  return dim3((unsigned)gx, (unsigned)gy);
+}
+
+#define RSVD_FOR_2D(rows, cols)                                                        \
+  for (int64_t j = blockIdx.y; j < (cols); j += gridDim.y)                             \
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < (rows);       \
+         i += (int64_t)gridDim.x * blockDim.x)
+
 int gaussian_f64(Context* c, uint64_t seed, int64_t rows, int64_t cols, int64_t col0,
                  int64_t rows_total, double* out, int64_t ld) {
   if (rows * cols == 0) return 0;
@@ -94,18 +111,13 @@ int stream_copy(Context* c, const void* src, void* dst, int64_t bytes) {
 template <typename S, typename D>
 __global__ void copy2d_kernel(int64_t rows, int64_t cols, const S* __restrict__ src, int64_t lds,
                               D* __restrict__ dst, int64_t ldd) {
-  int64_t total = rows * cols;
-  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
-       idx += (int64_t)gridDim.x * blockDim.x) {
-    int64_t i = idx % rows, j = idx / rows;
-    dst[i + j * ldd] = (D)src[i + j * lds];
-  }
+  RSVD_FOR_2D(rows, cols) dst[i + j * ldd] = (D)src[i + j * lds];
 }
 
 int copy2d_f64(Context* c, int64_t rows, int64_t cols, const double* src, int64_t lds, double* dst,
                int64_t ldd) {
   if (rows * cols == 0) return 0;
-  hipLaunchKernelGGL((copy2d_kernel<double, double>), dim3(grid_for(rows * cols)), dim3(256), 0,
+  hipLaunchKernelGGL((copy2d_kernel<double, double>), grid2d(rows, cols), dim3(256), 0,
                      c->stream, rows, cols, src, lds, dst, ldd);
   RSVD_LAUNCH_CHECK();
   return 0;
@@ -113,7 +125,7 @@ int copy2d_f64(Context* c, int64_t rows, int64_t cols, const double* src, int64_
 int copy2d_f32(Context* c, int64_t rows, int64_t cols, const float* src, int64_t lds, float* dst,
                int64_t ldd) {
   if (rows * cols == 0) return 0;
-  hipLaunchKernelGGL((copy2d_kernel<float, float>), dim3(grid_for(rows * cols)), dim3(256), 0,
+  hipLaunchKernelGGL((copy2d_kernel<float, float>), grid2d(rows, cols), dim3(256), 0,
                      c->stream, rows, cols, src, lds, dst, ldd);
   RSVD_LAUNCH_CHECK();
   return 0;
@@ -121,7 +133,7 @@ int copy2d_f32(Context* c, int64_t rows, int64_t cols, const float* src, int64_t
 int f32_to_f64(Context* c, int64_t rows, int64_t cols, const float* src, int64_t lds, double* dst,
                int64_t ldd) {
   if (rows * cols == 0) return 0;
-  hipLaunchKernelGGL((copy2d_kernel<float, double>), dim3(grid_for(rows * cols)), dim3(256), 0,
+  hipLaunchKernelGGL((copy2d_kernel<float, double>), grid2d(rows, cols), dim3(256), 0,
                      c->stream, rows, cols, src, lds, dst, ldd);
   RSVD_LAUNCH_CHECK();
   return 0;
@@ -129,7 +141,7 @@ int f32_to_f64(Context* c, int64_t rows, int64_t cols, const float* src, int64_t
 int f64_to_f32(Context* c, int64_t rows, int64_t cols, const double* src, int64_t lds, float* dst,
                int64_t ldd) {
   if (rows * cols == 0) return 0;
-  hipLaunchKernelGGL((copy2d_kernel<double, float>), dim3(grid_for(rows * cols)), dim3(256), 0,
+  hipLaunchKernelGGL((copy2d_kernel<double, float>), grid2d(rows, cols), dim3(256), 0,
                      c->stream, rows, cols, src, lds, dst, ldd);
   RSVD_LAUNCH_CHECK();
   return 0;
@@ -137,17 +149,12 @@ int f64_to_f32(Context* c, int64_t rows, int64_t cols, const double* src, int64_
 
 __global__ void fill2d_kernel(int64_t rows, int64_t cols, double v, double* __restrict__ dst,
                               int64_t ldd) {
-  int64_t total = rows * cols;
-  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
-       idx += (int64_t)gridDim.x * blockDim.x) {
-    int64_t i = idx % rows, j = idx / rows;
-    dst[i + j * ldd] = v;
-  }
+  RSVD_FOR_2D(rows, cols) dst[i + j * ldd] = v;
 }
 int fill2d_f64(Context* c, int64_t rows, int64_t cols, double v, double* dst, int64_t ldd) {
   if (rows * cols == 0) return 0;
-  hipLaunchKernelGGL(fill2d_kernel, dim3(grid_for(rows * cols)), dim3(256), 0, c->stream, rows,
-                     cols, v, dst, ldd);
+  hipLaunchKernelGGL(fill2d_kernel, grid2d(rows, cols), dim3(256), 0, c->stream, rows, cols, v,
+                     dst, ldd);
   RSVD_LAUNCH_CHECK();
   return 0;
 }
@@ -179,19 +186,18 @@ int transpose_f64(Context* c, int64_t rows, int64_t cols, const double* src, int
 
 __global__ void scale_cols_kernel(int64_t rows, int64_t cols, double* __restrict__ a, int64_t lda,
                                   const double* __restrict__ s, int invert) {
-  int64_t total = rows * cols;
-  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
-       idx += (int64_t)gridDim.x * blockDim.x) {
-    int64_t i = idx % rows, j = idx / rows;
-    double f = invert ? 1.0 / s[j] : s[j];
-    a[i + j * lda] *= f;
+  for (int64_t j = blockIdx.y; j < cols; j += gridDim.y) {
+    const double f = invert ? 1.0 / s[j] : s[j];
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < rows;
+         i += (int64_t)gridDim.x * blockDim.x)
+      a[i + j * lda] *= f;
   }
 }
 int scale_cols_f64(Context* c, int64_t rows, int64_t cols, double* a, int64_t lda,
                    const double* colscale, int invert) {
   if (rows * cols == 0) return 0;
-  hipLaunchKernelGGL(scale_cols_kernel, dim3(grid_for(rows * cols)), dim3(256), 0, c->stream,
-                     rows, cols, a, lda, colscale, invert);
+  hipLaunchKernelGGL(scale_cols_kernel, grid2d(rows, cols), dim3(256), 0, c->stream, rows, cols,
+                     a, lda, colscale, invert);
   RSVD_LAUNCH_CHECK();
   return 0;
 }
