@@ -229,33 +229,29 @@ __device__ __forceinline__ void chunk_store(float (*Xs)[kL + 1], const float (&p
 // staged in 64-row chunks.  Wave w takes rows 16 w .. 16 w + 15 of every
 // chunk (four k steps of 4) for the six upper 16 x 16 tiles; the waves'
 // partial tiles are added in wave order at the end (deterministic).
-__device__ void lds_gram(const float* __restrict__ X, int64_t R, int l, float (*Xs)[kL + 1],
-                         double (*G)[kLP]) {
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+// One staged chunk's contribution to the wave's six upper 16 x 16 Gram tiles
+// (wave w: chunk rows 16 w .. 16 w + 15, four k steps of 4).
+__device__ __forceinline__ void gram_chunk(const float (*Xs)[kL + 1], d4_t (&acc)[6]) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int li = lane & 15, lk = lane >> 4;
-  d4_t acc[6];
 #pragma unroll
-  for (int t = 0; t < 6; ++t) acc[t] = d4_t{0.0, 0.0, 0.0, 0.0};
-  float pre[kChunkLoads];
-  chunk_fetch(X, R, l, 0, pre);
-  for (int64_t r0 = 0; r0 < R; r0 += 64) {
-    __syncthreads();
-    chunk_store(Xs, pre);
-    __syncthreads();
-    if (r0 + 64 < R) chunk_fetch(X, R, l, r0 + 64, pre);  // in flight during the MFMAs
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
-      const int row = 16 * wave + 4 * s + lk;
-      const double v0 = (double)Xs[row][li], v1 = (double)Xs[row][16 + li],
-                   v2 = (double)Xs[row][32 + li];
-      acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(v0, v0, acc[0], 0, 0, 0);
-      acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(v0, v1, acc[1], 0, 0, 0);
-      acc[2] = __builtin_amdgcn_mfma_f64_16x16x4f64(v0, v2, acc[2], 0, 0, 0);
-      acc[3] = __builtin_amdgcn_mfma_f64_16x16x4f64(v1, v1, acc[3], 0, 0, 0);
-      acc[4] = __builtin_amdgcn_mfma_f64_16x16x4f64(v1, v2, acc[4], 0, 0, 0);
-      acc[5] = __builtin_amdgcn_mfma_f64_16x16x4f64(v2, v2, acc[5], 0, 0, 0);
-    }
+  for (int s = 0; s < 4; ++s) {
+    const int row = 16 * wave + 4 * s + lk;
+    const double v0 = (double)Xs[row][li], v1 = (double)Xs[row][16 + li],
+                 v2 = (double)Xs[row][32 + li];
+    acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(v0, v0, acc[0], 0, 0, 0);
+    acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(v0, v1, acc[1], 0, 0, 0);
+    acc[2] = __builtin_amdgcn_mfma_f64_16x16x4f64(v0, v2, acc[2], 0, 0, 0);
+    acc[3] = __builtin_amdgcn_mfma_f64_16x16x4f64(v1, v1, acc[3], 0, 0, 0);
+    acc[4] = __builtin_amdgcn_mfma_f64_16x16x4f64(v1, v2, acc[4], 0, 0, 0);
+    acc[5] = __builtin_amdgcn_mfma_f64_16x16x4f64(v2, v2, acc[5], 0, 0, 0);
   }
+}
+
+// G (both triangles) = the sum of the four waves' tiles, added in wave order.
+__device__ void gram_store(const d4_t (&acc)[6], double (*G)[kLP]) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int li = lane & 15, lk = lane >> 4;
   const int ti[6] = {0, 0, 0, 1, 1, 2}, tj[6] = {0, 1, 2, 1, 2, 2};
   for (int w = 0; w < 4; ++w) {
     __syncthreads();
@@ -274,6 +270,23 @@ __device__ void lds_gram(const float* __restrict__ X, int64_t R, int l, float (*
   __syncthreads();
 }
 
+__device__ void lds_gram(const float* __restrict__ X, int64_t R, int l, float (*Xs)[kL + 1],
+                         double (*G)[kLP]) {
+  d4_t acc[6];
+#pragma unroll
+  for (int t = 0; t < 6; ++t) acc[t] = d4_t{0.0, 0.0, 0.0, 0.0};
+  float pre[kChunkLoads];
+  chunk_fetch(X, R, l, 0, pre);
+  for (int64_t r0 = 0; r0 < R; r0 += 64) {
+    __syncthreads();
+    chunk_store(Xs, pre);
+    __syncthreads();
+    if (r0 + 64 < R) chunk_fetch(X, R, l, r0 + 64, pre);  // in flight during the MFMAs
+    gram_chunk(Xs, acc);
+  }
+  gram_store(acc, G);
+}
+
 // In-place upper Cholesky of G (l x l, upper triangle used) -> R in the upper
 // triangle (G = R^T R, zero below), then Ri = R^-1 (upper).  shift is added
 // to the diagonal first.  Returns through *ok (shared) whether every pivot
@@ -281,8 +294,8 @@ __device__ void lds_gram(const float* __restrict__ X, int64_t R, int l, float (*
 // Factorisation: unnormalised right-looking elimination, one barrier per
 // step: thread (row a = tid % 64, columns 12 (tid / 64) ..) keeps its 12
 // entries in registers, subtracts U_j,a U_j,c / U_j,j at step j and publishes
-// row j + 1 when it owns it.  Inverse: one thread per column, fully unrolled
-// back substitution in registers.
+// row j + 1 when it owns it.  Inverse: one thread per column, back
+// substitution into the LDS result.
 __device__ void lds_chol_inv(double (*G)[kLP], double (*Ri)[kLP], int l, double shift, int* ok) {
   __shared__ double piv[kL], inv[kL];
   const int tid = threadIdx.x, a = tid & 63, c0 = (tid >> 6) * 12;
@@ -326,19 +339,23 @@ __device__ void lds_chol_inv(double (*G)[kLP], double (*Ri)[kLP], int l, double 
     G[i][c] = v;
   }
   __syncthreads();
+  // column c of R^-1 by back substitution, the solved entries kept in Ri
+  // itself (registers would cap the occupancy at two workgroups a CU)
   if (tid < kL) {
     const int c = tid;
-    double x[kL];
-#pragma unroll
     for (int r = kL - 1; r >= 0; --r) {
-      double s = 0.0;
-#pragma unroll
-      for (int u = kL - 1; u > r; --u) s = fma(G[r][u], x[u], s);
-      const double ir = (r < l) ? inv[r] : 0.0;
-      x[r] = (r > c) ? 0.0 : ((r == c) ? ir : -s * ir);
+      double v = 0.0;
+      if (c < l && r <= c) {
+        if (r == c) {
+          v = inv[r];
+        } else {
+          double s = 0.0;
+          for (int u = r + 1; u <= c; ++u) s = fma(G[r][u], Ri[u][c], s);
+          v = -s * inv[r];
+        }
+      }
+      Ri[r][c] = v;
     }
-#pragma unroll
-    for (int r = 0; r < kL; ++r) Ri[r][c] = (c < l) ? x[r] : 0.0;
   }
   __syncthreads();
 }
@@ -396,7 +413,7 @@ __device__ void lds_rows_times(const float* X, int64_t R, int l, float (*Xs)[kL 
 // ---------------------------------------------------------------- orth
 // CholeskyQR2 of Y_i (FROM_Z: Z_i) per matrix, one workgroup each.
 template <bool FROM_Z>
-__global__ void __launch_bounds__(256, 2)
+__global__ void __launch_bounds__(256, 3)
 borth_kernel(const BMat* __restrict__ mats, const int* __restrict__ order, float* __restrict__ work,
              int* __restrict__ flags) {
   __shared__ float Xs[64][kL + 1];
@@ -428,19 +445,19 @@ __device__ __forceinline__ int circle(int pos, int r, int N) {
 // once (round-robin pairing) and publish it with the pair's two indices;
 // after a barrier the 2 x 2 blocks of H (pairs bi x bj: thread t takes bi in
 // {t / 16, t / 16 + 16}, bj in {t % 16, t % 16 + 16}) and the (row, pair)
-// entries of J (row t % 64, pairs t / 64 + 4 i) are updated in parallel into
-// the second H buffer -- fixed strides, no division by the runtime size.
+// entries of J (row t % 64, pairs t / 64 + 4 i) are updated in parallel, in
+// place (the pairs partition the indices, so the blocks are disjoint, and the
+// step's rotations were read before the barrier) -- fixed strides, no
+// division by the runtime size.
 // Sweeps until a sweep's rotations pass the loose stop of rotation() (the
 // outputs are fp32: eigenvalues good to ~1e-8 relative, eigenvectors of
 // separated eigenvalues to ~1e-8).
-__device__ void lds_jacobi(double (*H)[kLP], double (*H2)[kLP], double (*J)[kLP], int np,
-                           int* sig_flag) {
+__device__ void lds_jacobi(double (*H)[kLP], double (*J)[kLP], int np, int* sig_flag) {
   __shared__ double rc[kL / 2], rs[kL / 2];
   __shared__ int pp[kL / 2], qq[kL / 2];
   const int tid = threadIdx.x;
   const int half = np / 2;
   double (*Hc)[kLP] = H;
-  double (*Hn)[kLP] = H2;
   for (int e = tid; e < np * np; e += 256) J[e / np][e % np] = (e / np == e % np) ? 1.0 : 0.0;
   __syncthreads();
   // H is positive semidefinite (a Gram matrix): ||H||_F <= trace(H)
@@ -477,10 +494,10 @@ __device__ void lds_jacobi(double (*H)[kLP], double (*H2)[kLP], double (*J)[kLP]
         double y00 = ci * x00 - si * x10, y01 = ci * x01 - si * x11;
         double y10 = si * x00 + ci * x10, y11 = si * x01 + ci * x11;
         if (bi == bj && si != 0.0) y01 = y10 = 0.0;
-        Hn[pi][pj] = y00;
-        Hn[pi][qj] = y01;
-        Hn[qi][pj] = y10;
-        Hn[qi][qj] = y11;
+        Hc[pi][pj] = y00;
+        Hc[pi][qj] = y01;
+        Hc[qi][pj] = y10;
+        Hc[qi][qj] = y11;
       }
       static_assert(kL <= 64, "a J row per lane of a wave");
       for (int k = tid >> 6; k < half; k += 4) {
@@ -493,9 +510,6 @@ __device__ void lds_jacobi(double (*H)[kLP], double (*H2)[kLP], double (*J)[kLP]
         J[x][qk] = sk * jp + ck * jq;
       }
       __syncthreads();
-      double (*tmp)[kLP] = Hc;
-      Hc = Hn;
-      Hn = tmp;
     }
     if (sig & 2) atomicOr(sig_flag, 1);
     __syncthreads();
@@ -503,20 +517,15 @@ __device__ void lds_jacobi(double (*H)[kLP], double (*H2)[kLP], double (*J)[kLP]
     __syncthreads();
     if (!any) break;
   }
-  // leave the converged matrix in H
-  if (Hc != H) {
-    for (int e = tid; e < np * np; e += 256) H[e / np][e % np] = Hc[e / np][e % np];
-  }
   __syncthreads();
 }
 
-__global__ void __launch_bounds__(256, 2)
+__global__ void __launch_bounds__(256, 3)
 bfinal_kernel(const BMat* __restrict__ mats, const int* __restrict__ mat_order,
               float* __restrict__ work, float* __restrict__ U,
               double* __restrict__ S, float* __restrict__ V) {
   __shared__ float Xs[64][kL + 1];
   __shared__ double H[kL][kLP];
-  __shared__ double H2[kL][kLP];
   __shared__ double J[kL][kLP];
   __shared__ double sig[kL];
   __shared__ int order[kL];
@@ -533,7 +542,7 @@ bfinal_kernel(const BMat* __restrict__ mats, const int* __restrict__ mat_order,
     for (int e = tid; e < np; e += 256) H[l][e] = H[e][l] = 0.0;
   }
   __syncthreads();
-  lds_jacobi(H, H2, J, np, &flag);
+  lds_jacobi(H, J, np, &flag);
   // ascending order of the l real eigenvalues: rank by counting
   if (tid < l) {
     const double lam = H[tid][tid];
@@ -553,16 +562,17 @@ bfinal_kernel(const BMat* __restrict__ mats, const int* __restrict__ mat_order,
     S[d.s + tid] = s;
   }
   __syncthreads();
-  // selected eigenvectors (columns order[l - k + j]) into H2, 1 / sigma
+  // selected eigenvectors (columns order[l - k + j]) into H (its diagonal
+  // was read above), 1 / sigma
   for (int e = tid; e < kL * kL; e += 256) {
     const int c = e % kL, j = e / kL;
-    H2[c][j] = (c < l && j < k) ? J[c][order[l - k + j]] : 0.0;  // zero padding: finite
+    H[c][j] = (c < l && j < k) ? J[c][order[l - k + j]] : 0.0;  // zero padding: finite
   }
   if (tid < k) sig[tid] = 1.0 / sig[tid];
   __syncthreads();
   // U = Q Uhat_sel (m x k), V = B^T Uhat_sel Sigma^-1 (n x k)
-  lds_rows_times(Q, d.m, l, Xs, H2, k, nullptr, U + d.u, d.m);
-  lds_rows_times(Bt, d.n, l, Xs, H2, k, sig, V + d.v, d.n);
+  lds_rows_times(Q, d.m, l, Xs, H, k, nullptr, U + d.u, d.m);
+  lds_rows_times(Bt, d.n, l, Xs, H, k, sig, V + d.v, d.n);
 }
 
 __global__ void bclear_kernel(int* __restrict__ p) {
