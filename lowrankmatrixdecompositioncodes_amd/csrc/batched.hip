@@ -22,12 +22,15 @@
 // Sketch widths are padded to 48 columns (three 16-wide tiles; l_i = k_i + p
 // <= 48), padding columns are zero throughout.  The pass kernels have
 // variants (rows per wave RT = 1, 2, 4) chosen per batch layout by timing
-// them on the batch itself on the first call (the autotuner); the choice is
-// cached with the graph.
+// them on the batch itself on the first call (the autotuner), together with a
+// variant that splits each item's k range over the four waves of a workgroup;
+// the choice is cached with the graph.
 #include <hip/hip_bf16.h>
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -112,12 +115,16 @@ btile_kernel(const BMat* __restrict__ mats, const BItem* __restrict__ items,
 // One wave per item: rows [16 RT tile, 16 RT (tile + 1)) of op(A_i) (M x K)
 // times the tiled skinny operand (K x 48) -> C (M x 48, ld M) in fp32.
 // TA: op(A) = A^T (M = n, K = m, k-contiguous); else M = m, K = n.
-template <bool TA, int RT>
+// KS: one item per workgroup, its k blocks split over the four waves (four
+// times the waves in flight for few, long items -- op(A) = A^T of a tall
+// matrix); the partial products are added in wave order through LDS.
+template <bool TA, int RT, bool KS = false>
 __global__ void __launch_bounds__(256)
 bpass_kernel(const BMat* __restrict__ mats, const BItem* __restrict__ items, int nitems,
              const float* __restrict__ A, float* __restrict__ work,
              const bf16x8_t* __restrict__ tiles) {
-  const int wid = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int wave = threadIdx.x >> 6;
+  const int wid = KS ? (int)blockIdx.x : (int)blockIdx.x * 4 + wave;
   if (wid >= nitems) return;
   const int lane = threadIdx.x & 63;
   const BItem it = items[wid];
@@ -137,7 +144,8 @@ bpass_kernel(const BMat* __restrict__ mats, const BItem* __restrict__ items, int
 #pragma unroll
     for (int t = 0; t < kNT; ++t) acc[r][t] = f4v_t{0.f, 0.f, 0.f, 0.f};
 
-  for (int64_t kb = 0; kb < KB; ++kb) {
+  const int64_t kb_lo = KS ? KB * wave / 4 : 0, kb_hi = KS ? KB * (wave + 1) / 4 : KB;
+  for (int64_t kb = kb_lo; kb < kb_hi; ++kb) {
     bf16x8_t sh[kNT], sl[kNT];
 #pragma unroll
     for (int t = 0; t < kNT; ++t) {
@@ -183,6 +191,23 @@ bpass_kernel(const BMat* __restrict__ mats, const BItem* __restrict__ items, int
         acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(sl[t], ah, acc[r][t], 0, 0, 0);
       }
     }
+  }
+  if constexpr (KS) {
+    __shared__ f4v_t part[3][RT][kNT][64];
+    if (wave > 0) {
+#pragma unroll
+      for (int r = 0; r < RT; ++r)
+#pragma unroll
+        for (int t = 0; t < kNT; ++t) part[wave - 1][r][t][lane] = acc[r][t];
+    }
+    __syncthreads();
+    if (wave > 0) return;
+#pragma unroll
+    for (int w = 0; w < 3; ++w)
+#pragma unroll
+      for (int r = 0; r < RT; ++r)
+#pragma unroll
+        for (int t = 0; t < kNT; ++t) acc[r][t] += part[w][r][t][lane];
   }
   // D[n][m]: lane holds n = 16 t + 4 (lane / 16) + rr, m = lane % 16
 #pragma unroll
@@ -601,7 +626,7 @@ struct Plan {
   DBuf<int> order;           // by_cost on the device (per-matrix kernels)
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  double tuned_ms[2][3] = {{0, 0, 0}, {0, 0, 0}};
+  double tuned_ms[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
   double stage_ms[4] = {0, 0, 0, 0};  // orth of Y, orth of Z, final solve, tiling (one launch)
   ~Plan() {
     if (exec) (void)hipGraphExecDestroy(exec);
@@ -619,7 +644,13 @@ std::map<std::vector<int64_t>, std::unique_ptr<Plan>>& g_plans =
     *new std::map<std::vector<int64_t>, std::unique_ptr<Plan>>();
 std::string g_last_config;
 
-std::vector<BItem> pass_items(const Plan& P, bool ta, int rt) {
+// rt: rows per wave 1, 2 or 4 (16 rt rows an item, four items a workgroup),
+// or kRtSplitK: 64-row items, one a workgroup, k split over its waves.
+constexpr int kRtSplitK = 5;
+int rt_rows(int rt) { return rt == kRtSplitK ? 4 : rt; }
+
+std::vector<BItem> pass_items(const Plan& P, bool ta, int rt_code) {
+  const int rt = rt_rows(rt_code);
   std::vector<BItem> v;
   for (int i : P.by_cost) {
     const BMat& d = P.host[i];
@@ -652,18 +683,20 @@ int upload_items(Context* c, const std::vector<BItem>& v, DBuf<BItem>& buf, int*
 
 int launch_pass(Context* c, Plan& P, bool ta, int rt, const BItem* items, int nitems) {
   if (nitems == 0) return 0;
-  const dim3 grid((unsigned)((nitems + 3) / 4));
-#define RSVD_BPASS(TA_, RT_)                                                                 \
-  hipLaunchKernelGGL((bpass_kernel<TA_, RT_>), grid, dim3(256), 0, c->stream, P.mats.p, items, \
-                     nitems, P.A, P.work.p, (const bf16x8_t*)P.tiles.p)
+  const dim3 grid((unsigned)(rt == kRtSplitK ? nitems : (nitems + 3) / 4));
+#define RSVD_BPASS(TA_, RT_, KS_)                                                        \
+  hipLaunchKernelGGL((bpass_kernel<TA_, RT_, KS_>), grid, dim3(256), 0, c->stream, P.mats.p, \
+                     items, nitems, P.A, P.work.p, (const bf16x8_t*)P.tiles.p)
   if (ta) {
-    if (rt == 1) RSVD_BPASS(true, 1);
-    else if (rt == 2) RSVD_BPASS(true, 2);
-    else RSVD_BPASS(true, 4);
+    if (rt == 1) RSVD_BPASS(true, 1, false);
+    else if (rt == 2) RSVD_BPASS(true, 2, false);
+    else if (rt == 4) RSVD_BPASS(true, 4, false);
+    else RSVD_BPASS(true, 4, true);
   } else {
-    if (rt == 1) RSVD_BPASS(false, 1);
-    else if (rt == 2) RSVD_BPASS(false, 2);
-    else RSVD_BPASS(false, 4);
+    if (rt == 1) RSVD_BPASS(false, 1, false);
+    else if (rt == 2) RSVD_BPASS(false, 2, false);
+    else if (rt == 4) RSVD_BPASS(false, 4, false);
+    else RSVD_BPASS(false, 4, true);
   }
 #undef RSVD_BPASS
   RSVD_LAUNCH_CHECK();
@@ -723,15 +756,25 @@ int enqueue_all(Context* c, Plan& P) {
 // Autotuner: time each RT variant of both pass kernels on this batch (after
 // one untimed run so the operands are real) and keep the fastest.
 int autotune(Context* c, Plan& P) {
-  static const int kRT[3] = {1, 2, 4};
+  static const int kRT[4] = {1, 2, 4, kRtSplitK};
+  // RSVD_BATCHED_PASS = "nn,tn" (each 1, 2, 4 or kRtSplitK) fixes the pass
+  // variants instead of timing them (diagnostics and tests)
+  int fnn = 0, ftn = 0;
+  const char* force = getenv("RSVD_BATCHED_PASS");
+  auto known = [](int v) { return v == 1 || v == 2 || v == 4 || v == kRtSplitK; };
+  const bool forced = force && sscanf(force, "%d,%d", &fnn, &ftn) == 2 && known(fnn) && known(ftn);
   hipEvent_t e0, e1;
   RSVD_HIP(hipEventCreate(&e0));
   RSVD_HIP(hipEventCreate(&e1));
   int rc = 0;
-  for (int ta = 0; ta < 2 && rc == 0; ++ta) {
+  if (forced) {
+    P.rt_nn = fnn;
+    P.rt_tn = ftn;
+  }
+  for (int ta = 0; ta < 2 && rc == 0 && !forced; ++ta) {
     float best = 1e30f;
     int best_rt = 1;
-    for (int v = 0; v < 3 && rc == 0; ++v) {
+    for (int v = 0; v < 4 && rc == 0; ++v) {
       const int rt = kRT[v];
       DBuf<BItem> items;
       int n = 0;
@@ -880,12 +923,14 @@ int fixed_rank_batched(Context* c, int count, const int* m, const int* n, const 
     if (ce != hipSuccess) return set_error("batched: graph capture failed: %s", hipGetErrorString(ce));
     P->graph = g;
     RSVD_HIP(hipGraphInstantiate(&P->exec, g, nullptr, nullptr, 0));
-    char buf[320];
+    char buf[512];
     snprintf(buf, sizeof buf,
-             "rt_nn=%d rt_tn=%d (NN ms %.3f/%.3f/%.3f, TN ms %.3f/%.3f/%.3f for RT 1/2/4); "
+             "rt_nn=%d rt_tn=%d (NN ms %.3f/%.3f/%.3f/%.3f, TN ms %.3f/%.3f/%.3f/%.3f for RT "
+             "1/2/4/4 with k split over 4 waves (%d)); "
              "one launch each: orth Y %.3f ms, orth Z %.3f ms, final %.3f ms, tile %.3f ms",
              P->rt_nn, P->rt_tn, P->tuned_ms[0][0], P->tuned_ms[0][1], P->tuned_ms[0][2],
-             P->tuned_ms[1][0], P->tuned_ms[1][1], P->tuned_ms[1][2], P->stage_ms[0],
+             P->tuned_ms[0][3], P->tuned_ms[1][0], P->tuned_ms[1][1], P->tuned_ms[1][2],
+             P->tuned_ms[1][3], kRtSplitK, P->stage_ms[0],
              P->stage_ms[1], P->stage_ms[2], P->stage_ms[3]);
     g_last_config = buf;
     g_plans[key] = std::move(np);

@@ -74,6 +74,31 @@ def test_batched_q1_and_orthonormal_factors(gpu_lib):
         assert np.all(s > 0)
 
 
+@pytest.mark.parametrize("variant", ["1,1", "2,2", "4,4", "5,5"])
+def test_batched_pass_variants_agree(gpu_lib, batch, monkeypatch, variant):
+    """Every pass variant the autotuner chooses from (rows per wave 1 / 2 / 4,
+    and 5 = four rows with the k range split over the workgroup's waves),
+    fixed through RSVD_BATCHED_PASS, gives the tuned run's singular values to
+    the fp32 level."""
+    lib = gpu_lib.load()
+    lib.rsvd_release_cache()
+    assert batch.run(q=2, seed=900) == 0
+    want = [x[1] for x in batch.results()]
+    monkeypatch.setenv("RSVD_BATCHED_PASS", variant)
+    lib.rsvd_release_cache()
+    try:
+        assert batch.run(q=2, seed=900) == 0
+        cfg = gpu_lib.batched_last_config()
+        got = [x[1] for x in batch.results()]
+    finally:
+        monkeypatch.delenv("RSVD_BATCHED_PASS")
+        lib.rsvd_release_cache()
+    nn, tn = variant.split(",")
+    assert f"rt_nn={nn} rt_tn={tn}" in cfg, cfg
+    worst = max(np.abs(g - w).max() for g, w in zip(got, want))
+    assert worst < 2e-6, worst
+
+
 def test_batched_rejects_bad_sizes(gpu_lib):
     b = gpu_lib.Batch([100, 100], [60, 60], [45, 10], 8)  # k + p = 53 > 48
     with pytest.raises(gpu_lib.RsvdError):
