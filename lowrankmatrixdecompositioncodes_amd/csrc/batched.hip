@@ -570,10 +570,11 @@ bfinal_kernel(const BMat* __restrict__ mats, const int* __restrict__ mat_order,
   lds_jacobi(H, J, np, &flag);
   // ascending order of the l real eigenvalues: rank by counting
   if (tid < l) {
-    const double lam = H[tid][tid];
+    // NaN ranks as +Inf, ties by index: the ranks are a permutation
+    const double lam = isnan(H[tid][tid]) ? INFINITY : H[tid][tid];
     int rk = 0;
     for (int j = 0; j < l; ++j) {
-      const double o = H[j][j];
+      const double o = isnan(H[j][j]) ? INFINITY : H[j][j];
       rk += (o < lam) || (o == lam && j < tid);
     }
     order[rk] = tid;
@@ -582,7 +583,7 @@ bfinal_kernel(const BMat* __restrict__ mats, const int* __restrict__ mat_order,
   // the last k in ascending order: output j <- eigenpair order[l - k + j]
   if (tid < k) {
     const int e = order[l - k + tid];
-    const double s = sqrt(fmax(H[e][e], 0.0));
+    const double s = H[e][e] < 0.0 ? 0.0 : sqrt(H[e][e]);  // NaN stays NaN
     sig[tid] = s;
     S[d.s + tid] = s;
   }

@@ -202,6 +202,35 @@ int scale_cols_f64(Context* c, int64_t rows, int64_t cols, double* a, int64_t ld
   return 0;
 }
 
+__global__ void fill_int_kernel(int* __restrict__ p, int v) {
+  if (threadIdx.x == 0) *p = v;
+}
+
+// *flag |= 1 when any of x[0 .. n) is NaN or infinite.
+__global__ void nonfinite_kernel(int64_t n, const double* __restrict__ x, int* __restrict__ flag) {
+  bool bad = false;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    bad = bad || !isfinite(x[i]);
+  if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr(flag, 1);
+}
+
+int check_finite_f64(Context* c, const double* x, int64_t n, const char* what) {
+  if (n <= 0) return 0;
+  DBuf<int> flag;
+  RSVD_TRY(flag.alloc(1));
+  hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(64), 0, c->stream, flag.p, 0);
+  RSVD_LAUNCH_CHECK();
+  hipLaunchKernelGGL(nonfinite_kernel, dim3(grid_for(n)), dim3(256), 0, c->stream, n, x, flag.p);
+  RSVD_LAUNCH_CHECK();
+  int h = 0;
+  RSVD_TRY(read_small(c, &h, flag.p, sizeof h));
+  if (h)
+    return set_error("%s: non-finite singular values (the input holds NaN or Inf); the outputs "
+                     "are not usable", what);
+  return 0;
+}
+
 // Sum of squares, deterministic: fixed grid of partial sums, then one block.
 template <bool DIFF>
 __global__ void sumsq_stage1(int64_t rows, int64_t cols, const double* __restrict__ a,

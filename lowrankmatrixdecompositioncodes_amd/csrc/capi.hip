@@ -87,6 +87,16 @@ int stream_sync(Context* c) {
 
 bool valid(const mat* M) { return M && M->nrows >= 0 && M->ncols >= 0 && (M->d || !M->nrows || !M->ncols); }
 
+// The reference returns whatever a NaN or Inf in M propagates to; so does
+// this host API, and it also leaves the reason in rsvd_last_error.
+static void flag_nonfinite(const std::vector<double>& sv, const char* what) {
+  for (double x : sv)
+    if (!std::isfinite(x)) {
+      set_error("%s: non-finite singular values (the input holds NaN or Inf)", what);
+      return;
+    }
+}
+
 mat* diag_mat(const std::vector<double>& s, int k) {
   mat* S = matrix_new(k, k);
   for (int i = 0; i < k; ++i) S->d[(size_t)i * k + i] = s[i];
@@ -532,6 +542,7 @@ void low_rank_svd_rand_decomp_fixed_rank(mat* M, int k, int p, int vnum, int q, 
   *U = Uh;
   *S = diag_mat(sv, k);
   *V = Vh;
+  flag_nonfinite(sv, "low_rank_svd_rand_decomp_fixed_rank");
 }
 
 void low_rank_svd_blockrand_decomp_fixed_rank_or_prec(mat* M, int k, int p, double TOL, int vnum,
@@ -571,6 +582,7 @@ void low_rank_svd_blockrand_decomp_fixed_rank_or_prec(mat* M, int k, int p, doub
   *U = Uh;
   *S = diag_mat(sv, fr);
   *V = Vh;
+  flag_nonfinite(sv, "low_rank_svd_blockrand_decomp_fixed_rank_or_prec");
 }
 
 static void qb_host(mat* M, int kstep, int nstep, double TOL, int q, int s, int* frank, mat** Q,
@@ -996,7 +1008,7 @@ int rsvd_dev_fixed_rank(const double* dA, long long m, long long n, long long ld
   om.ld = n;
   RSVD_TRY(fixed_rank_f64(api.c, dA, m, n, lda, k, p, vnum, q, s, om, dU, ldu, dS, dV, ldv,
                           nullptr));
-  return stream_sync(api.c);
+  return check_finite_f64(api.c, dS, k, "rsvd_dev_fixed_rank");
 }
 
 int rsvd_dev_fixed_rank_noalloc(const double* dA, long long m, long long n, long long lda, int k,
@@ -1021,7 +1033,7 @@ int rsvd_dev_fixed_rank_noalloc(const double* dA, long long m, long long n, long
   ScratchScope scope(api.c);
   RSVD_TRY(fixed_rank_f64(api.c, dA, m, n, lda, k, p, vnum, q, s, om, dU, ldu, dS, dV, ldv,
                           nullptr));
-  return stream_sync(api.c);
+  return check_finite_f64(api.c, dS, k, "rsvd_dev_fixed_rank_noalloc");
 }
 
 long long rsvd_qb_max_cols(long long m, long long n, int kstep, int nstep, int is_new) {
@@ -1063,6 +1075,7 @@ int rsvd_dev_blockrand(const double* dA, long long m, long long n, long long lda
   BlockrandOut out;
   RSVD_TRY(blockrand_f64(api.c, dA, m, n, lda, k, p, TOL, vnum, kstep, q, s, om, &out));
   const int fr = out.frank;
+  RSVD_TRY(check_finite_f64(api.c, out.S.p, fr, "rsvd_dev_blockrand"));
   OutBufs ob;
   RSVD_TRY(ob.alloc(sizeof(double) * (size_t)m * fr, sizeof(double) * (size_t)fr,
                     sizeof(double) * (size_t)n * fr));
@@ -1373,7 +1386,7 @@ int rsvd_dev_fixed_rank_mp(const float* dA, long long m, long long n, long long 
   if (api.rc) return api.rc;
   RSVD_TRY(fixed_rank_mp(api.c, dA, m, n, lda, k, p, vnum, q, s, seed, dOmega, precision, dU,
                          ldu, dS, dV, ldv, nullptr));
-  return stream_sync(api.c);
+  return check_finite_f64(api.c, dS, k, "rsvd_dev_fixed_rank_mp");
 }
 
 int rsvd_dev_randqb_mp(const float* dA, long long m, long long n, long long lda, int kstep,
@@ -1405,6 +1418,7 @@ int rsvd_dev_blockrand_mp(const float* dA, long long m, long long n, long long l
   RSVD_TRY(blockrand_mp(api.c, dA, m, n, lda, k, p, TOL, vnum, kstep, q, s, seed, dRN, precision,
                         &out));
   const int fr = out.frank;
+  RSVD_TRY(check_finite_f64(api.c, out.S.p, fr, "rsvd_dev_blockrand_mp"));
   OutBufs ob;
   RSVD_TRY(ob.alloc(sizeof(float) * (size_t)m * fr, sizeof(double) * (size_t)fr,
                     sizeof(float) * (size_t)n * fr));

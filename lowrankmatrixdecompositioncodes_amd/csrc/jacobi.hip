@@ -504,10 +504,13 @@ __global__ void rank_kernel(int64_t n, const double* __restrict__ v, int descend
                             int64_t* __restrict__ rank) {
   for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n;
        j += (int64_t)gridDim.x * blockDim.x) {
-    const double x = v[j];
+    // NaN ranks as +Inf and ties go by index: a strict total order, so the
+    // ranks are a permutation whatever the values (a NaN would otherwise
+    // compare false both ways and two entries would share a rank)
+    const double x = isnan(v[j]) ? INFINITY : v[j];
     int64_t r = 0;
     for (int64_t i = 0; i < n; ++i) {
-      const double y = v[i];
+      const double y = isnan(v[i]) ? INFINITY : v[i];
       bool before = descending ? (y > x) : (y < x);
       if (before || (y == x && i < j)) ++r;
     }

@@ -52,7 +52,8 @@ int symmetrize_upper_f64(Context* c, int64_t n, double* G) {
 __global__ void sqrt_kernel(int64_t n, const double* __restrict__ in, double* __restrict__ out) {
   for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n;
        j += (int64_t)gridDim.x * blockDim.x)
-    out[j] = sqrt(fmax(in[j], 0.0));  // a PSD eigenvalue may round to -eps
+    // a PSD eigenvalue may round to -eps; a NaN stays NaN (fmax would hide it)
+    out[j] = in[j] < 0.0 ? 0.0 : sqrt(in[j]);
 }
 
 static int grid1(int64_t n) {

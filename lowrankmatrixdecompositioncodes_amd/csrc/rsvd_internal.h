@@ -265,6 +265,9 @@ int scale_cols_f64(Context* c, int64_t rows, int64_t cols, double* a, int64_t ld
                    const double* colscale, int invert);
 int copy2d_f32(Context* c, int64_t rows, int64_t cols, const float* src, int64_t lds, float* dst,
                int64_t ldd);
+// Error (set_error, "<what>: non-finite singular values ...") when any of
+// x[0 .. n) (device) is NaN or infinite; synchronises.
+int check_finite_f64(Context* c, const double* x, int64_t n, const char* what);
 // out[j] = sqrt(in[j])
 int sqrt_f64(Context* c, int64_t n, const double* in, double* out);
 int f32_to_f64(Context* c, int64_t rows, int64_t cols, const float* src, int64_t lds, double* dst,

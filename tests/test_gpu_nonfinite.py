@@ -1,0 +1,84 @@
+"""Non-finite input (NaN or Inf in A).
+
+The reference has no status codes: a NaN or Inf in M propagates into its
+outputs without notice (MKL/rra.c:73-234, 239-381).  Here every
+decomposition returns (bounded loops: Jacobi sweeps, TOL steps and the
+CholeskyQR fallback all have fixed limits), the host API still hands back its
+outputs and leaves the reason in rsvd_last_error, the device API returns an
+error code, and in a batch a NaN stays in its own matrix.
+"""
+import ctypes as C
+import time
+
+import numpy as np
+import pytest
+
+from tests.helpers import lowrank_matrix
+
+pytestmark = pytest.mark.gpu
+
+
+def _matrix(seed=3, m=300, n=200):
+    return lowrank_matrix(m, n, np.exp(-np.arange(n) / 10.0), seed=seed)
+
+
+def test_host_fixed_rank_reports_nan(gpu_lib):
+    M = _matrix()
+    M[17, 5] = np.nan
+    with pytest.raises(gpu_lib.RsvdError, match="non-finite singular values"):
+        gpu_lib.low_rank_svd_rand_decomp_fixed_rank(M, 20, 5, 2, 2, 1)
+    # the next clean call starts with a clean error state
+    U, S, V = gpu_lib.low_rank_svd_rand_decomp_fixed_rank(_matrix(), 20, 5, 2, 2, 1)
+    assert np.all(np.isfinite(S))
+
+
+@pytest.mark.parametrize("bad", [np.inf, np.nan])
+def test_device_fixed_rank_fp64_and_mixed_return_errors(gpu_lib, bad):
+    rp = gpu_lib
+    lib = rp.load()
+    M = _matrix(4)
+    M[0, 0] = bad
+    m, n, k, p = M.shape[0], M.shape[1], 20, 5
+    A64 = rp.DeviceMatrix.from_host(M)
+    U, S, V = rp.DeviceMatrix(m, k), rp.DeviceMatrix(k, 1), rp.DeviceMatrix(n, k)
+    rc = lib.rsvd_dev_fixed_rank(A64.ptr, m, n, m, k, p, 2, 2, 1, 7, None, U.ptr, m, S.ptr,
+                                 V.ptr, n)
+    assert rc != 0 and "non-finite" in rp.last_error()
+    A32 = rp.DeviceMatrix.from_host(M.astype(np.float32), np.float32)
+    U32, V32 = rp.DeviceMatrix(m, k, np.float32), rp.DeviceMatrix(n, k, np.float32)
+    rc = lib.rsvd_dev_fixed_rank_mp(A32.ptr, m, n, m, k, p, 2, 2, 1, 7, None, 2, U32.ptr, m,
+                                    S.ptr, V32.ptr, n)
+    assert rc != 0 and "non-finite" in rp.last_error()
+
+
+def test_blockrand_tol_mode_nan_returns(gpu_lib):
+    """TOL mode cannot stop on a NaN norm: it runs its bounded number of
+    steps, then reports the non-finite result."""
+    M = _matrix(5)
+    M[100, 150] = np.nan
+    t0 = time.perf_counter()
+    with pytest.raises(gpu_lib.RsvdError, match="non-finite singular values"):
+        gpu_lib.low_rank_svd_blockrand_decomp_fixed_rank_or_prec(M, -1, 5, 1e-6, 2, 16, 1, 1)
+    assert time.perf_counter() - t0 < 60
+
+
+def test_batched_nan_stays_in_its_matrix(gpu_lib):
+    from tests.test_gpu_batched import make_batch
+    rp = gpu_lib
+    lib = rp.load()
+    b = make_batch(rp, 12, 300, 200, 21)
+    assert b.run(q=2, seed=5) == 0
+    clean = [x[1].copy() for x in b.results()]
+    # one NaN into matrix 3
+    i = 3
+    host = b.A.to_host()[:, 0].copy()
+    host[b.a_off[i] + 7] = np.nan
+    assert lib.rsvd_memcpy_h2d(b.A.ptr, host.ctypes.data, host.nbytes) == 0
+    st = b.run(q=2, seed=5)
+    assert st != 0  # its CholeskyQR met a non-positive (NaN) pivot
+    res = b.results()
+    for j, (u, s, v) in enumerate(res):
+        if j == i:
+            assert not np.all(np.isfinite(s))
+        else:
+            assert np.array_equal(s, clean[j])
