@@ -8,6 +8,10 @@
  * returning int give 0 on success and nonzero on failure, with the reason in
  * rsvd_last_error().  Device pointers are HIP device allocations (e.g. from
  * rsvd_dev_alloc or hipMalloc) on the current device; matrices are column-major.
+ * A NaN or Inf in the input matrix makes the device decompositions
+ * (rsvd_dev_fixed_rank*, rsvd_dev_blockrand*) return nonzero ("non-finite
+ * singular values"); the reference's host API still returns its outputs and
+ * sets rsvd_last_error() with the same message.
  */
 #ifndef RSVDPACK_EXT_H
 #define RSVDPACK_EXT_H
