@@ -32,6 +32,8 @@ int main(int argc, char **argv) {
     const char *err = NULL;
     if (mock_call(3, out, nrhs, in, &err)) {
         printf("ERR %s\n", err);
+        for (int i = 0; i < 6; ++i) mock_free((mxArray *)in[i]);
+        matrix_delete(A);
         return 3;
     }
     const char *names[3] = {"U", "S", "V"};
