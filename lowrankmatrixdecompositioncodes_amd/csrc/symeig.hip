@@ -228,7 +228,7 @@ __device__ void jac_apply(int bid, int64_t n, const double* __restrict__ Gin,
                           double* __restrict__ Gout, int64_t ldg, double* __restrict__ V,
                           int64_t ldv, int nblocks, int N, int round,
                           const double* __restrict__ Jall, int ntg, double* T, double* Ja,
-                          double* Jb, double* X, int64_t* ra, int64_t* cb) {
+                          double* Jb, double* X, int64_t* ra, int64_t* cb, bool skip) {
   static_assert(B == 16, "the tile products are 32 x 32, one 16 x 16 quadrant a wave");
   constexpr int S = 2 * B, HP = S + 1;
   const int pairs = N / 2;
@@ -245,6 +245,7 @@ __device__ void jac_apply(int bid, int64_t n, const double* __restrict__ Gin,
       cb[tid] = pair_col<B>(w2, tid, round, N, nblocks, n);
     }
     __syncthreads();
+    if (skip) return;
     const double* J1 = Jall + (size_t)w1 * S * S;
     const double* J2 = Jall + (size_t)w2 * S * S;
     for (int e = tid; e < S * S; e += 256) {
@@ -291,6 +292,7 @@ __device__ void jac_apply(int bid, int64_t n, const double* __restrict__ Gin,
   const int64_t r0 = (int64_t)(bid / pairs) * (2 * S);
   if (tid < S) cb[tid] = pair_col<B>(w, tid, round, N, nblocks, n);
   __syncthreads();
+  if (skip) return;
   const double* Jw = Jall + (size_t)w * S * S;
   for (int e = tid; e < 2 * S * S; e += 256) {
     const int r = e % (2 * S), c = e / (2 * S);
@@ -333,8 +335,16 @@ symj_step_kernel(int64_t n, const double* __restrict__ Gin, double* __restrict__
                  int round1, int nsolve, const double* __restrict__ fro2,
                  const double* __restrict__ Jin, const double* __restrict__ Hin,
                  double* __restrict__ Jout, double* __restrict__ Hout, int ntg,
-                 unsigned long long* __restrict__ flag) {
+                 unsigned long long* __restrict__ flag,
+                 const unsigned long long* __restrict__ prev, unsigned long long mask) {
   constexpr int S = 2 * B, HP = S + 1, BP = B + 1;
+  // Device-side stop: every launch of a sweep reads the previous sweep's
+  // rotation flags (complete: that sweep's launches have finished) and does
+  // nothing when they say it converged, so the host can queue a sweep before
+  // it has read the last one's flags.  Read here, tested after the first
+  // barrier (before any global write), so the load overlaps the set-up.
+  const unsigned long long pv = prev ? *prev : mask;
+  const bool skip = (pv & mask) == 0;
   // four S x HP regions.  Apply role: T, Ja, Jb, X.  Solve role: Ha; the
   // column halves JAh, JBh (stored transposed, B x HP each), then Jl; Tab,
   // then Hb; X3 (S x BP).  Each later tenant is first written after the last
@@ -346,7 +356,7 @@ symj_step_kernel(int64_t n, const double* __restrict__ Gin, double* __restrict__
   const int tid = threadIdx.x;
   if ((int)blockIdx.x >= nsolve) {
     jac_apply<B>((int)blockIdx.x - nsolve, n, Gin, Gout, ldg, V, ldv, nblocks, N, round, Jin, ntg,
-                 sm, sm + S * HP, sm + 2 * S * HP, sm + 3 * S * HP, c0, c1);
+                 sm, sm + S * HP, sm + 2 * S * HP, sm + 3 * S * HP, c0, c1, skip);
     return;
   }
   double* Ha = sm;
@@ -377,6 +387,7 @@ symj_step_kernel(int64_t n, const double* __restrict__ Gin, double* __restrict__
     c1[tid] = pair_col<B>(wb, tid, round, N, nblocks, n);
   }
   __syncthreads();
+  if (skip) return;
   const double* JA = Jin + (size_t)wa * S * S;
   const double* JB = Jin + (size_t)wb * S * S;
   const double* HA = Hin + (size_t)wa * S * S;
@@ -480,7 +491,23 @@ int run_sweeps(Context* c, int64_t n, double* G, int64_t ldg, double* V, int64_t
                        (const double*)G, ldg, nblocks, N, 0, (const double*)fro2.p, Jb.p, Hb.p,
                        flags.p);
     RSVD_LAUNCH_CHECK();
-    for (; sweep < max_sweeps; ++sweep) {
+    // The first launches of sweep s are queued before the host looks at
+    // sweep s - 1's flags (copied to pinned memory behind sweep s - 1 and
+    // waited for while those launches run); when s - 1 converged they return
+    // at once (the device-side stop above), the rest of the sweep is never
+    // queued, and the state is the one sweep s started from.
+    const unsigned long long mask = loose ? 2ULL : 1ULL;
+    constexpr int kAhead = 4;  // launches of sweep s queued before the wait
+    volatile unsigned long long* slot = static_cast<unsigned long long*>(c->pinned);
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    RSVD_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+    RSVD_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+    int rc = 0;
+    bool done = false;
+    for (; sweep < max_sweeps && rc == 0 && !done; ++sweep) {
+      double* cur0 = cur;
+      const int jc0 = jc;
+      const unsigned long long* prev = sweep > 0 ? flags.p + sweep - 1 : nullptr;
       for (int r = 0; r < N - 1; ++r) {
         const bool wrap = r + 1 == N - 1;
         const int r1 = wrap ? 0 : r + 1;
@@ -489,21 +516,43 @@ int run_sweeps(Context* c, int64_t n, double* G, int64_t ldg, double* V, int64_t
                            n, (const double*)cur, nxt, ldg, V, ldv, nblocks, N, r, r1, pairs,
                            (const double*)fro2.p, (const double*)(Jb.p + jc * js),
                            (const double*)(Hb.p + jc * js), Jb.p + (1 - jc) * js,
-                           Hb.p + (1 - jc) * js, ntg, fl);
-        RSVD_LAUNCH_CHECK();
+                           Hb.p + (1 - jc) * js, ntg, fl, prev, mask);
+        if (hipGetLastError() != hipSuccess) {
+          rc = set_error("symeig: step launch failed");
+          break;
+        }
         double* tmp = cur;
         cur = nxt;
         nxt = tmp;
         jc = 1 - jc;
+        // after the first kAhead launches (or the whole sweep, if shorter):
+        // sweep - 1's verdict
+        if (sweep > 0 && (r + 1 == kAhead || (r + 1 == N - 1 && N - 1 < kAhead))) {
+          if (hipEventSynchronize(ev[(sweep - 1) & 1]) != hipSuccess) {
+            rc = set_error("symeig: flag wait failed");
+            break;
+          }
+          // no pair met in sweep - 1 was above kStopTol (or counted by the
+          // loose stop): this sweep's launches did nothing
+          if ((slot[(sweep - 1) & 1] & mask) == 0) {
+            cur = cur0;
+            jc = jc0;
+            done = true;
+            break;
+          }
+        }
       }
-      unsigned long long h = 0;
-      RSVD_TRY(read_small(c, &h, flags.p + sweep, sizeof(h)));
-      // no pair met in this sweep was above kStopTol (or counted by the loose stop)
-      if ((h & (loose ? 2ULL : 1ULL)) == 0) {
-        ++sweep;
+      if (rc || done) break;
+      if (hipMemcpyAsync((void*)(slot + (sweep & 1)), flags.p + sweep, sizeof(unsigned long long),
+                         hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+          hipEventRecord(ev[sweep & 1], c->stream) != hipSuccess) {
+        rc = set_error("symeig: flag copy failed");
         break;
       }
     }
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+    if (rc) return rc;
   }
   if (cur != G)
     RSVD_HIP(hipMemcpy2DAsync(G, ldg * sizeof(double), cur, ldg * sizeof(double),
