@@ -1244,6 +1244,8 @@ int rsvd_dev_learned_lowrank(const float* dTheta, const double* dA, const float*
   double a2 = 0;
   RSVD_HIP(hipMemcpyAsync(&a2, nrm.p, sizeof a2, hipMemcpyDeviceToHost, c->stream));
   RSVD_TRY(stream_sync(c));
+  if (!std::isfinite(a2))
+    return set_error("rsvd_dev_learned_lowrank: non-finite ||A||_F (the input holds NaN or Inf)");
   int q = 0;
   for (int i = 1; i < 5; ++i)
     if (o[i] > o[q]) q = i;

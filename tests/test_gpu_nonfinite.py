@@ -62,6 +62,30 @@ def test_blockrand_tol_mode_nan_returns(gpu_lib):
     assert time.perf_counter() - t0 < 60
 
 
+def test_learned_path_refuses_nan(gpu_lib):
+    """The learned front end sees the NaN in ||A||_F and stops before
+    predicting anything."""
+    from lowrankmatrixdecompositioncodes_amd import learned
+    rp = gpu_lib
+    lib = rp.load()
+    fm = learned.family(3, 77)
+    host = fm.A64.to_host()
+    host[10, 20] = np.nan
+    A64 = rp.DeviceMatrix.from_host(host)
+    A32 = rp.DeviceMatrix.from_host(host.astype(np.float32), np.float32)
+    theta = rp.DeviceMatrix(lib.rsvd_mlp_num_params(), 1, np.float32)
+    assert lib.rsvd_dev_mlp_init(1, theta.ptr) == 0
+    fr, q, k, fb = C.c_int(0), C.c_int(0), C.c_int(0), C.c_int(0)
+    res = C.c_double(0)
+    dU, dS, dV = C.c_void_p(), C.c_void_p(), C.c_void_p()
+    m, n = host.shape
+    rc = lib.rsvd_dev_learned_lowrank(theta.ptr, A64.ptr, A32.ptr, m, n, m, 1e-4, 10, 0.1, 32, 2,
+                                      5, C.byref(fr), C.byref(dU), C.byref(dS), C.byref(dV),
+                                      C.byref(q), C.byref(k), C.byref(fb), C.byref(res))
+    assert rc != 0 and "non-finite" in rp.last_error()
+    assert not dU.value and not dS.value and not dV.value
+
+
 def test_batched_nan_stays_in_its_matrix(gpu_lib):
     from tests.test_gpu_batched import make_batch
     rp = gpu_lib
