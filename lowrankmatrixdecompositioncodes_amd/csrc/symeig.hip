@@ -36,13 +36,27 @@ __device__ __forceinline__ int player(int pos, int r, int N) {
   return pos == 0 ? 0 : 1 + ((pos - 1 + r) % (N - 1));
 }
 
-// Global column of slot s (0..2B-1) of block pair w in round r; -1 when the
-// slot is padding (dummy block or past n).
+// The columns of block pair w in round r: slot s (0..2B-1) is column
+// first[s / B] + s % B, or -1 when the slot is padding (dummy block or past n).
+template <int B>
+struct PairCols {
+  int64_t first[2];
+  int64_t n;
+  __device__ __forceinline__ PairCols(int w, int r, int N, int nblocks, int64_t n_) : n(n_) {
+    const int ba = player(w, r, N), bb = player(N - 1 - w, r, N);
+    first[0] = ba < nblocks ? (int64_t)ba * B : -1;
+    first[1] = bb < nblocks ? (int64_t)bb * B : -1;
+  }
+  __device__ __forceinline__ int64_t col(int s) const {
+    const int64_t f = first[s >= B];
+    const int64_t c = f + (s & (B - 1));
+    return (f >= 0 && c < n) ? c : -1;
+  }
+};
+
 template <int B>
 __device__ __forceinline__ int64_t pair_col(int w, int s, int r, int N, int nblocks, int64_t n) {
-  const int blk = (s < B) ? player(w, r, N) : player(N - 1 - w, r, N);
-  const int64_t col = (int64_t)blk * B + (s & (B - 1));
-  return (blk < nblocks && col < n) ? col : -1;
+  return PairCols<B>(w, r, N, nblocks, n).col(s);
 }
 
 // One workgroup per block pair: H = G(cols, cols), one inner sweep, J out.
@@ -185,50 +199,63 @@ symj_solve_kernel(int64_t n, const double* __restrict__ G, int64_t ldg, int nblo
 
 typedef double d4_t __attribute__((ext_vector_type(4)));
 
-// The wave's 16 x 16 quadrant (ti, tj) = (wave / 2, wave % 2) of the 32 x 32
-// product op(P) Q, P and Q in LDS (row-major, padded HP), on the FP64 matrix
-// cores: eight v_mfma_f64_16x16x4f64 steps along k.  srcA carries Q[k][n] and
-// srcB op(P)[m][k] (lane l: n or m = l % 16, k = 4 s + l / 16), so lane l's
-// accumulator register r holds C[16 ti + l % 16][16 tj + 4 r + l / 16].  Two
-// LDS reads per 16 multiply-adds a lane, where the register-blocked scalar
-// product read one per two.
+// The 16 x 16 block (bm, bn) of the product op(P) Q with inner dimension 32,
+// P and Q in LDS (row-major, leading dimensions lp and lq; op(P) = P^T reads
+// P[k][m]), on the FP64 matrix cores: eight v_mfma_f64_16x16x4f64 steps along
+// k.  srcA carries Q[k][n] and srcB op(P)[m][k] (lane l: n or m = l % 16,
+// k = 4 s + l / 16), so lane l's accumulator register r holds
+// C[16 bm + l % 16][16 bn + 4 r + l / 16].
 template <bool TA>
-__device__ __forceinline__ d4_t tile_mfma(const double* __restrict__ P,
-                                          const double* __restrict__ Q) {
-  constexpr int HP = 33;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int ti = wave >> 1, tj = wave & 1, li = lane & 15, lk = lane >> 4;
+__device__ __forceinline__ d4_t blk_mfma(const double* __restrict__ P, int lp,
+                                         const double* __restrict__ Q, int lq, int bm, int bn) {
+  const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
   d4_t acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
   for (int s = 0; s < 8; ++s) {
     const int k = 4 * s + lk;
-    const double q = Q[k * HP + 16 * tj + li];
-    const double p = TA ? P[k * HP + 16 * ti + li] : P[(16 * ti + li) * HP + k];
+    const double q = Q[k * lq + 16 * bn + li];
+    const double p = TA ? P[k * lp + 16 * bm + li] : P[(16 * bm + li) * lp + k];
     acc = __builtin_amdgcn_mfma_f64_16x16x4f64(q, p, acc, 0, 0, 0);
   }
   return acc;
 }
 
-// Stores a wave's quadrant (the layout of tile_mfma) into the 32 x 32 tile C.
-__device__ __forceinline__ void tile_store(const d4_t& acc, double* __restrict__ C) {
-  constexpr int HP = 33;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int ti = wave >> 1, tj = wave & 1, li = lane & 15, lk = lane >> 4;
+// Stores block (bm, bn) (the layout of blk_mfma) into C (leading dimension ldc).
+__device__ __forceinline__ void blk_store(const d4_t& acc, double* __restrict__ C, int ldc,
+                                          int bm, int bn) {
+  const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
 #pragma unroll
-  for (int r = 0; r < 4; ++r) C[(16 * ti + li) * HP + 16 * tj + 4 * r + lk] = acc[r];
+  for (int r = 0; r < 4; ++r) C[(16 * bm + li) * ldc + 16 * bn + 4 * r + lk] = acc[r];
+}
+
+// The wave's 16 x 16 quadrant (wave / 2, wave % 2) of the 32 x 32 product
+// op(P) Q, P and Q padded to HP = 33.  Two LDS reads per 16 multiply-adds a
+// lane, where a register-blocked scalar product reads one per two.
+template <bool TA>
+__device__ __forceinline__ d4_t tile_mfma(const double* __restrict__ P,
+                                          const double* __restrict__ Q) {
+  const int wave = threadIdx.x >> 6;
+  return blk_mfma<TA>(P, 33, Q, 33, wave >> 1, wave & 1);
+}
+
+__device__ __forceinline__ void tile_store(const d4_t& acc, double* __restrict__ C) {
+  const int wave = threadIdx.x >> 6;
+  blk_store(acc, C, 33, wave >> 1, wave & 1);
 }
 
 // Apply round r's rotations (the apply role of symj_step_kernel): blocks
 // [0, ntg) take the upper pair tiles (w1 <= w2) of Gin and write J_w1^T T J_w2
 // to both triangles of Gout; the rest take 2S-row strips of V (in place):
 // V(rows, cols(w)) <- V(rows, cols(w)) J_w.  Every element of Gout and V is
-// written by exactly one workgroup.
+// written by exactly one workgroup.  Every global read of a workgroup is
+// issued before its first wait (one memory latency), ahead of the skip test;
+// padding slots read an in-bounds element and drop it.
 template <int B>
 __device__ void jac_apply(int bid, int64_t n, const double* __restrict__ Gin,
                           double* __restrict__ Gout, int64_t ldg, double* __restrict__ V,
                           int64_t ldv, int nblocks, int N, int round,
                           const double* __restrict__ Jall, int ntg, double* T, double* Ja,
-                          double* Jb, double* X, int64_t* ra, int64_t* cb, bool skip) {
+                          double* Jb, double* X, bool skip) {
   static_assert(B == 16, "the tile products are 32 x 32, one 16 x 16 quadrant a wave");
   constexpr int S = 2 * B, HP = S + 1;
   const int pairs = N / 2;
@@ -240,21 +267,30 @@ __device__ void jac_apply(int bid, int64_t n, const double* __restrict__ Gin,
       ++w1;
     }
     const int w2 = w1 + bid;
-    if (tid < S) {
-      ra[tid] = pair_col<B>(w1, tid, round, N, nblocks, n);
-      cb[tid] = pair_col<B>(w2, tid, round, N, nblocks, n);
-    }
-    __syncthreads();
-    if (skip) return;
+    const PairCols<B> pa(w1, round, N, nblocks, n), pb(w2, round, N, nblocks, n);
     const double* J1 = Jall + (size_t)w1 * S * S;
     const double* J2 = Jall + (size_t)w2 * S * S;
-    for (int e = tid; e < S * S; e += 256) {
-      const int r = e % S, c = e / S;
-      const int64_t gr = ra[r], gc = cb[c];
-      T[r * HP + c] = (gr >= 0 && gc >= 0) ? Gin[gr + gc * ldg] : 0.0;
-      Ja[r * HP + c] = J1[r + c * S];
-      Jb[r * HP + c] = J2[r + c * S];
+    // thread: row r of the tile, columns c0 + 8 it
+    const int r = tid % S, c0 = tid / S;
+    const int64_t gr = pa.col(r);
+    double tv[4], av[4], bv[4];
+#pragma unroll
+    for (int it = 0; it < 4; ++it) {
+      const int c = c0 + 8 * it;
+      const int64_t gc = pb.col(c);
+      const double g = Gin[(gr < 0 ? 0 : gr) + (gc < 0 ? 0 : gc) * ldg];
+      tv[it] = (gr >= 0 && gc >= 0) ? g : 0.0;
+      av[it] = J1[r + c * S];
+      bv[it] = J2[r + c * S];
     }
+#pragma unroll
+    for (int it = 0; it < 4; ++it) {
+      const int c = c0 + 8 * it;
+      T[r * HP + c] = tv[it];
+      Ja[r * HP + c] = av[it];
+      Jb[r * HP + c] = bv[it];
+    }
+    if (skip) return;
     __syncthreads();
     tile_store(tile_mfma<false>(T, Jb), X);  // X = T J_w2
     __syncthreads();
@@ -264,58 +300,63 @@ __device__ void jac_apply(int bid, int64_t n, const double* __restrict__ Gin,
     __syncthreads();
     if (w1 == w2) {
       // diagonal pair tile: the upper triangle, mirrored
-      for (int e = tid; e < S * S; e += 256) {
-        const int r = e % S, c = e / S;
-        const int64_t gr = ra[r], gc = cb[c];
-        if (gr < 0 || gc < 0) continue;
-        Gout[gr + gc * ldg] = (r <= c) ? T[r * HP + c] : T[c * HP + r];
+#pragma unroll
+      for (int it = 0; it < 4; ++it) {
+        const int c = c0 + 8 * it;
+        const int64_t gc = pb.col(c);
+        if (gr >= 0 && gc >= 0) Gout[gr + gc * ldg] = (r <= c) ? T[r * HP + c] : T[c * HP + r];
       }
     } else {
-      for (int e = tid; e < S * S; e += 256) {
-        const int r = e % S, c = e / S;
-        const int64_t gr = ra[r], gc = cb[c];
+      const int64_t gr2 = pb.col(r);
+#pragma unroll
+      for (int it = 0; it < 4; ++it) {
+        const int c = c0 + 8 * it;
+        const int64_t gc = pb.col(c), gc2 = pa.col(c);
         if (gr >= 0 && gc >= 0) Gout[gr + gc * ldg] = T[r * HP + c];
-      }
-      for (int e = tid; e < S * S; e += 256) {
-        const int r = e % S, c = e / S;  // mirror: row cb[r], column ra[c]
-        const int64_t gr = cb[r], gc = ra[c];
-        if (gr >= 0 && gc >= 0) Gout[gr + gc * ldg] = T[c * HP + r];
+        // mirror: row cols(w2)[r], column cols(w1)[c]
+        if (gr2 >= 0 && gc2 >= 0) Gout[gr2 + gc2 * ldg] = T[c * HP + r];
       }
     }
     return;
   }
   if (V == nullptr) return;
   // V strip: rows [r0, r0 + 2S) of the pair's columns, in two S-row halves
-  // (T and X), loaded together so one memory latency covers both
+  // (T and X).  Thread: row r of the strip, columns cq + 4 it; J_w element
+  // (tid % S, tid / S + 8 it).
   bid -= ntg;
   const int w = bid % pairs;
   const int64_t r0 = (int64_t)(bid / pairs) * (2 * S);
-  if (tid < S) cb[tid] = pair_col<B>(w, tid, round, N, nblocks, n);
-  __syncthreads();
-  if (skip) return;
+  const PairCols<B> pc(w, round, N, nblocks, n);
   const double* Jw = Jall + (size_t)w * S * S;
-  for (int e = tid; e < 2 * S * S; e += 256) {
-    const int r = e % (2 * S), c = e / (2 * S);
-    const int64_t gc = cb[c];
-    const double v = (gc >= 0 && r0 + r < n) ? V[(r0 + r) + gc * ldv] : 0.0;
-    if (r < S) T[r * HP + c] = v;
-    else X[(r - S) * HP + c] = v;
+  const int r = tid % (2 * S), cq = tid / (2 * S);
+  const bool row_ok = r0 + r < n;
+  double vv[8], jv[4];
+#pragma unroll
+  for (int it = 0; it < 8; ++it) {
+    const int64_t gc = pc.col(cq + 4 * it);
+    const double v = V[(row_ok ? r0 + r : 0) + (gc < 0 ? 0 : gc) * ldv];
+    vv[it] = (gc >= 0 && row_ok) ? v : 0.0;
   }
-  for (int e = tid; e < S * S; e += 256) {
-    const int r = e % S, c = e / S;
-    Jb[r * HP + c] = Jw[r + c * S];
-  }
+  const int jr = tid % S, jc0 = tid / S;
+#pragma unroll
+  for (int it = 0; it < 4; ++it) jv[it] = Jw[jr + (jc0 + 8 * it) * S];
+  double* Vh = r < S ? T + r * HP : X + (r - S) * HP;  // this thread's row of the strip
+#pragma unroll
+  for (int it = 0; it < 8; ++it) Vh[cq + 4 * it] = vv[it];
+#pragma unroll
+  for (int it = 0; it < 4; ++it) Jb[jr * HP + jc0 + 8 * it] = jv[it];
+  if (skip) return;
   __syncthreads();
   const d4_t o1 = tile_mfma<false>(T, Jb), o2 = tile_mfma<false>(X, Jb);
   __syncthreads();
   tile_store(o1, T);
   tile_store(o2, X);
   __syncthreads();
-  for (int e = tid; e < 2 * S * S; e += 256) {
-    const int r = e % (2 * S), c = e / (2 * S);
-    const int64_t gc = cb[c];
-    if (gc >= 0 && r0 + r < n)
-      V[(r0 + r) + gc * ldv] = (r < S) ? T[r * HP + c] : X[(r - S) * HP + c];
+  if (!row_ok) return;
+#pragma unroll
+  for (int it = 0; it < 8; ++it) {
+    const int64_t gc = pc.col(cq + 4 * it);
+    if (gc >= 0) V[(r0 + r) + gc * ldv] = Vh[cq + 4 * it];
   }
 }
 
@@ -341,31 +382,30 @@ symj_step_kernel(int64_t n, const double* __restrict__ Gin, double* __restrict__
   // Device-side stop: every launch of a sweep reads the previous sweep's
   // rotation flags (complete: that sweep's launches have finished) and does
   // nothing when they say it converged, so the host can queue a sweep before
-  // it has read the last one's flags.  Read here, tested after the first
-  // barrier (before any global write), so the load overlaps the set-up.
+  // it has read the last one's flags.  Read here and tested after the other
+  // global reads are issued (before any global write), so one memory latency
+  // covers them all.
   const unsigned long long pv = prev ? *prev : mask;
   const bool skip = (pv & mask) == 0;
-  // four S x HP regions.  Apply role: T, Ja, Jb, X.  Solve role: Ha; the
-  // column halves JAh, JBh (stored transposed, B x HP each), then Jl; Tab,
-  // then Hb; X3 (S x BP).  Each later tenant is first written after the last
-  // read of the earlier one, and the footprint of the apply role lets four
+  // Apply role: four S x HP regions T, Ja, Jb, X.  Solve role: Ha, Jl and
+  // Tab (later Hb) S x HP each, then JBq, X3 and JAq S x BP each; no region
+  // changes tenant before the inner sweep.  Both footprints let four
   // workgroups share a CU.
-  static_assert(2 * B * HP <= S * HP && S * BP <= S * HP, "solve role fits the apply regions");
-  __shared__ double sm[4 * S * HP];
-  __shared__ int64_t c0[S], c1[S];
+  constexpr int kSm = (4 * S * HP > 3 * S * HP + 3 * S * BP) ? 4 * S * HP : 3 * S * HP + 3 * S * BP;
+  __shared__ double sm[kSm];
   const int tid = threadIdx.x;
   if ((int)blockIdx.x >= nsolve) {
     jac_apply<B>((int)blockIdx.x - nsolve, n, Gin, Gout, ldg, V, ldv, nblocks, N, round, Jin, ntg,
-                 sm, sm + S * HP, sm + 2 * S * HP, sm + 3 * S * HP, c0, c1, skip);
+                 sm, sm + S * HP, sm + 2 * S * HP, sm + 3 * S * HP, skip);
     return;
   }
   double* Ha = sm;
   double* Jl = sm + S * HP;
-  double* JAh = Jl;            // JAh[c * HP + i], until H_ab is formed
-  double* JBh = Jl + B * HP;   // JBh[c * HP + i]
   double* Tab = sm + 2 * S * HP;
-  double* Hb = Tab;            // from the first inner step on
-  double* X3 = sm + 3 * S * HP;
+  double* Hb = Tab;  // from the first inner step on
+  double* JBq = sm + 3 * S * HP;  // JBq[i * BP + c] = J_wb(i, hb B + c)
+  double* X3 = JBq + S * BP;      // Tab JBq
+  double* JAq = X3 + S * BP;      // JAq[i * BP + c] = J_wa(i, ha B + c)
   const int w = blockIdx.x;
   // the blocks of pair w in round1 and where they sat in round r
   const int xa = player(w, round1, N), xb = player(N - 1 - w, round1, N);
@@ -382,64 +422,68 @@ symj_step_kernel(int64_t n, const double* __restrict__ Gin, double* __restrict__
   int wa, ha, wb, hb;
   locate(xa, &wa, &ha);
   locate(xb, &wb, &hb);
-  if (tid < S) {
-    c0[tid] = pair_col<B>(wa, tid, round, N, nblocks, n);
-    c1[tid] = pair_col<B>(wb, tid, round, N, nblocks, n);
-  }
-  __syncthreads();
-  if (skip) return;
+  const PairCols<B> pa(wa, round, N, nblocks, n), pb(wb, round, N, nblocks, n);
   const double* JA = Jin + (size_t)wa * S * S;
   const double* JB = Jin + (size_t)wb * S * S;
   const double* HA = Hin + (size_t)wa * S * S;
   const double* HB = Hin + (size_t)wb * S * S;
-  for (int e = tid; e < S * S; e += 256) {
-    const int i = e % S, j = e / S;
-    const int64_t gi = c0[i], gj = c1[j];
-    Tab[i * HP + j] = (gi >= 0 && gj >= 0) ? Gin[gi + gj * ldg] : 0.0;
+  // every global read first: thread row i of Tab (columns j0 + 8 it), of JAq
+  // and JBq (columns j0 + 8 it), and entry (dc, dd) of the two diagonal blocks
+  const int i = tid % S, j0 = tid / S;
+  const int dc = tid % B, dd = tid / B;
+  const int64_t gi = pa.col(i);
+  double tv[4], av[2], bv[2];
+#pragma unroll
+  for (int it = 0; it < 4; ++it) {
+    // padding slots read column 0 / row 0 (in bounds) and drop the value
+    const int64_t gj = pb.col(j0 + 8 * it);
+    const double g = Gin[(gi < 0 ? 0 : gi) + (gj < 0 ? 0 : gj) * ldg];
+    tv[it] = (gi >= 0 && gj >= 0) ? g : 0.0;
   }
-  for (int e = tid; e < S * B; e += 256) {
-    const int i = e % S, c = e / S;
-    JAh[c * HP + i] = JA[i + (ha * B + c) * S];
-    JBh[c * HP + i] = JB[i + (hb * B + c) * S];
+#pragma unroll
+  for (int it = 0; it < 2; ++it) {
+    av[it] = JA[i + (ha * B + j0 + 8 * it) * S];
+    bv[it] = JB[i + (hb * B + j0 + 8 * it) * S];
   }
-  {
-    // diagonal blocks of the new tile from round r's final tiles
-    const int c = tid % B, d = tid / B;
-    Ha[c * HP + d] = HA[(ha * B + c) + (ha * B + d) * S];
-    Ha[(B + c) * HP + B + d] = HB[(hb * B + c) + (hb * B + d) * S];
+  const double hav = HA[(ha * B + dc) + (ha * B + dd) * S];
+  const double hbv = HB[(hb * B + dc) + (hb * B + dd) * S];
+  // the rotation floor goes through LDS: its load then completes with the
+  // others instead of being deferred to its first use in the inner sweep
+  __shared__ double flo_s;
+  if (tid == 0) flo_s = 4.0 * 2.220446049250313e-16 * sqrt(*fro2);
+  // J starts as the identity (its region has no other tenant)
+#pragma unroll
+  for (int it = 0; it < 4; ++it) Jl[i * HP + j0 + 8 * it] = (i == j0 + 8 * it) ? 1.0 : 0.0;
+#pragma unroll
+  for (int it = 0; it < 4; ++it) Tab[i * HP + j0 + 8 * it] = tv[it];
+#pragma unroll
+  for (int it = 0; it < 2; ++it) {
+    JAq[i * BP + j0 + 8 * it] = av[it];
+    JBq[i * BP + j0 + 8 * it] = bv[it];
   }
-  const double flo = 4.0 * 2.220446049250313e-16 * sqrt(*fro2);
+  // diagonal blocks of the new tile from round r's final tiles
+  Ha[dc * HP + dd] = hav;
+  Ha[(B + dc) * HP + B + dd] = hbv;
+  // the skip test after the LDS stores, so that no global read waits for it
+  if (skip) return;
   __syncthreads();
-  {
-    // X3 = Tab JBh (S x B): thread (row i, columns 2 k, 2 k + 1)
-    const int i = tid % S, cc = 2 * (tid / S);
-    double s0 = 0.0, s1 = 0.0;
-#pragma unroll 8
-    for (int j = 0; j < S; ++j) {
-      const double t = Tab[i * HP + j];
-      s0 = fma(t, JBh[cc * HP + j], s0);
-      s1 = fma(t, JBh[(cc + 1) * HP + j], s1);
+  // off-diagonal block H_ab = J_a^T G_r(cols(wa), cols(wb)) J_b on the FP64
+  // matrix cores: X3 = Tab JBq (S x B, waves 0 and 1), then H_ab = JAq^T X3
+  // (B x B, wave 0), mirrored into H_ba
+  const int wave = tid >> 6;
+  if (wave < 2) blk_store(blk_mfma<false>(Tab, HP, JBq, BP, wave, 0), X3, BP, wave, 0);
+  __syncthreads();
+  if (wave == 0) {
+    const d4_t h = blk_mfma<true>(JAq, BP, X3, BP, 0, 0);
+    const int lane = tid & 63, c = lane & 15, d0 = lane >> 4;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      Ha[c * HP + B + 4 * r + d0] = h[r];
+      Ha[(B + 4 * r + d0) * HP + c] = h[r];
     }
-    X3[i * BP + cc] = s0;
-    X3[i * BP + cc + 1] = s1;
   }
   __syncthreads();
-  {
-    // H_ab = JAh^T X3 (B x B), mirrored into H_ba
-    const int c = tid % B, d = tid / B;
-    double acc = 0.0;
-#pragma unroll 8
-    for (int i = 0; i < S; ++i) acc = fma(JAh[c * HP + i], X3[i * BP + d], acc);
-    Ha[c * HP + B + d] = acc;
-    Ha[(B + d) * HP + c] = acc;
-  }
-  __syncthreads();
-  // JAh / JBh are dead: J starts as the identity (first read after the
-  // barrier inside the inner sweep's first step)
-  for (int e = tid; e < S * S; e += 256) {
-    const int i = e % S, j = e / S;
-    Jl[i * HP + j] = (i == j) ? 1.0 : 0.0;
-  }
+  const double flo = flo_s;
   int sig = 0;
   const double* Hf = jac_inner<B>(Ha, Hb, Jl, round1 > 0, flo, &sig);
   jac_out<B>(Hf, Jl, w, Jout, Hout, sig, flag);
