@@ -295,7 +295,7 @@ __device__ void gram_store(const d4_t (&acc)[6], double (*G)[kLP]) {
   __syncthreads();
 }
 
-__device__ void lds_gram(const float* __restrict__ X, int64_t R, int l, float (*Xs)[kL + 1],
+__device__ __forceinline__ void lds_gram(const float* __restrict__ X, int64_t R, int l, float (*Xs)[kL + 1],
                          double (*G)[kLP]) {
   d4_t acc[6];
 #pragma unroll
@@ -312,6 +312,85 @@ __device__ void lds_gram(const float* __restrict__ X, int64_t R, int l, float (*
   gram_store(acc, G);
 }
 
+// borth's Gram as a call of its own: inlined there it raises the kernel's
+// register pressure past the three-workgroup budget (bfinal inlines it)
+__device__ __noinline__ void lds_gram_call(const float* __restrict__ X, int64_t R, int l,
+                                           float (*Xs)[kL + 1], double (*G)[kLP]) {
+  lds_gram(X, R, l, Xs, G);
+}
+
+// P(pi.., pk..) Q(qk.., qj..): one 16 x 16 x 16 product of LDS blocks on the
+// FP64 matrix cores (four k steps of 4; the layout above): lane l's register
+// r holds row 4 r + l / 16, column l % 16 of the product.
+__device__ __forceinline__ d4_t mm16(const double (*P)[kLP], int pi, int pk,
+                                     const double (*Q)[kLP], int qk, int qj, d4_t acc) {
+  const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+#pragma unroll
+  for (int s = 0; s < 4; ++s)
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(P[pi + li][pk + 4 * s + lk],
+                                               Q[qk + 4 * s + lk][qj + li], acc, 0, 0, 0);
+  return acc;
+}
+__device__ __forceinline__ void st16(double (*C)[kLP], int ci, int cj, const d4_t& v,
+                                     double scale) {
+  const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) C[ci + 4 * r + lk][cj + li] = scale * v[r];
+}
+
+// Ri = R^-1 for the upper-triangular R in G (kL x kL; rows and columns >= l
+// zero, rinv[i] = 1 / R_ii for i < l), by 16 x 16 blocks: the three diagonal
+// blocks D_b = R_bb^-1 at once (48 threads, one column each, column-oriented
+// substitution in registers: a step is one multiply and independent
+// multiply-adds), then on the FP64 matrix cores
+//   Ri01 = -D0 R01 D1,  Ri12 = -D1 R12 D2,  Ri02 = -D0 (R01 Ri12 + R02 D2).
+// Entries in rows or columns >= l, and below the diagonal, are zero.
+__device__ __forceinline__ void blocked_tri_inv(const double (*G)[kLP], double (*Ri)[kLP], int l,
+                                const double* rinv) {
+  static_assert(kL == 48, "three 16 x 16 blocks");
+  const int tid = threadIdx.x, wave = tid >> 6;
+  // the lower blocks are zero
+  for (int e = tid; e < 3 * 256; e += 256) {
+    const int t = e >> 8, i = (e >> 4) & 15, j = e & 15;
+    const int bi = t == 0 ? 1 : 2, bj = t == 2 ? 1 : 0;
+    Ri[16 * bi + i][16 * bj + j] = 0.0;
+  }
+  if (tid < 48) {
+    const int b = tid >> 4, cc = tid & 15, o = 16 * b;
+    double x[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) x[r] = (r == cc && o + cc < l) ? 1.0 : 0.0;
+#pragma unroll
+    for (int r = 15; r >= 0; --r) {
+      x[r] *= (o + r < l) ? rinv[o + r] : 0.0;
+#pragma unroll
+      for (int q = 0; q < r; ++q) x[q] = fma(-G[o + q][o + r], x[r], x[q]);
+      // keep each step's reads in the step (hoisted together they spill)
+      __builtin_amdgcn_sched_barrier(0);
+    }
+#pragma unroll
+    for (int r = 0; r < 16; ++r) Ri[o + r][o + cc] = x[r];
+  }
+  __syncthreads();
+  // T01 = R01 D1 and T12 = R12 D2 into the (still free) blocks of Ri01, Ri12
+  const d4_t z = {0.0, 0.0, 0.0, 0.0};
+  if (wave == 0) st16(Ri, 0, 16, mm16(G, 0, 16, Ri, 16, 16, z), 1.0);
+  if (wave == 1) st16(Ri, 16, 32, mm16(G, 16, 32, Ri, 32, 32, z), 1.0);
+  __syncthreads();
+  // Ri01 = -D0 T01, Ri12 = -D1 T12, in place: a wave reads its whole block
+  // through the products before its stores (LDS operations of one wave stay
+  // in program order)
+  if (wave == 0) st16(Ri, 0, 16, mm16(Ri, 0, 0, Ri, 0, 16, z), -1.0);
+  if (wave == 1) st16(Ri, 16, 32, mm16(Ri, 16, 16, Ri, 16, 32, z), -1.0);
+  __syncthreads();
+  if (wave == 0) {
+    // U = R01 Ri12 + R02 D2 into Ri02's block, then Ri02 = -D0 U in place
+    st16(Ri, 0, 32, mm16(G, 0, 32, Ri, 32, 32, mm16(G, 0, 16, Ri, 16, 32, z)), 1.0);
+    st16(Ri, 0, 32, mm16(Ri, 0, 0, Ri, 0, 32, z), -1.0);
+  }
+  __syncthreads();
+}
+
 // In-place upper Cholesky of G (l x l, upper triangle used) -> R in the upper
 // triangle (G = R^T R, zero below), then Ri = R^-1 (upper).  shift is added
 // to the diagonal first.  Returns through *ok (shared) whether every pivot
@@ -319,9 +398,8 @@ __device__ void lds_gram(const float* __restrict__ X, int64_t R, int l, float (*
 // Factorisation: unnormalised right-looking elimination, one barrier per
 // step: thread (row a = tid % 64, columns 12 (tid / 64) ..) keeps its 12
 // entries in registers, subtracts U_j,a U_j,c / U_j,j at step j and publishes
-// row j + 1 when it owns it.  Inverse: one thread per column, back
-// substitution into the LDS result.
-__device__ void lds_chol_inv(double (*G)[kLP], double (*Ri)[kLP], int l, double shift, int* ok) {
+// row j + 1 when it owns it.  Inverse: blocked_tri_inv.
+__device__ __noinline__ void lds_chol_inv(double (*G)[kLP], double (*Ri)[kLP], int l, double shift, int* ok) {
   __shared__ double piv[kL], inv[kL];
   const int tid = threadIdx.x, a = tid & 63, c0 = (tid >> 6) * 12;
   if (tid == 0) *ok = 1;
@@ -364,25 +442,7 @@ __device__ void lds_chol_inv(double (*G)[kLP], double (*Ri)[kLP], int l, double 
     G[i][c] = v;
   }
   __syncthreads();
-  // column c of R^-1 by back substitution, the solved entries kept in Ri
-  // itself (registers would cap the occupancy at two workgroups a CU)
-  if (tid < kL) {
-    const int c = tid;
-    for (int r = kL - 1; r >= 0; --r) {
-      double v = 0.0;
-      if (c < l && r <= c) {
-        if (r == c) {
-          v = inv[r];
-        } else {
-          double s = 0.0;
-          for (int u = r + 1; u <= c; ++u) s = fma(G[r][u], Ri[u][c], s);
-          v = -s * inv[r];
-        }
-      }
-      Ri[r][c] = v;
-    }
-  }
-  __syncthreads();
+  blocked_tri_inv(G, Ri, l, inv);
 }
 
 // out (R x kout, ld ldo) = scale_j * X W for X (R x 48, ld R, first l
@@ -450,7 +510,7 @@ borth_kernel(const BMat* __restrict__ mats, const int* __restrict__ order, float
   float* X = work + (FROM_Z ? d.z : d.y);
   const int l = d.l;
   for (int pass = 0; pass < 2; ++pass) {
-    lds_gram(X, R, l, Xs, G);
+    lds_gram_call(X, R, l, Xs, G);
     double tr = 0.0;
     for (int j = 0; j < l; ++j) tr += G[j][j];
     __syncthreads();  // every thread has read the diagonal before it is shifted
