@@ -71,11 +71,25 @@ namespace {
 
 constexpr int CB = 32, CHP = CB + 1;
 
-__device__ __forceinline__ void ci_load(const double* __restrict__ M, int64_t ld, int64_t r0,
-                                        int64_t c0, int rows, int cols, double (*T)[CHP]) {
-  for (int e = threadIdx.x; e < CB * CB; e += 256) {
-    const int i = e & (CB - 1), j = e >> 5;
-    T[i][j] = (i < rows && j < cols) ? M[(r0 + i) + (c0 + j) * ld] : 0.0;
+// A 32 x 32 tile of M at (r0, c0) (rows x cols valid, zero beyond) in two
+// halves: ci_fetch issues the thread's four loads (entries tid + 256 q; the
+// padding reads an in-bounds element and drops it), ci_put stores them to
+// LDS, so that every tile of a launch is fetched before the first wait.
+__device__ __forceinline__ void ci_fetch(const double* __restrict__ M, int64_t ld, int64_t r0,
+                                         int64_t c0, int rows, int cols, double (&v)[4]) {
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int e = threadIdx.x + 256 * q, i = e & (CB - 1), j = e >> 5;
+    const bool in = i < rows && j < cols;
+    const double x = M[(r0 + (in ? i : 0)) + (c0 + (in ? j : 0)) * ld];
+    v[q] = in ? x : 0.0;
+  }
+}
+__device__ __forceinline__ void ci_put(const double (&v)[4], double (*T)[CHP]) {
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int e = threadIdx.x + 256 * q;
+    T[e & (CB - 1)][e >> 5] = v[q];
   }
 }
 
@@ -250,18 +264,31 @@ __device__ void ci_tri_inv(const double (*Rk)[CHP], const double* invd, int kb,
   __syncthreads();
 }
 
-// acc[q] += sum_j op(A)[a][j] B[j][c0 + q] for thread (a, c0) over the padded
-// 32 x 32 tiles (op(A) = A^T when TA).
+typedef double ci_d4 __attribute__((ext_vector_type(4)));
+
+// acc += the wave's 16 x 16 quadrant (wave / 2, wave % 2) of op(A) B over the
+// padded 32 x 32 tiles (op(A) = A^T when TA), on the FP64 matrix cores: eight
+// v_mfma_f64_16x16x4f64 steps (srcA lane l: op(A)[l % 16][k = l / 16], srcB
+// lane l: B[k = l / 16][l % 16]).  Register r of lane l is the entry at
+// ci_row(r), ci_col() of the tile.
 template <bool TA>
-__device__ __forceinline__ void ci_mma(const double (*A)[CHP], const double (*B)[CHP],
-                                       double (&acc)[4]) {
-  const int a = threadIdx.x & (CB - 1), c0 = (threadIdx.x >> 5) * 4;
-#pragma unroll 8
-  for (int j = 0; j < CB; ++j) {
-    const double av = TA ? A[j][a] : A[a][j];
+__device__ __forceinline__ ci_d4 ci_mma(const double (*A)[CHP], const double (*B)[CHP],
+                                        ci_d4 acc) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int li = lane & 15, lk = lane >> 4, ti = wave >> 1, tj = wave & 1;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) acc[q] += av * B[j][c0 + q];
+  for (int s = 0; s < CB / 4; ++s) {
+    const int k = 4 * s + lk;
+    const double av = TA ? A[k][16 * ti + li] : A[16 * ti + li][k];
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, B[k][16 * tj + li], acc, 0, 0, 0);
   }
+  return acc;
+}
+__device__ __forceinline__ int ci_row(int r) {
+  return 16 * (threadIdx.x >> 7) + 4 * r + ((threadIdx.x & 63) >> 4);
+}
+__device__ __forceinline__ int ci_col() {
+  return 16 * ((threadIdx.x >> 6) & 1) + (threadIdx.x & 15);
 }
 
 __global__ void __launch_bounds__(256)
@@ -272,7 +299,7 @@ cholinv_panel_kernel(int64_t n, double* __restrict__ G, double* __restrict__ R,
       T4[CB][CHP];
   __shared__ double refd[CB], piv[CB], sqd[CB], invd[CB];
   __shared__ int bad;
-  const int tid = threadIdx.x, a = tid & (CB - 1), c0 = (tid >> 5) * 4;
+  const int tid = threadIdx.x;
   const int64_t p = (int64_t)k * CB;
   const int kb = (int)((n - p) < CB ? (n - p) : CB);
   const int nb = (int)((n + CB - 1) / CB);
@@ -300,22 +327,37 @@ cholinv_panel_kernel(int64_t n, double* __restrict__ G, double* __restrict__ R,
   }
   const int64_t pi = (int64_t)i * CB, pti = (int64_t)ti * CB, ptj = (int64_t)tj * CB;
 
-  // ---- loads (all issued before the serial factorisation)
-  ci_load(G, n, p, p, kb, kb, Dm);
-  if (role == 0) {
-    ci_load(G, n, p, pti, kb, bw(ti), T1);
-    if (tj != ti) ci_load(G, n, p, ptj, kb, bw(tj), T2);
-    ci_load(G, n, pti, ptj, bw(ti), bw(tj), T3);
-  } else if (role == 1) {
-    ci_load(G, n, p, ptj, kb, bw(tj), T1);
-    if (i < k) {
-      ci_load(Acc, n, pi, p, CB, kb, T2);
-      ci_load(Acc, n, pi, ptj, CB, bw(tj), T3);
+  // ---- loads: every tile of the workgroup fetched before the first wait
+  // (one memory latency ahead of the serial factorisation)
+  {
+    double vd[4], v1[4], v2[4], v3[4];
+    const double rv = (tid < kb) ? ((k == 0) ? G[(p + tid) * (n + 1)] : diag0[p + tid]) : 0.0;
+    ci_fetch(G, n, p, p, kb, kb, vd);
+    if (role == 0) {
+      ci_fetch(G, n, p, pti, kb, bw(ti), v1);
+      if (tj != ti) ci_fetch(G, n, p, ptj, kb, bw(tj), v2);
+      ci_fetch(G, n, pti, ptj, bw(ti), bw(tj), v3);
+      ci_put(v1, T1);
+      if (tj != ti) ci_put(v2, T2);
+      ci_put(v3, T3);
+    } else if (role == 1) {
+      ci_fetch(G, n, p, ptj, kb, bw(tj), v1);
+      if (i < k) {
+        ci_fetch(Acc, n, pi, p, CB, kb, v2);
+        ci_fetch(Acc, n, pi, ptj, CB, bw(tj), v3);
+      }
+      ci_put(v1, T1);
+      if (i < k) {
+        ci_put(v2, T2);
+        ci_put(v3, T3);
+      }
+    } else if (i < k) {
+      ci_fetch(Acc, n, pi, p, CB, kb, v2);
+      ci_put(v2, T2);
     }
-  } else if (i < k) {
-    ci_load(Acc, n, pi, p, CB, kb, T2);
+    ci_put(vd, Dm);
+    if (tid < kb) refd[tid] = rv;
   }
-  if (tid < kb) refd[tid] = (k == 0) ? G[(p + tid) * (n + 1)] : diag0[p + tid];
   if (tid == 0) bad = 0;
   __syncthreads();
   if (role == 0 && k == 0 && ti == tj && tid < bw(ti)) diag0[pti + tid] = T3[tid][tid];
@@ -323,26 +365,27 @@ cholinv_panel_kernel(int64_t n, double* __restrict__ G, double* __restrict__ R,
 
   ci_tri_inv(Dm, invd, kb, Xk, T4);
 
+  const ci_d4 z4 = {0.0, 0.0, 0.0, 0.0};
+  const int cl = ci_col();
   if (role == 0) {
     // R_k,ti = X_kk^T G_k,ti into T4 and R_k,tj into Dm (R_kk is not needed
     // here any more), then the trailing tile
-    double pa[4] = {0.0, 0.0, 0.0, 0.0}, pb[4] = {0.0, 0.0, 0.0, 0.0};
-    ci_mma<true>(Xk, T1, pa);
-    if (tj != ti) ci_mma<true>(Xk, T2, pb);
+    const ci_d4 pa = ci_mma<true>(Xk, T1, z4);
+    const ci_d4 pb = (tj != ti) ? ci_mma<true>(Xk, T2, z4) : z4;
+    // Dm's last reads (inside ci_tri_inv) were before its closing barrier
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      T4[a][c0 + q] = pa[q];
-      if (tj != ti) Dm[a][c0 + q] = pb[q];
+    for (int r = 0; r < 4; ++r) {
+      T4[ci_row(r)][cl] = pa[r];
+      if (tj != ti) Dm[ci_row(r)][cl] = pb[r];
     }
     __syncthreads();
     const double(*Pj)[CHP] = (tj != ti) ? Dm : T4;
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    ci_mma<true>(T4, Pj, acc);
+    const ci_d4 acc = ci_mma<true>(T4, Pj, z4);
     const int wi = bw(ti), wj = bw(tj);
-    if (a < wi) {
 #pragma unroll
-      for (int q = 0; q < 4; ++q)
-        if (c0 + q < wj) G[(pti + a) + (ptj + c0 + q) * n] = T3[a][c0 + q] - acc[q];
+    for (int r = 0; r < 4; ++r) {
+      const int rw = ci_row(r);
+      if (rw < wi && cl < wj) G[(pti + rw) + (ptj + cl) * n] = T3[rw][cl] - acc[r];
     }
     if (ti == tj) {
       for (int e = tid; e < CB * CB; e += 256) {
@@ -357,27 +400,26 @@ cholinv_panel_kernel(int64_t n, double* __restrict__ G, double* __restrict__ R,
   // also R_k,tj = X_kk^T G_k,tj into Dm
   const double(*Xi)[CHP] = Xk;
   {
-    double pb[4] = {0.0, 0.0, 0.0, 0.0}, xa[4] = {0.0, 0.0, 0.0, 0.0};
-    if (role == 1) ci_mma<true>(Xk, T1, pb);
-    if (i < k) ci_mma<false>(T2, Xk, xa);
+    const ci_d4 pb = (role == 1) ? ci_mma<true>(Xk, T1, z4) : z4;
+    const ci_d4 xa = (i < k) ? ci_mma<false>(T2, Xk, z4) : z4;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      if (role == 1) Dm[a][c0 + q] = pb[q];
-      if (i < k) T4[a][c0 + q] = -xa[q];
+    for (int r = 0; r < 4; ++r) {
+      if (role == 1) Dm[ci_row(r)][cl] = pb[r];
+      if (i < k) T4[ci_row(r)][cl] = -xa[r];
     }
     __syncthreads();
     if (i < k) Xi = T4;
   }
   if (role == 1) {
-    double acc[4];
+    ci_d4 acc;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) acc[q] = (i < k) ? T3[a][c0 + q] : 0.0;
-    ci_mma<false>(Xi, Dm, acc);
+    for (int r = 0; r < 4; ++r) acc[r] = (i < k) ? T3[ci_row(r)][cl] : 0.0;
+    acc = ci_mma<false>(Xi, Dm, acc);
     const int rows = (i == k) ? kb : CB, wj = bw(tj);
-    if (a < rows) {
 #pragma unroll
-      for (int q = 0; q < 4; ++q)
-        if (c0 + q < wj) Acc[(pi + a) + (ptj + c0 + q) * n] = acc[q];
+    for (int r = 0; r < 4; ++r) {
+      const int rw = ci_row(r);
+      if (rw < rows && cl < wj) Acc[(pi + rw) + (ptj + cl) * n] = acc[r];
     }
     return;
   }
