@@ -263,6 +263,13 @@ constexpr int kMT2T = 2;              // row tiles per wave (op(A) = A^T)
 constexpr int kNTW2 = 9;
 constexpr int kBNT2 = 2 * kNTW2;
 constexpr int kST2 = 3;
+// op(A) = A^T with 288 < l <= 544: one 128 x 544 tile per workgroup (17
+// column tiles a wave, two LDS stages, 140 KiB), so that every A tile is
+// loaded once instead of by two column blocks; the k range is split to fill
+// the CUs.  At m = 65536, n = 16384, l = 544: 4.02 -> 3.19 ms a pass with the
+// slice reduction (the same tile with three stages does not fit the LDS).
+constexpr int kNTWW = 17;
+constexpr int kBNTW = 2 * kNTWW;
 
 // v_mfma_f32_16x16x32 on 8 16-bit values per lane: bf16 (SPLIT 1, 3) or fp16
 // (SPLIT 2), the fp16 fragments carried in the bf16 vector type bit for bit.
@@ -281,17 +288,19 @@ __device__ __forceinline__ f4v_t mfma16(bf16x8_t a, bf16x8_t b, f4v_t c) {
 // lo, scaled per column by 1 / b_n, with a_c folded in for op(A) = A), two
 // products hi hi + lo hi; the epilogue multiplies by b_n (bscale) and, for
 // op(A) = A^T, by the output row's a_c.
-template <bool TA, int SPLIT, bool VEC, int MT>
+template <bool TA, int SPLIT, bool VEC, int MT, int NTW = kNTW2, int ST = kST2>
 __global__ void __launch_bounds__(kT2)
 apass2_kernel(int64_t M, int64_t N, int64_t KBtot, int64_t KBper, float alpha,
               const float* __restrict__ A, int64_t lda, const bf16x8_t* __restrict__ St,
               int64_t NTP, float beta, float* __restrict__ C, int64_t ldc,
               float* __restrict__ Wk, int gx, int gy, int ks, const float* __restrict__ ascale,
               const float* __restrict__ bscale) {
+  static_assert(ST == 2 || ST == 3, "a ring of two or three stages");
+  constexpr int BNT = 2 * NTW;  // column tiles per workgroup
   constexpr int NP = (SPLIT >= 2) ? 2 : 1;
-  constexpr int NCH = NP * kBNT2;             // 1 KiB chunks per stage
+  constexpr int NCH = NP * BNT;             // 1 KiB chunks per stage
   constexpr int NLD = (NCH + kW2 - 1) / kW2;  // chunk loads per wave per stage
-  __shared__ bf16x8_t Ss[kST2][NP][kBNT2][64];
+  __shared__ bf16x8_t Ss[ST][NP][BNT][64];
   __shared__ bf16x8_t Sdummy[64];
 
   const int tid = threadIdx.x, lane = tid & 63;
@@ -321,7 +330,7 @@ apass2_kernel(int64_t M, int64_t N, int64_t KBtot, int64_t KBper, float alpha,
   const int64_t nk = kbe - kb0;
   const int64_t klast = kbe - 1;
   const int64_t mw = (int64_t)bx * (64 * MT) + wm * 16 * MT;  // this wave's first row
-  const int64_t ntw = (int64_t)by * kBNT2;          // workgroup's first n tile
+  const int64_t ntw = (int64_t)by * BNT;          // workgroup's first n tile
 
   // ---- S stage loads: chunk c = (plane c / 18, tile c % 18), wave-strided;
   // the padding slots reload chunk 0 into Sdummy so every wave issues NLD
@@ -337,12 +346,12 @@ apass2_kernel(int64_t M, int64_t N, int64_t KBtot, int64_t KBper, float alpha,
   for (int q = 0; q < NLD; ++q) {
     const int c = wave + kW2 * q;
     sreal[q] = c < NCH;
-    const int pl = sreal[q] ? c / kBNT2 : 0, t = sreal[q] ? c % kBNT2 : 0;
+    const int pl = sreal[q] ? c / BNT : 0, t = sreal[q] ? c % BNT : 0;
     sbase[q] = ((int64_t)pl * KBtot * NTP + ntw + t) * 64;
-    soff[q] = pl * kBNT2 + t;
+    soff[q] = pl * BNT + t;
   }
   const int64_t kstride = NTP * 64;  // St elements per k block
-  auto loadS = [&](int64_t kb, bf16x8_t (*dst)[kBNT2][64]) {
+  auto loadS = [&](int64_t kb, bf16x8_t (*dst)[BNT][64]) {
     kb = kb < klast ? kb : klast;
     if constexpr (TA) {
       const int64_t koff = kb * kstride;
@@ -357,7 +366,7 @@ apass2_kernel(int64_t M, int64_t N, int64_t KBtot, int64_t KBper, float alpha,
       for (int q = 0; q < NLD; ++q) {
         const int c = wave + kW2 * q;
         const bool real = c < NCH;
-        const int pl = real ? c / kBNT2 : 0, t = real ? c % kBNT2 : 0;
+        const int pl = real ? c / BNT : 0, t = real ? c % BNT : 0;
         const bf16x8_t* g = St + ((pl * KBtot + kb) * NTP + ntw + t) * 64 + lane;
         lds_ptr_t d = real ? (lds_ptr_t)&dst[pl][t][0] : (lds_ptr_t)&Sdummy[0];
         __builtin_amdgcn_global_load_lds((gbl_ptr_t)g, d, 16, 0, 0);
@@ -434,9 +443,9 @@ apass2_kernel(int64_t M, int64_t N, int64_t KBtot, int64_t KBper, float alpha,
     }
   };
 
-  f4v_t acc[kNTW2][MT];
+  f4v_t acc[NTW][MT];
 #pragma unroll
-  for (int j = 0; j < kNTW2; ++j)
+  for (int j = 0; j < NTW; ++j)
 #pragma unroll
     for (int i = 0; i < MT; ++i) acc[j][i] = f4v_t{0.f, 0.f, 0.f, 0.f};
 
@@ -450,8 +459,14 @@ apass2_kernel(int64_t M, int64_t N, int64_t KBtot, int64_t KBper, float alpha,
   // 3.55 -> 3.22 ms, bitwise the same output).  op(A) = A^T, whose A loads
   // are 16-byte vectors, measured 3% slower that way and keeps both at the
   // start of the block.
-  auto step = [&](int64_t kb, bf16x8_t (*cur)[kBNT2][64], bf16x8_t (*nx2)[kBNT2][64]) {
-    if constexpr (TA) {
+  auto step = [&](int64_t kb, bf16x8_t (*cur)[BNT][64], bf16x8_t (*nx2)[BNT][64]) {
+    if constexpr (ST == 2) {
+      // two stages: S(kb + 1) into the stage read in the previous block,
+      // issued before the A loads so that the split's wait for A(kb + 1)
+      // (vector memory counters retire in order) also covers it
+      loadS(kb + 1, nx2);
+      if constexpr (TA) loadA(kb + 1);
+    } else if constexpr (TA) {
       loadA(kb + 1);
       loadS(kb + 2, nx2);
     }
@@ -459,12 +474,12 @@ apass2_kernel(int64_t M, int64_t N, int64_t KBtot, int64_t KBper, float alpha,
     if constexpr (!TA) {
       // column tiles in pairs, so that consecutive MFMAs on one accumulator are
       // 2 MT apart (measured 8% faster for op(A) = A; slower for op(A) = A^T)
-      constexpr int NPR = (kNTW2 + 1) / 2;
+      constexpr int NPR = (NTW + 1) / 2;
       bf16x8_t sh[2][2], sl[2][2];
 #pragma unroll
       for (int q = 0; q < 2; ++q) {
-        sh[0][q] = cur[0][wn * kNTW2 + q][lane];
-        if (SPLIT >= 2) sl[0][q] = cur[NP - 1][wn * kNTW2 + q][lane];
+        sh[0][q] = cur[0][wn * NTW + q][lane];
+        if (SPLIT >= 2) sl[0][q] = cur[NP - 1][wn * NTW + q][lane];
       }
       static_assert(NPR >= MT, "one row tile of A per column pair");
 #pragma unroll
@@ -475,9 +490,9 @@ apass2_kernel(int64_t M, int64_t N, int64_t KBtot, int64_t KBper, float alpha,
 #pragma unroll
           for (int q = 0; q < 2; ++q) {
             const int jn = 2 * (jp + 1) + q;
-            if (jn < kNTW2) {
-              sh[b ^ 1][q] = cur[0][wn * kNTW2 + jn][lane];
-              if (SPLIT >= 2) sl[b ^ 1][q] = cur[NP - 1][wn * kNTW2 + jn][lane];
+            if (jn < NTW) {
+              sh[b ^ 1][q] = cur[0][wn * NTW + jn][lane];
+              if (SPLIT >= 2) sl[b ^ 1][q] = cur[NP - 1][wn * NTW + jn][lane];
             }
           }
         }
@@ -485,7 +500,7 @@ apass2_kernel(int64_t M, int64_t N, int64_t KBtot, int64_t KBper, float alpha,
 #pragma unroll
         for (int i = 0; i < MT; ++i)
           acc[j0][i] = mfma16<SPLIT>(sh[b][0], ah[i], acc[j0][i]);
-        if (j1 < kNTW2) {
+        if (j1 < NTW) {
 #pragma unroll
           for (int i = 0; i < MT; ++i)
             acc[j1][i] = mfma16<SPLIT>(sh[b][1], ah[i], acc[j1][i]);
@@ -494,7 +509,7 @@ apass2_kernel(int64_t M, int64_t N, int64_t KBtot, int64_t KBper, float alpha,
 #pragma unroll
           for (int i = 0; i < MT; ++i)
             acc[j0][i] = mfma16<SPLIT>(sh[b][0], al[i], acc[j0][i]);
-          if (j1 < kNTW2) {
+          if (j1 < NTW) {
 #pragma unroll
             for (int i = 0; i < MT; ++i)
               acc[j1][i] = mfma16<SPLIT>(sh[b][1], al[i], acc[j1][i]);
@@ -504,7 +519,7 @@ apass2_kernel(int64_t M, int64_t N, int64_t KBtot, int64_t KBper, float alpha,
 #pragma unroll
           for (int i = 0; i < MT; ++i)
             acc[j0][i] = mfma16<SPLIT>(sl[b][0], ah[i], acc[j0][i]);
-          if (j1 < kNTW2) {
+          if (j1 < NTW) {
 #pragma unroll
             for (int i = 0; i < MT; ++i)
               acc[j1][i] = mfma16<SPLIT>(sl[b][1], ah[i], acc[j1][i]);
@@ -514,14 +529,14 @@ apass2_kernel(int64_t M, int64_t N, int64_t KBtot, int64_t KBper, float alpha,
       }
     } else {
       bf16x8_t sh[2], sl[2];
-      sh[0] = cur[0][wn * kNTW2][lane];
-      if (SPLIT >= 2) sl[0] = cur[NP - 1][wn * kNTW2][lane];
+      sh[0] = cur[0][wn * NTW][lane];
+      if (SPLIT >= 2) sl[0] = cur[NP - 1][wn * NTW][lane];
 #pragma unroll
-      for (int j = 0; j < kNTW2; ++j) {
+      for (int j = 0; j < NTW; ++j) {
         const int b = j & 1;
-        if (j + 1 < kNTW2) {
-          sh[b ^ 1] = cur[0][wn * kNTW2 + j + 1][lane];
-          if (SPLIT >= 2) sl[b ^ 1] = cur[NP - 1][wn * kNTW2 + j + 1][lane];
+        if (j + 1 < NTW) {
+          sh[b ^ 1] = cur[0][wn * NTW + j + 1][lane];
+          if (SPLIT >= 2) sl[b ^ 1] = cur[NP - 1][wn * NTW + j + 1][lane];
         }
 #pragma unroll
         for (int i = 0; i < MT; ++i)
@@ -541,21 +556,26 @@ apass2_kernel(int64_t M, int64_t N, int64_t KBtot, int64_t KBper, float alpha,
       }
     }
     split();
-    if constexpr (!TA) loadS(kb + 2, nx2);
+    if constexpr (!TA && ST == 3) loadS(kb + 2, nx2);
     __syncthreads();
   };
 
   loadS(kb0, Ss[0]);
-  loadS(kb0 + 1, Ss[1]);
+  if constexpr (ST == 3) loadS(kb0 + 1, Ss[1]);
   loadA(kb0);
   split();
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   int st = 0;
   for (int64_t s = 0; s < nk; ++s) {
-    const int st2 = (st >= 1) ? st - 1 : st + 2;  // (s + 2) % 3
-    step(kb0 + s, Ss[st], Ss[st2]);
-    st = (st == 2) ? 0 : st + 1;
+    if constexpr (ST == 2) {
+      step(kb0 + s, Ss[st], Ss[st ^ 1]);
+      st ^= 1;
+    } else {
+      const int st2 = (st >= 1) ? st - 1 : st + 2;  // (s + 2) % 3
+      step(kb0 + s, Ss[st], Ss[st2]);
+      st = (st == 2) ? 0 : st + 1;
+    }
   }
   // drain the trailing direct loads before the workgroup exits
   __builtin_amdgcn_s_waitcnt(0);
@@ -565,13 +585,13 @@ apass2_kernel(int64_t M, int64_t N, int64_t KBtot, int64_t KBper, float alpha,
   const int64_t ldo = (ks > 1) ? M : ldc;
   const float rowsc = (SPLIT == 2 && TA) ? ascale[bx] : 1.0f;
 #pragma unroll
-  for (int j = 0; j < kNTW2; ++j) {
+  for (int j = 0; j < NTW; ++j) {
 #pragma unroll
     for (int i = 0; i < MT; ++i) {
       const int64_t m = mw + i * 16 + (lane & 15);
 #pragma unroll
       for (int rr = 0; rr < 4; ++rr) {
-        const int64_t n = (ntw + wn * kNTW2 + j) * 16 + 4 * (lane >> 4) + rr;
+        const int64_t n = (ntw + wn * NTW + j) * 16 + 4 * (lane >> 4) + rr;
         if (m < M && n < N) {
           float* cp = out + m + n * ldo;
           const float a = (SPLIT == 2) ? acc[j][i][rr] * (bscale[n] * rowsc) : acc[j][i][rr];
@@ -882,11 +902,15 @@ int apass_bf16(Context* c, const char* tag, bool ta, int64_t M, int64_t N, int64
   // op(A) = A^T, whose rows are separate 128-byte streams per k block (columns
   // of A): 128 of them measured 6% faster than the 64 x 576 tile at m = 65536,
   // n = 16384, l = 544, and 192 or 256 of them slower (open DRAM pages and
-  // TLB entries).  The k range is split while the grid is short of the CU
-  // count (keeping at least 8 k blocks per slice).
+  // TLB entries); for op(A) = A^T with 288 < l <= 544 the 128 x 544 tile
+  // (kNTWW).  The k range is split while the grid is short of the CU count
+  // (keeping at least 8 k blocks per slice).
   const int64_t bm2 = ta ? 64 * kMT2T : kBM2;
   const int64_t gx2 = (M + bm2 - 1) / bm2;
-  const int64_t gy2 = ((N + 15) / 16 + kBNT2 - 1) / kBNT2;
+  const int64_t ntiles = (N + 15) / 16;
+  const bool wide = ta && ntiles > kBNT2 && ntiles <= kBNTW;
+  const int64_t bnt2 = wide ? kBNTW : kBNT2;
+  const int64_t gy2 = (ntiles + bnt2 - 1) / bnt2;
   // k slices: one workgroup per CU runs at a time, so the grid runs in
   // ceil(gx gy ks / cus) waves; take the smallest ks (at least 8 k blocks a
   // slice) whose waves are within 3% of the best fill, e.g. 314 workgroups
@@ -907,7 +931,7 @@ int apass_bf16(Context* c, const char* tag, bool ta, int64_t M, int64_t N, int64
   const int64_t gx = use2 ? gx2 : (M + kBM - 1) / kBM;
   const int64_t gy = use2 ? gy2 : ((N + 15) / 16 + kBNT - 1) / kBNT;
   if (gy > 65535 || gx > 0x7fffffff) return set_error("apass: grid too large");
-  const int64_t NTP = gy * (use2 ? kBNT2 : kBNT);
+  const int64_t NTP = gy * (use2 ? bnt2 : kBNT);
   const int planes = (split == 3) ? 2 : 1;
   DBuf<bf16x8_t> St;
   RSVD_TRY(St.alloc((size_t)(planes * KB * NTP * 64)));
@@ -930,12 +954,20 @@ int apass_bf16(Context* c, const char* tag, bool ta, int64_t M, int64_t N, int64
     const unsigned blocks = (unsigned)(gxp * gy * ks);
     {
       KTimer kt(c, tag);
-#define RSVD_APASS2(TA_, S_, V_)                                                               \
-  hipLaunchKernelGGL((apass2_kernel<TA_, S_, V_, TA_ ? kMT2T : kMT2>), dim3(blocks), dim3(kT2), 0, \
-                     c->stream, M, N, KB,                                                      \
+#define RSVD_APASS2(TA_, S_, V_, ...)                                                     \
+  hipLaunchKernelGGL((apass2_kernel<TA_, S_, V_, TA_ ? kMT2T : kMT2, ##__VA_ARGS__>), dim3(blocks), \
+                     dim3(kT2), 0, c->stream, M, N, KB,                                           \
                      KBper, alpha, A, lda, (const bf16x8_t*)St.p, NTP, beta, C, ldc, Wk.p,        \
                      (int)gxp, (int)gy, ks, (const float*)nullptr, (const float*)nullptr)
-      if (ta) {
+      if (ta && wide) {
+        if (split == 3) {
+          if (vec) RSVD_APASS2(true, 3, true, kNTWW, 2);
+          else RSVD_APASS2(true, 3, false, kNTWW, 2);
+        } else {
+          if (vec) RSVD_APASS2(true, 1, true, kNTWW, 2);
+          else RSVD_APASS2(true, 1, false, kNTWW, 2);
+        }
+      } else if (ta) {
         if (split == 3) {
           if (vec) RSVD_APASS2(true, 3, true);
           else RSVD_APASS2(true, 3, false);

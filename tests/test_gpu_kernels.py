@@ -355,14 +355,18 @@ def test_gemm_bf16x3_is_fp32_accurate(gpu_lib, ta, tb):
 @pytest.mark.parametrize("mode", ["bf16", "bf16x3"])
 @pytest.mark.parametrize("M,N,K", [(16400, 544, 300), (16384, 40, 1000), (16448, 600, 77),
                                    (50000, 300, 1000), (49999, 40, 1001), (16384, 544, 4096),
-                                   (8192, 64, 16384)])
+                                   (8192, 64, 16384), (20000, 400, 1001)])
 def test_gemm_bf16_streaming_path(gpu_lib, ta, mode, M, N, K):
     """Shapes with a skinny right operand take the A-streaming kernels: the
     64 x 576 tile (A read once, more than one column block when N > 576,
-    unaligned rows when K % 4 != 0; every op(A) = A^T case), and for op(A) = A
-    with at least a CU's worth of 192-row blocks the 192 x 288 tile with its
-    LDS ring (two column blocks for N = 300, a K % 32 remainder for K = 1001,
-    the k range split over eight slices for 8192 x 64 x 16384).
+    unaligned rows when K % 4 != 0) for short grids, and with at least a CU's
+    worth of row blocks the LDS-ring kernel: 256 x 288 tiles for op(A) = A
+    (two column blocks for N = 300, a K % 32 remainder for K = 1001, the k
+    range split over eight slices for 8192 x 64 x 16384), 128 x 288 for
+    op(A) = A^T, or for op(A) = A^T with 288 < N <= 544 one 128 x 544 tile
+    with two LDS stages (16384 x 544 x 4096: two k slices; 50000 x 300 x 1000:
+    three slices and the K remainder; 20000 x 400 x 1001: rows of A not
+    16-byte aligned).
     bf16: equal to the exact product of the bf16-rounded operands up to fp32
     accumulation; bf16x3: fp32-accurate."""
     rng = np.random.default_rng(M + N + K)
