@@ -4,8 +4,10 @@
 //
 // Two-sided cyclic block Jacobi.  The n columns are cut into blocks of B = 16;
 // a round pairs the blocks with the circle method (every block meets every
-// other once per sweep).  A round is two launches:
-//   solve : one workgroup per block pair loads the pair's S x S (S = 2B)
+// other once per sweep).  A round has two roles, which one launch
+// (symj_step_kernel) runs side by side -- the apply of round r next to the
+// solve of round r + 1:
+//   solve : one workgroup per block pair forms the pair's S x S (S = 2B)
 //           diagonal tile H of G and runs an inner sweep of two-sided Jacobi
 //           on it (B disjoint rotations per step; all pairs in the first round
 //           of a sweep, only the cross-block pairs afterwards), accumulating
@@ -59,12 +61,6 @@ __device__ __forceinline__ int64_t pair_col(int w, int s, int r, int N, int nblo
   return PairCols<B>(w, r, N, nblocks, n).col(s);
 }
 
-// One workgroup per block pair: H = G(cols, cols), one inner sweep, J out.
-// A step: threads 0..B-1 compute the rotation of local pair tid once and
-// publish it; after a barrier thread t = (bi, bj) (bi = t / B, bj = t % B)
-// applies R_bi^T . R_bj to the 2 x 2 block of H at (pair bi rows, pair bj
-// columns) -- the blocks partition H -- and R_bi to rows 2 bj, 2 bj + 1 of J's
-// columns of pair bi.  H is double-buffered.
 // Local pairing of step t of the inner sweep.  Round 0 of a sweep runs the
 // full cyclic sweep of the 2B columns (S - 1 steps, every pair once, within
 // and across the two blocks); the other rounds only meet the B x B cross pairs
