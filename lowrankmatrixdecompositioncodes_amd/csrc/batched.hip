@@ -145,19 +145,23 @@ bpass_kernel(const BMat* __restrict__ mats, const BItem* __restrict__ items, int
     for (int t = 0; t < kNT; ++t) acc[r][t] = f4v_t{0.f, 0.f, 0.f, 0.f};
 
   const int64_t kb_lo = KS ? KB * wave / 4 : 0, kb_hi = KS ? KB * (wave + 1) / 4 : KB;
-  for (int64_t kb = kb_lo; kb < kb_hi; ++kb) {
+  // one k block's operands in registers: the skinny operand's hi / lo
+  // fragments and the lane's eight A values of each row tile
+  struct Frag {
     bf16x8_t sh[kNT], sl[kNT];
+    float av[RT][8];
+  };
+  auto fetch = [&](int64_t kb, Frag& f) {
 #pragma unroll
     for (int t = 0; t < kNT; ++t) {
-      sh[t] = T[(kb * kNT + t) * 64 + lane];
-      sl[t] = T[((KB + kb) * kNT + t) * 64 + lane];
+      f.sh[t] = T[(kb * kNT + t) * 64 + lane];
+      f.sl[t] = T[((KB + kb) * kNT + t) * 64 + lane];
     }
     const int64_t k0 = kb * 32 + 8 * (lane >> 4);
 #pragma unroll
     for (int r = 0; r < RT; ++r) {
       const int64_t row = r0 + r * 16 + (lane & 15);
       const bool rok = row < M;
-      float av[8];
       if (TA) {
         const float* p = Ai + k0 + row * lda;
         if (vec && rok && k0 + 8 <= K) {
@@ -165,33 +169,50 @@ bpass_kernel(const BMat* __restrict__ mats, const BItem* __restrict__ items, int
           const f4s_t y = *reinterpret_cast<const f4s_t*>(p + 4);
 #pragma unroll
           for (int e = 0; e < 4; ++e) {
-            av[e] = x[e];
-            av[4 + e] = y[e];
+            f.av[r][e] = x[e];
+            f.av[r][4 + e] = y[e];
           }
         } else {
 #pragma unroll
-          for (int e = 0; e < 8; ++e) av[e] = (rok && k0 + e < K) ? p[e] : 0.0f;
+          for (int e = 0; e < 8; ++e) f.av[r][e] = (rok && k0 + e < K) ? p[e] : 0.0f;
         }
       } else {
         const float* p = Ai + row + k0 * lda;
 #pragma unroll
-        for (int e = 0; e < 8; ++e) av[e] = (rok && k0 + e < K) ? p[e * lda] : 0.0f;
+        for (int e = 0; e < 8; ++e) f.av[r][e] = (rok && k0 + e < K) ? p[e * lda] : 0.0f;
       }
+    }
+  };
+  auto compute = [&](const Frag& f) {
+#pragma unroll
+    for (int r = 0; r < RT; ++r) {
       bf16x8_t ah, al;
 #pragma unroll
       for (int e = 0; e < 8; ++e) {
-        const __bf16 hb = (__bf16)av[e];
+        const __bf16 hb = (__bf16)f.av[r][e];
         ah[e] = hb;
-        al[e] = (__bf16)(av[e] - (float)hb);
+        al[e] = (__bf16)(f.av[r][e] - (float)hb);
       }
 #pragma unroll
       for (int t = 0; t < kNT; ++t) {
-        acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(sh[t], ah, acc[r][t], 0, 0, 0);
-        acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(sh[t], al, acc[r][t], 0, 0, 0);
-        acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(sl[t], ah, acc[r][t], 0, 0, 0);
+        acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(f.sh[t], ah, acc[r][t], 0, 0, 0);
+        acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(f.sh[t], al, acc[r][t], 0, 0, 0);
+        acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(f.sl[t], ah, acc[r][t], 0, 0, 0);
       }
     }
+  };
+  // two register buffers, the next block's loads issued before the current
+  // block's split and products (unrolled by two so the buffers swap by name)
+  Frag f0, f1;
+  int64_t kb = kb_lo;
+  if (kb < kb_hi) fetch(kb, f0);
+  for (; kb + 1 < kb_hi; kb += 2) {
+    fetch(kb + 1, f1);
+    compute(f0);
+    if (kb + 2 < kb_hi) fetch(kb + 2, f0);
+    compute(f1);
   }
+  if (kb < kb_hi) compute(f0);
   if constexpr (KS) {
     __shared__ f4v_t part[3][RT][kNT][64];
     if (wave > 0) {
