@@ -74,12 +74,29 @@ struct BItem {
 };
 
 // ---------------------------------------------------------------- Omega
+// Column j of Omega_i holds elements j n_i .. j n_i + n_i - 1 of matrix i's
+// stream; a thread takes one Philox block (four consecutive elements) of that
+// range and writes the entries it covers.  Padding columns (j >= l_i) are zero.
 __global__ void bomega_kernel(const BMat* __restrict__ mats, float* __restrict__ work) {
   const BMat d = mats[blockIdx.x];
   const int j = blockIdx.y;
   float* Z = work + d.z + (int64_t)j * d.n;
-  for (int64_t r = threadIdx.x; r < d.n; r += blockDim.x)
-    Z[r] = (j < d.l) ? (float)gaussian_at(d.seed, (uint64_t)((int64_t)j * d.n + r)) : 0.0f;
+  if (j >= d.l) {
+    for (int64_t r = threadIdx.x; r < d.n; r += blockDim.x) Z[r] = 0.0f;
+    return;
+  }
+  const int64_t e0 = (int64_t)j * d.n, qn = d.n / 4 + 2;
+  for (int64_t q = threadIdx.x; q < qn; q += blockDim.x) {
+    const int64_t b = (e0 >> 2) + q;
+    if (4 * b >= e0 + d.n) continue;
+    double g[4];
+    gaussian_block4(d.seed, (uint64_t)b, g);
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+      const int64_t r = 4 * b + w - e0;
+      if (r >= 0 && r < d.n) Z[r] = (float)g[w];
+    }
+  }
 }
 
 // ---------------------------------------------------------------- tiling

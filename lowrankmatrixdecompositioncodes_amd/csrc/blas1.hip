@@ -10,15 +10,29 @@
 
 namespace rsvd {
 
+// A thread takes one Philox block (four consecutive elements of the stream)
+// of a column's element range [e0, e0 + rows), e0 = (col0 + j) rows_total,
+// and writes the entries it covers: one block per four elements instead of
+// one per element.  Grid: x over the qn = rows / 4 + 2 blocks that can touch
+// a column, y over columns.
 template <typename T>
 __global__ void gaussian_kernel(uint64_t seed, int64_t rows, int64_t cols, int64_t col0,
                                 int64_t rows_total, T* __restrict__ out, int64_t ld) {
-  int64_t total = rows * cols;
-  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
-       idx += (int64_t)gridDim.x * blockDim.x) {
-    int64_t i = idx % rows, j = idx / rows;
-    uint64_t e = (uint64_t)((col0 + j) * rows_total + i);
-    out[i + j * ld] = (T)gaussian_at(seed, e);
+  const int64_t qn = rows / 4 + 2;
+  for (int64_t j = blockIdx.y; j < cols; j += gridDim.y) {
+    const int64_t e0 = (col0 + j) * rows_total;
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < qn;
+         q += (int64_t)gridDim.x * blockDim.x) {
+      const int64_t b = (e0 >> 2) + q;
+      if (4 * b >= e0 + rows) continue;
+      double g[4];
+      gaussian_block4(seed, (uint64_t)b, g);
+#pragma unroll
+      for (int w = 0; w < 4; ++w) {
+        const int64_t i = 4 * b + w - e0;
+        if (i >= 0 && i < rows) out[i + j * ld] = (T)g[w];
+      }
+    }
   }
 }
 
@@ -60,7 +74,7 @@ static inline dim3 grid2d(int64_t rows, int64_t cols) {
 int gaussian_f64(Context* c, uint64_t seed, int64_t rows, int64_t cols, int64_t col0,
                  int64_t rows_total, double* out, int64_t ld) {
   if (rows * cols == 0) return 0;
-  hipLaunchKernelGGL(gaussian_kernel<double>, dim3(grid_for(rows * cols)), dim3(256), 0,
+  hipLaunchKernelGGL(gaussian_kernel<double>, grid2d(rows / 4 + 2, cols), dim3(256), 0,
                      c->stream, seed, rows, cols, col0, rows_total, out, ld);
   RSVD_LAUNCH_CHECK();
   return 0;
@@ -68,7 +82,7 @@ int gaussian_f64(Context* c, uint64_t seed, int64_t rows, int64_t cols, int64_t 
 int gaussian_f32(Context* c, uint64_t seed, int64_t rows, int64_t cols, int64_t col0,
                  int64_t rows_total, float* out, int64_t ld) {
   if (rows * cols == 0) return 0;
-  hipLaunchKernelGGL(gaussian_kernel<float>, dim3(grid_for(rows * cols)), dim3(256), 0,
+  hipLaunchKernelGGL(gaussian_kernel<float>, grid2d(rows / 4 + 2, cols), dim3(256), 0,
                      c->stream, seed, rows, cols, col0, rows_total, out, ld);
   RSVD_LAUNCH_CHECK();
   return 0;

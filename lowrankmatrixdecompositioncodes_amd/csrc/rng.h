@@ -102,4 +102,16 @@ static __device__ __forceinline__ double gaussian_at(uint64_t seed, uint64_t e) 
   return (double)(float)dev_ndtri(u);
 }
 
+// Elements 4 b .. 4 b + 3 of the stream from one Philox block: g[w] equals
+// gaussian_at(seed, 4 b + w) bit for bit.
+static __device__ __forceinline__ void gaussian_block4(uint64_t seed, uint64_t b, double (&g)[4]) {
+  uint64_t c[4] = {b, 0, 0, 0};
+  philox4x64(c, seed, 0);
+#pragma unroll
+  for (int w = 0; w < 4; ++w) {
+    const double u = ((double)(c[w] >> 11) + 0.5) * 0x1.0p-53;
+    g[w] = (double)(float)dev_ndtri(u);
+  }
+}
+
 }  // namespace rsvd
